@@ -1,0 +1,207 @@
+/*
+ * lqcs_device.h -- thin C-ABI over the MI355X (gfx950) statevector engine.
+ *
+ * The C++20 host library (include/lqcs/...) and any FFI (ctypes, cgo, JNI)
+ * reach the HIP kernels only through these entry points. Plain pointers and
+ * sizes; no HIP or torch types. Every call returns an lqcs_status; the text
+ * of the last error on the calling thread is lqcs_dev_last_error().
+ *
+ * State layout in HBM: 2^n complex128 amplitudes, interleaved (re, im), index
+ * bit q = qubit q (little endian), 256-byte aligned -- the device image of
+ * the reference's aligned_vector<complex_t> (include/lqcs/core/types.hpp:11-16).
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * LiteQuantumComputingSimulator).
+ */
+#ifndef LQCS_DEVICE_H
+#define LQCS_DEVICE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int lqcs_status;
+enum {
+    LQCS_OK = 0,
+    LQCS_ERR_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference */
+    LQCS_ERR_OUT_OF_RANGE = 2,     /* std::out_of_range */
+    LQCS_ERR_LOGIC = 3,            /* std::logic_error */
+    LQCS_ERR_OUT_OF_MEMORY = 4,    /* device allocation failed */
+    LQCS_ERR_DEVICE = 5,           /* HIP runtime error */
+    LQCS_ERR_NO_DEVICE = 6         /* no usable gfx950 device */
+};
+
+/* Largest register the device engine accepts (2^34 amplitudes = 256 GiB). */
+#define LQCS_DEVICE_MAX_QUBITS 34
+
+typedef struct lqcs_dev_state lqcs_dev_state;
+
+/* ---- context ------------------------------------------------------------ */
+const char* lqcs_dev_last_error(void);
+/* Selects the device and creates the engine stream. Idempotent. */
+lqcs_status lqcs_dev_init(int device);
+/* Device name / CU count / HBM bytes of the active device. */
+lqcs_status lqcs_dev_info(char* name, size_t name_len, int* cus, uint64_t* hbm_bytes);
+lqcs_status lqcs_dev_synchronize(void);
+
+/* ---- state: replaces Statevector (include/lqcs/backend/statevector.hpp:12-37)
+ * lqcs_state_create = Statevector(n) (src/backend/statevector.cpp:11-25). */
+lqcs_status lqcs_state_create(uint32_t n_qubits, lqcs_dev_state** out);
+lqcs_status lqcs_state_destroy(lqcs_dev_state* s);
+lqcs_status lqcs_state_num_qubits(const lqcs_dev_state* s, uint32_t* n_qubits);
+/* Reinitialise to the basis state |index>. */
+lqcs_status lqcs_state_set_basis(lqcs_dev_state* s, uint64_t index);
+/* Host <-> device copies of the whole register, (re, im) interleaved. */
+lqcs_status lqcs_state_upload(lqcs_dev_state* s, const double* amps);
+lqcs_status lqcs_state_download(const lqcs_dev_state* s, double* amps);
+lqcs_status lqcs_state_copy(lqcs_dev_state* dst, const lqcs_dev_state* src);
+/* Raw device address of the amplitudes (for zero-copy interop). */
+lqcs_status lqcs_state_device_ptr(lqcs_dev_state* s, void** ptr);
+
+/* ---- operation stream: replaces the kernel seam lqcs::kernels::*
+ * (src/backend/kernels/kernels.hpp:13-62), fed by the dispatcher
+ * apply_instruction (src/backend/statevector_simulator.cpp:22-85). */
+enum {
+    LQCS_OP_SINGLE = 0,           /* apply_single_qubit: targets[0], 2x2 */
+    LQCS_OP_DIAGONAL_SINGLE = 1,  /* apply_diagonal_single_qubit: targets[0], d0,d1 */
+    LQCS_OP_CONTROLLED_SINGLE = 2,/* apply_controlled_single_qubit: 1 control, 2x2 */
+    LQCS_OP_SWAP = 3,             /* apply_swap: targets[0], targets[1] */
+    LQCS_OP_TWO_QUBIT = 4,        /* apply_two_qubit: 4x4, local bit0 <-> targets[0] */
+    LQCS_OP_CONTROLLED_UNITARY = 5,    /* apply_controlled_unitary: c controls, k targets */
+    LQCS_OP_CONTROLLED_PERMUTATION = 6,/* apply_controlled_permutation */
+    LQCS_OP_PAULI_ROTATION = 7    /* apply_pauli_rotation: x_mask, zy_mask, n_y, theta */
+};
+
+typedef struct lqcs_op {
+    uint32_t kind;
+    uint32_t n_controls;
+    uint32_t n_targets;
+    uint32_t qubit_offset;  /* into lqcs_program.qubits: controls, then targets */
+    uint64_t data_offset;   /* complex entries (matrix / d0,d1) or perm entries */
+    uint64_t x_mask;        /* PAULI_ROTATION: global X/Y bits */
+    uint64_t zy_mask;       /* PAULI_ROTATION: global Z/Y bits */
+    uint32_t n_y;
+    uint32_t reserved;
+    double theta;
+} lqcs_op;
+
+typedef struct lqcs_program {
+    const lqcs_op* ops;
+    uint64_t n_ops;
+    const uint32_t* qubits;
+    uint64_t n_qubits;
+    const double* cplx;     /* (re, im) pairs */
+    uint64_t n_cplx;
+    const uint64_t* perm;
+    uint64_t n_perm;
+} lqcs_program;
+
+/* Execution plan choice for lqcs_state_apply. */
+enum {
+    LQCS_PLAN_GATE_BY_GATE = 0,  /* one reference-seam kernel per op */
+    LQCS_PLAN_GREEDY = 1,        /* LDS-tiled passes, greedy grouping */
+    LQCS_PLAN_LEARNED = 2,       /* LDS-tiled passes, cost-model guided */
+    LQCS_PLAN_AUTO = 3           /* engine default */
+};
+
+typedef struct lqcs_exec_options {
+    int32_t plan;         /* LQCS_PLAN_* */
+    int32_t tile_qubits;  /* tile width L (0 = engine default) */
+    int32_t low_qubits;   /* contiguous low qubits kept in every tile (0 = default) */
+    int32_t reserved;
+} lqcs_exec_options;
+
+/* Applies the whole op stream in order, in place. opts may be NULL. */
+lqcs_status lqcs_state_apply(lqcs_dev_state* s, const lqcs_program* prog,
+                             const lqcs_exec_options* opts);
+/* Plan summary of the last lqcs_state_apply on this thread: HBM passes and
+ * kernel launches it issued. */
+lqcs_status lqcs_last_plan_stats(uint64_t* passes, uint64_t* launches);
+
+/* ---- reductions and measurement ---------------------------------------- */
+/* Statevector::norm (statevector.cpp:27-31) squared, before the sqrt. */
+lqcs_status lqcs_state_norm_sq(const lqcs_dev_state* s, double* out);
+/* Statevector::probabilities (statevector.cpp:33-37) into host memory. */
+lqcs_status lqcs_state_probabilities(const lqcs_dev_state* s, double* out);
+/* Statevector::expectation_value (statevector.cpp:39-78) with the string
+ * already turned into global masks (leftmost char = qubit n-1). */
+lqcs_status lqcs_state_expectation_pauli(const lqcs_dev_state* s, uint64_t x_mask,
+                                         uint64_t zy_mask, uint32_t n_y, double* out);
+/* algorithms::expectation (src/algorithms/vqe.cpp:11-17): sum of coeff_t *
+ * <P_t> over n_terms Pauli terms in one call. */
+lqcs_status lqcs_state_expectation_terms(const lqcs_dev_state* s, const uint64_t* x_masks,
+                                         const uint64_t* zy_masks, const uint32_t* n_ys,
+                                         const double* coeffs, uint64_t n_terms, double* out);
+/* run_sampling (statevector_simulator.cpp:88-119): for each uniform draw u,
+ * the index upper_bound(cumulative, u * total). */
+lqcs_status lqcs_state_sample(const lqcs_dev_state* s, const double* draws, uint64_t n_draws,
+                              uint64_t* out_indices);
+/* collapse_qubit / reset_qubit (gate_kernels.cpp:227-260). */
+lqcs_status lqcs_state_collapse(lqcs_dev_state* s, uint32_t qubit, double u, int* outcome);
+lqcs_status lqcs_state_reset_qubit(lqcs_dev_state* s, uint32_t qubit, double u);
+/* kraus_probability (gate_kernels.cpp:262-277); k = 4 complex (8 doubles). */
+lqcs_status lqcs_state_kraus_probability(const lqcs_dev_state* s, uint32_t qubit,
+                                         const double* k, double* out);
+
+/* ---- per-shot trajectories: replaces run_per_shot + sample_kraus_branch +
+ * apply_trajectory_noise (statevector_simulator.cpp:122-190). The op stream
+ * carries one unitary op per circuit instruction plus marker ops below; draws
+ * hold n_shots * draws_per_shot uniforms in the reference's consumption order
+ * (shot s uses [s*D, (s+1)*D)). Several shots run as one batch on device. */
+enum {
+    LQCS_TOP_UNITARY = 0,  /* apply ops[op_index]; then noise on its qubits */
+    LQCS_TOP_MEASURE = 1,  /* collapse qubit, write clbit */
+    LQCS_TOP_RESET = 2     /* reset qubit */
+};
+typedef struct lqcs_traj_step {
+    uint32_t kind;
+    uint32_t qubit;        /* MEASURE / RESET */
+    uint32_t clbit;        /* MEASURE */
+    uint32_t op_index;     /* UNITARY: index into prog->ops */
+    uint32_t noise_qubit_offset;  /* UNITARY: qubits the channels act on */
+    uint32_t n_noise_qubits;
+} lqcs_traj_step;
+
+typedef struct lqcs_channel {
+    uint32_t n_ops;
+    const double* ops;     /* n_ops * 4 complex, row-major 2x2, (re,im) */
+} lqcs_channel;
+
+lqcs_status lqcs_trajectories_run(uint32_t n_qubits, uint32_t n_clbits,
+                                  const lqcs_program* prog, const lqcs_traj_step* steps,
+                                  uint64_t n_steps, const uint32_t* noise_qubits,
+                                  const lqcs_channel* channels, uint32_t n_channels,
+                                  const double* draws, uint64_t draws_per_shot,
+                                  uint64_t n_shots, uint32_t max_batch, uint8_t* clbits_out);
+
+/* ---- adjoint differentiation of <H> for Pauli-rotation parameters
+ * (replaces algorithms::parameter_shift_gradient, src/algorithms/vqe.cpp:68-94,
+ * for circuits whose parameters are RX/RY/RZ/PauliRotation angles).
+ * prog holds the bound circuit; param_op[p] is the op index of parameter p,
+ * whose generator is its PAULI_ROTATION masks (RX/RY/RZ lowered to 1-qubit
+ * Pauli rotations by the caller). Writes <H> to energy and dE/dtheta_p. */
+lqcs_status lqcs_adjoint_gradient(uint32_t n_qubits, const lqcs_program* prog,
+                                  const uint64_t* param_op, uint64_t n_params,
+                                  const uint64_t* x_masks, const uint64_t* zy_masks,
+                                  const uint32_t* n_ys, const double* coeffs,
+                                  uint64_t n_terms, double* energy, double* grad);
+
+/* Batched adjoint over many circuits of possibly different widths in shared
+ * launches (one launch per op step across the batch). Circuit c uses
+ * progs[c], n_qubits[c], params param_op[param_off[c] .. param_off[c+1]) and
+ * terms [term_off[c], term_off[c+1]); grads are written at param_off[c]. */
+lqcs_status lqcs_adjoint_gradient_batched(uint32_t n_circuits, const uint32_t* n_qubits,
+                                          const lqcs_program* progs,
+                                          const uint64_t* param_op, const uint64_t* param_off,
+                                          const uint64_t* x_masks, const uint64_t* zy_masks,
+                                          const uint32_t* n_ys, const double* coeffs,
+                                          const uint64_t* term_off, double* energies,
+                                          double* grads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

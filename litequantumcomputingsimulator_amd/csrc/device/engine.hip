@@ -1,0 +1,804 @@
+// Engine context, the pass planner/executor and the C-ABI of lqcs_device.h.
+//
+// The planner turns the op stream (the reference kernel seam) into HBM
+// passes. Greedy grouping walks the pending ops in order with a growing tile
+// qubit set S: an op joins the pass when every op before it that shares a
+// qubit has joined too, and its non-diagonal support fits in S (|S| <= L).
+// Ops that do not fit block their qubits for the rest of the walk, so every
+// op taken after a deferred one acts on disjoint qubits and commutes with it.
+#include "common.hpp"
+#include "engine.hpp"
+#include "reduce.hpp"
+#include "tile.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../../include/lqcs_device.h"
+
+struct lqcs_dev_state {
+    lqcs_dev::amp_t* d = nullptr;
+    uint32_t nq = 0;
+};
+
+namespace lqcs_dev {
+
+// ---- context ------------------------------------------------------------------
+namespace {
+
+struct Context {
+    bool ready = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 0;
+    std::vector<std::pair<char*, size_t>> blocks;
+    size_t used = 0;
+};
+
+Context& ctx() {
+    static Context c;
+    return c;
+}
+
+thread_local std::string g_last_error;
+thread_local uint64_t g_last_passes = 0, g_last_launches = 0;
+
+std::mutex& api_mutex() {
+    static std::mutex m;
+    return m;
+}
+
+}  // namespace
+
+void check_hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return;
+    const int code = (e == hipErrorOutOfMemory) ? LQCS_ERR_OUT_OF_MEMORY : LQCS_ERR_DEVICE;
+    throw Error(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+hipStream_t stream() { return ctx().stream; }
+int num_cus() { return ctx().cus; }
+
+void init_device(int device) {
+    Context& c = ctx();
+    if (c.ready) {
+        if (device != c.device) throw Error(LQCS_ERR_INVALID_ARGUMENT, "engine already bound to another device");
+        return;
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        throw Error(LQCS_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= count) throw Error(LQCS_ERR_INVALID_ARGUMENT, "device index out of range");
+    LQCS_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    LQCS_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0)
+        throw Error(LQCS_ERR_NO_DEVICE, std::string("engine is built for gfx950, found ") + prop.gcnArchName);
+    LQCS_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    c.cus = prop.multiProcessorCount;
+    c.device = device;
+    c.ready = true;
+}
+
+void ensure_ready() {
+    if (!ctx().ready) init_device(0);
+    LQCS_HIP(hipSetDevice(ctx().device));
+}
+
+void* arena(size_t bytes) {
+    Context& c = ctx();
+    bytes = (bytes + 255) & ~size_t{255};
+    if (c.blocks.empty() || c.used + bytes > c.blocks.back().second) {
+        size_t sz = std::max<size_t>(bytes, c.blocks.empty() ? (size_t{64} << 20) : c.blocks.back().second * 2);
+        char* p = nullptr;
+        LQCS_HIP(hipMalloc(&p, sz));
+        c.blocks.emplace_back(p, sz);
+        c.used = 0;
+    }
+    void* p = c.blocks.back().first + c.used;
+    c.used += bytes;
+    return p;
+}
+
+void arena_reset() {
+    Context& c = ctx();
+    if (c.blocks.size() > 1) {  // fold into one block big enough for next time
+        LQCS_HIP(hipStreamSynchronize(c.stream));
+        size_t total = 0;
+        for (auto& b : c.blocks) {
+            total += b.second;
+            LQCS_HIP(hipFree(b.first));
+        }
+        c.blocks.clear();
+        char* p = nullptr;
+        LQCS_HIP(hipMalloc(&p, total));
+        c.blocks.emplace_back(p, total);
+    }
+    c.used = 0;
+}
+
+void sync() { LQCS_HIP(hipStreamSynchronize(stream())); }
+
+// ---- op decoding ------------------------------------------------------------
+namespace {
+
+amp_t cplx_at(const lqcs_program& p, uint64_t i) {
+    if (i >= p.n_cplx) throw Error(LQCS_ERR_INVALID_ARGUMENT, "op data offset beyond complex pool");
+    return make_double2(p.cplx[2 * i], p.cplx[2 * i + 1]);
+}
+
+bool is_zero(amp_t a) { return a.x == 0.0 && a.y == 0.0; }
+
+}  // namespace
+
+DecodedOp decode(const lqcs_program& p, uint64_t k, uint32_t nq) {
+    const lqcs_op& o = p.ops[k];
+    DecodedOp d;
+    d.kind = o.kind;
+    if (uint64_t(o.qubit_offset) + o.n_controls + o.n_targets > p.n_qubits)
+        throw Error(LQCS_ERR_INVALID_ARGUMENT, "op qubits beyond qubit pool");
+    d.controls.assign(p.qubits + o.qubit_offset, p.qubits + o.qubit_offset + o.n_controls);
+    d.targets.assign(p.qubits + o.qubit_offset + o.n_controls,
+                     p.qubits + o.qubit_offset + o.n_controls + o.n_targets);
+    uint64_t seen = 0;
+    for (uint32_t q : d.controls) {
+        if (q >= nq) throw Error(LQCS_ERR_OUT_OF_RANGE, "control qubit out of range");
+        if (seen & (uint64_t{1} << q)) throw Error(LQCS_ERR_INVALID_ARGUMENT, "duplicate qubit");
+        seen |= uint64_t{1} << q;
+        d.cmask |= uint64_t{1} << q;
+    }
+    for (uint32_t q : d.targets) {
+        if (q >= nq) throw Error(LQCS_ERR_OUT_OF_RANGE, "target qubit out of range");
+        if (seen & (uint64_t{1} << q)) throw Error(LQCS_ERR_INVALID_ARGUMENT, "duplicate qubit");
+        seen |= uint64_t{1} << q;
+    }
+    d.touched = seen;
+    const uint64_t full = (nq >= 64) ? ~uint64_t{0} : ((uint64_t{1} << nq) - 1);
+    auto need_targets = [&](uint32_t n) {
+        if (d.targets.size() != n) throw Error(LQCS_ERR_INVALID_ARGUMENT, "wrong target count for op kind");
+    };
+    switch (o.kind) {
+        case LQCS_OP_SINGLE:
+        case LQCS_OP_CONTROLLED_SINGLE:
+            need_targets(1);
+            if (o.kind == LQCS_OP_CONTROLLED_SINGLE && d.controls.size() != 1)
+                throw Error(LQCS_ERR_INVALID_ARGUMENT, "controlled single op needs one control");
+            for (int i = 0; i < 4; ++i) d.m.push_back(cplx_at(p, o.data_offset + i));
+            break;
+        case LQCS_OP_DIAGONAL_SINGLE:
+            need_targets(1);
+            for (int i = 0; i < 2; ++i) d.m.push_back(cplx_at(p, o.data_offset + i));
+            break;
+        case LQCS_OP_SWAP:
+            need_targets(2);
+            break;
+        case LQCS_OP_TWO_QUBIT:
+            need_targets(2);
+            for (int i = 0; i < 16; ++i) d.m.push_back(cplx_at(p, o.data_offset + i));
+            break;
+        case LQCS_OP_CONTROLLED_UNITARY: {
+            if (d.targets.empty()) throw Error(LQCS_ERR_INVALID_ARGUMENT, "unitary op without targets");
+            if (d.targets.size() > 12) throw Error(LQCS_ERR_LOGIC, "dense op wider than 12 qubits");
+            const uint64_t D = uint64_t{1} << d.targets.size();
+            for (uint64_t i = 0; i < D * D; ++i) d.m.push_back(cplx_at(p, o.data_offset + i));
+            break;
+        }
+        case LQCS_OP_CONTROLLED_PERMUTATION: {
+            if (d.targets.empty()) throw Error(LQCS_ERR_INVALID_ARGUMENT, "permutation op without targets");
+            if (d.targets.size() > 12) throw Error(LQCS_ERR_LOGIC, "permutation wider than 12 qubits");
+            const uint64_t D = uint64_t{1} << d.targets.size();
+            if (o.data_offset + D > p.n_perm) throw Error(LQCS_ERR_INVALID_ARGUMENT, "perm beyond pool");
+            std::vector<uint8_t> hit(D, 0);
+            for (uint64_t i = 0; i < D; ++i) {
+                const uint64_t v = p.perm[o.data_offset + i];
+                if (v >= D || hit[v]) throw Error(LQCS_ERR_INVALID_ARGUMENT, "permutation is not a bijection");
+                hit[v] = 1;
+                d.perm.push_back(v);
+            }
+            break;
+        }
+        case LQCS_OP_PAULI_ROTATION:
+            if (!d.controls.empty() || !d.targets.empty())
+                throw Error(LQCS_ERR_INVALID_ARGUMENT, "pauli rotation takes masks, not qubits");
+            if ((o.x_mask | o.zy_mask) & ~full) throw Error(LQCS_ERR_OUT_OF_RANGE, "pauli mask out of range");
+            d.x = o.x_mask;
+            d.zy = o.zy_mask;
+            d.n_y = o.n_y;
+            d.theta = o.theta;
+            d.touched = o.x_mask | o.zy_mask;
+            break;
+        default:
+            throw Error(LQCS_ERR_INVALID_ARGUMENT, "unknown op kind");
+    }
+    // Diagonal recognition (exact zeros off the diagonal).
+    if (o.kind == LQCS_OP_DIAGONAL_SINGLE) d.diagonal = true;
+    if ((o.kind == LQCS_OP_SINGLE || o.kind == LQCS_OP_CONTROLLED_SINGLE) && is_zero(d.m[1]) && is_zero(d.m[2]))
+        d.diagonal = true;
+    if (o.kind == LQCS_OP_TWO_QUBIT || (o.kind == LQCS_OP_CONTROLLED_UNITARY && d.targets.size() <= 2)) {
+        const uint64_t D = uint64_t{1} << d.targets.size();
+        bool diag = true;
+        for (uint64_t r = 0; r < D && diag; ++r)
+            for (uint64_t c = 0; c < D; ++c)
+                if (r != c && !is_zero(d.m[r * D + c])) { diag = false; break; }
+        d.diagonal = diag;
+    }
+    if (o.kind == LQCS_OP_PAULI_ROTATION && d.x == 0) d.diagonal = true;
+    return d;
+}
+
+// ---- gate-by-gate execution -------------------------------------------------
+void run_single_op(amp_t* s, uint32_t nq, const DecodedOp& d, uint32_t batch) {
+    const uint32_t* t = d.targets.data();
+    switch (d.kind) {
+        case LQCS_OP_SINGLE: {
+            Mat2 m{{d.m[0], d.m[1], d.m[2], d.m[3]}};
+            launch_single(s, nq, t[0], m, batch);
+            return;
+        }
+        case LQCS_OP_DIAGONAL_SINGLE:
+            launch_diag_single(s, nq, t[0], d.m[0], d.m[1], batch);
+            return;
+        case LQCS_OP_CONTROLLED_SINGLE: {
+            Mat2 m{{d.m[0], d.m[1], d.m[2], d.m[3]}};
+            launch_controlled_single(s, nq, d.controls[0], t[0], m, batch);
+            return;
+        }
+        case LQCS_OP_SWAP:
+            launch_swap(s, nq, t[0], t[1], batch);
+            return;
+        case LQCS_OP_TWO_QUBIT: {
+            Mat4 m;
+            for (int i = 0; i < 16; ++i) m.m[i] = d.m[i];
+            launch_two_qubit(s, nq, t[0], t[1], m, batch);
+            return;
+        }
+        case LQCS_OP_CONTROLLED_UNITARY:
+            launch_controlled_unitary(s, nq, d.controls.data(), d.controls.size(), t,
+                                      d.targets.size(), d.m.data(), batch);
+            return;
+        case LQCS_OP_CONTROLLED_PERMUTATION:
+            launch_controlled_permutation(s, nq, d.controls.data(), d.controls.size(), t,
+                                          d.targets.size(), d.perm.data(), batch);
+            return;
+        case LQCS_OP_PAULI_ROTATION:
+            launch_pauli_rotation(s, nq, d.x, d.zy, d.n_y, d.theta, batch);
+            return;
+        default:
+            throw Error(LQCS_ERR_INVALID_ARGUMENT, "unknown op kind");
+    }
+}
+
+// ---- tiled planning ------------------------------------------------------------
+namespace {
+
+// Qubits whose values must vary inside the tile for this op; ~0 when the op
+// cannot run inside a tile at all.
+uint64_t tile_requirement(const DecodedOp& d) {
+    if (d.diagonal && (d.kind != LQCS_OP_CONTROLLED_UNITARY || d.targets.size() <= 2)) return 0;
+    switch (d.kind) {
+        case LQCS_OP_SINGLE:
+        case LQCS_OP_CONTROLLED_SINGLE:
+            return uint64_t{1} << d.targets[0];
+        case LQCS_OP_SWAP:
+        case LQCS_OP_TWO_QUBIT:
+            return (uint64_t{1} << d.targets[0]) | (uint64_t{1} << d.targets[1]);
+        case LQCS_OP_CONTROLLED_UNITARY:
+        case LQCS_OP_CONTROLLED_PERMUTATION:
+            if (d.targets.size() > 2) return ~uint64_t{0};
+            return d.targets.size() == 1 ? (uint64_t{1} << d.targets[0])
+                                         : ((uint64_t{1} << d.targets[0]) | (uint64_t{1} << d.targets[1]));
+        case LQCS_OP_PAULI_ROTATION:
+            return d.x;
+        default:
+            return ~uint64_t{0};
+    }
+}
+
+uint32_t local_pos(uint64_t smask, uint32_t q) {
+    return static_cast<uint32_t>(__builtin_popcountll(smask & ((uint64_t{1} << q) - 1)));
+}
+
+uint32_t to_local_mask(uint64_t smask, uint64_t global) {
+    uint32_t m = 0;
+    for (uint64_t g = global & smask; g; g &= g - 1) m |= 1u << local_pos(smask, __builtin_ctzll(g));
+    return m;
+}
+
+TileOp make_tile_op(const DecodedOp& d, uint64_t smask) {
+    TileOp t;
+    std::memset(&t, 0, sizeof t);
+    t.lctrl = to_local_mask(smask, d.cmask);
+    t.gctrl = d.cmask & ~smask;
+    auto set_diag_qubit = [&](int slot, uint32_t q) {
+        const uint64_t bit = uint64_t{1} << q;
+        if (smask & bit) (slot ? t.dl1 : t.dl0) = 1u << local_pos(smask, q);
+        else (slot ? t.dg1 : t.dg0) = bit;
+    };
+    if (d.kind == LQCS_OP_PAULI_ROTATION) {
+        t.kind = TK_PAULI;
+        t.xl = to_local_mask(smask, d.x);
+        t.zl = to_local_mask(smask, d.zy);
+        t.zg = d.zy & ~smask;
+        const double h = d.theta / 2;
+        static const amp_t kIPow[4] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
+        const amp_t ip = kIPow[d.n_y % 4];
+        const double ms = -std::sin(h);
+        t.m[0] = make_double2(std::cos(h), 0.0);
+        t.m[1] = make_double2(-ms * ip.y, ms * ip.x);  // (0, -sin) * i^ny
+        return t;
+    }
+    if (d.diagonal) {
+        t.kind = TK_DIAG;
+        if (d.kind == LQCS_OP_DIAGONAL_SINGLE) {
+            set_diag_qubit(0, d.targets[0]);
+            t.m[0] = d.m[0];
+            t.m[1] = d.m[1];
+        } else if (d.targets.size() == 1) {
+            set_diag_qubit(0, d.targets[0]);
+            t.m[0] = d.m[0];
+            t.m[1] = d.m[3];
+        } else {
+            set_diag_qubit(0, d.targets[0]);
+            set_diag_qubit(1, d.targets[1]);
+            for (int i = 0; i < 4; ++i) t.m[i] = d.m[i * 4 + i];
+        }
+        return t;
+    }
+    if (d.targets.size() == 1) {
+        t.kind = TK_DENSE1;
+        t.t0 = local_pos(smask, d.targets[0]);
+        if (d.kind == LQCS_OP_CONTROLLED_PERMUTATION) {
+            for (int c = 0; c < 2; ++c) t.m[d.perm[c] * 2 + c] = make_double2(1.0, 0.0);
+        } else {
+            for (int i = 0; i < 4; ++i) t.m[i] = d.m[i];
+        }
+        return t;
+    }
+    t.kind = TK_DENSE2;
+    t.t0 = local_pos(smask, d.targets[0]);
+    t.t1 = local_pos(smask, d.targets[1]);
+    if (d.kind == LQCS_OP_SWAP) {
+        t.m[0] = t.m[6] = t.m[9] = t.m[15] = make_double2(1.0, 0.0);
+    } else if (d.kind == LQCS_OP_CONTROLLED_PERMUTATION) {
+        for (int c = 0; c < 4; ++c) t.m[d.perm[c] * 4 + c] = make_double2(1.0, 0.0);
+    } else {
+        for (int i = 0; i < 16; ++i) t.m[i] = d.m[i];
+    }
+    return t;
+}
+
+}  // namespace
+
+Plan plan_greedy(const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t low) {
+    Plan plan;
+    std::vector<uint32_t> pending(ops.size());
+    for (uint32_t i = 0; i < ops.size(); ++i) pending[i] = i;
+    const uint64_t low_mask = (uint64_t{1} << low) - 1;
+    while (!pending.empty()) {
+        const DecodedOp& head = ops[pending[0]];
+        if (tile_requirement(head) == ~uint64_t{0}) {
+            PlanStep st;
+            st.tiled = false;
+            st.op = pending[0];
+            plan.steps.push_back(st);
+            pending.erase(pending.begin());
+            continue;
+        }
+        uint64_t S = low_mask, blocked = 0;
+        std::vector<uint32_t> taken, rest;
+        for (uint32_t idx : pending) {
+            const DecodedOp& d = ops[idx];
+            const uint64_t R = tile_requirement(d);
+            if ((d.touched & blocked) || R == ~uint64_t{0}) {
+                blocked |= d.touched;
+                rest.push_back(idx);
+                continue;
+            }
+            const uint64_t grown = S | R;
+            if (static_cast<uint32_t>(__builtin_popcountll(grown)) <= L) {
+                S = grown;
+                taken.push_back(idx);
+            } else {
+                blocked |= d.touched;
+                rest.push_back(idx);
+            }
+        }
+        // Pad S to exactly L qubits with the lowest unused ones.
+        for (uint32_t q = 0; q < nq && static_cast<uint32_t>(__builtin_popcountll(S)) < L; ++q)
+            S |= uint64_t{1} << q;
+        PlanStep st;
+        st.tiled = true;
+        st.smask = S;
+        for (uint32_t idx : taken) st.tile_ops.push_back(make_tile_op(ops[idx], S));
+        plan.steps.push_back(std::move(st));
+        pending.swap(rest);
+    }
+    return plan;
+}
+
+void execute_plan(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan,
+                  const std::vector<DecodedOp>& ops, uint32_t batch) {
+    size_t total = 0;
+    for (const auto& st : plan.steps) total += st.tile_ops.size();
+    std::vector<TileOp> flat;
+    flat.reserve(total);
+    std::vector<size_t> first;
+    for (const auto& st : plan.steps) {
+        first.push_back(flat.size());
+        flat.insert(flat.end(), st.tile_ops.begin(), st.tile_ops.end());
+    }
+    const TileOp* d_ops = flat.empty() ? nullptr : stage(flat.data(), flat.size());
+    for (size_t i = 0; i < plan.steps.size(); ++i) {
+        const PlanStep& st = plan.steps[i];
+        if (st.tiled) {
+            launch_tile_pass(s, nq, L, st.smask, d_ops + first[i],
+                             static_cast<uint32_t>(st.tile_ops.size()), batch);
+        } else {
+            run_single_op(s, nq, ops[st.op], batch);
+        }
+    }
+    g_last_passes = plan.steps.size();
+    g_last_launches = plan.steps.size();
+}
+
+TileChoice default_tile(uint32_t nq) {
+    TileChoice c;
+    if (nq < 6) return c;  // tiles unused: gate-by-gate
+    c.L = std::min<uint32_t>(12, nq <= 12 ? nq : std::max<uint32_t>(6, nq - 4));
+    c.low = std::min<uint32_t>(4, c.L - 2);
+    return c;
+}
+
+void apply_ops(amp_t* s, uint32_t nq, const std::vector<DecodedOp>& ops, int plan_kind,
+               uint32_t L, uint32_t low, uint32_t batch) {
+    if (plan_kind == LQCS_PLAN_AUTO) plan_kind = LQCS_PLAN_GREEDY;
+    const TileChoice def = default_tile(nq);
+    if (L == 0) L = def.L;
+    if (low == 0) low = def.low;
+    if (plan_kind == LQCS_PLAN_GATE_BY_GATE || L == 0) {
+        for (const auto& d : ops) run_single_op(s, nq, d, batch);
+        g_last_passes = ops.size();
+        g_last_launches = ops.size();
+        return;
+    }
+    if (L < 6 || L > 12 || L > nq) throw Error(LQCS_ERR_INVALID_ARGUMENT, "tile width must be 6..12 and <= n");
+    if (low + 2 > L) throw Error(LQCS_ERR_INVALID_ARGUMENT, "low qubits must leave two free tile slots");
+    const Plan plan = plan_greedy(ops, nq, L, low);
+    execute_plan(s, nq, L, plan, ops, batch);
+}
+
+void set_last_stats(uint64_t passes, uint64_t launches) {
+    g_last_passes = passes;
+    g_last_launches = launches;
+}
+
+}  // namespace lqcs_dev
+
+// ================================================================================
+// C-ABI
+// ================================================================================
+using namespace lqcs_dev;
+
+namespace {
+
+template <class F>
+lqcs_status guarded(F&& f) {
+    std::lock_guard<std::mutex> lock(api_mutex());
+    try {
+        f();
+        return LQCS_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return LQCS_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return LQCS_ERR_LOGIC;
+    }
+}
+
+void need(const void* p, const char* what) {
+    if (!p) throw Error(LQCS_ERR_INVALID_ARGUMENT, std::string(what) + " is null");
+}
+
+uint64_t amps_of(uint32_t nq) { return uint64_t{1} << nq; }
+
+}  // namespace
+
+extern "C" {
+
+const char* lqcs_dev_last_error(void) { return g_last_error.c_str(); }
+
+lqcs_status lqcs_dev_init(int device) {
+    return guarded([&] { init_device(device); });
+}
+
+lqcs_status lqcs_dev_info(char* name, size_t name_len, int* cus, uint64_t* hbm_bytes) {
+    return guarded([&] {
+        ensure_ready();
+        hipDeviceProp_t prop;
+        LQCS_HIP(hipGetDeviceProperties(&prop, ctx().device));
+        if (name && name_len) {
+            std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+        }
+        if (cus) *cus = prop.multiProcessorCount;
+        if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
+    });
+}
+
+lqcs_status lqcs_dev_synchronize(void) {
+    return guarded([&] {
+        ensure_ready();
+        sync();
+    });
+}
+
+lqcs_status lqcs_state_create(uint32_t n_qubits, lqcs_dev_state** out) {
+    return guarded([&] {
+        need(out, "out");
+        *out = nullptr;
+        if (n_qubits == 0) throw Error(LQCS_ERR_INVALID_ARGUMENT, "Statevector: num_qubits must be >= 1");
+        if (n_qubits > LQCS_DEVICE_MAX_QUBITS)
+            throw Error(LQCS_ERR_INVALID_ARGUMENT, "Statevector: " + std::to_string(n_qubits) +
+                                                       " qubits exceed the device engine limit");
+        ensure_ready();
+        arena_reset();
+        auto* st = new lqcs_dev_state;
+        st->nq = n_qubits;
+        const hipError_t e = hipMalloc(&st->d, amps_of(n_qubits) * sizeof(amp_t));
+        if (e != hipSuccess) {
+            delete st;
+            throw Error(LQCS_ERR_OUT_OF_MEMORY, "Statevector: device allocation of " +
+                                                    std::to_string(amps_of(n_qubits) * 16) + " bytes failed");
+        }
+        launch_fill_basis(st->d, n_qubits, 0);
+        *out = st;
+    });
+}
+
+lqcs_status lqcs_state_destroy(lqcs_dev_state* s) {
+    return guarded([&] {
+        if (!s) return;
+        sync();
+        LQCS_HIP(hipFree(s->d));
+        delete s;
+    });
+}
+
+lqcs_status lqcs_state_num_qubits(const lqcs_dev_state* s, uint32_t* n_qubits) {
+    return guarded([&] {
+        need(s, "state");
+        need(n_qubits, "n_qubits");
+        *n_qubits = s->nq;
+    });
+}
+
+lqcs_status lqcs_state_set_basis(lqcs_dev_state* s, uint64_t index) {
+    return guarded([&] {
+        need(s, "state");
+        if (index >= amps_of(s->nq)) throw Error(LQCS_ERR_OUT_OF_RANGE, "basis index out of range");
+        ensure_ready();
+        launch_fill_basis(s->d, s->nq, index);
+    });
+}
+
+lqcs_status lqcs_state_upload(lqcs_dev_state* s, const double* amps) {
+    return guarded([&] {
+        need(s, "state");
+        need(amps, "amps");
+        ensure_ready();
+        LQCS_HIP(hipMemcpyAsync(s->d, amps, amps_of(s->nq) * sizeof(amp_t), hipMemcpyHostToDevice, stream()));
+        sync();
+    });
+}
+
+lqcs_status lqcs_state_download(const lqcs_dev_state* s, double* amps) {
+    return guarded([&] {
+        need(s, "state");
+        need(amps, "amps");
+        ensure_ready();
+        LQCS_HIP(hipMemcpyAsync(amps, s->d, amps_of(s->nq) * sizeof(amp_t), hipMemcpyDeviceToHost, stream()));
+        sync();
+    });
+}
+
+lqcs_status lqcs_state_copy(lqcs_dev_state* dst, const lqcs_dev_state* src) {
+    return guarded([&] {
+        need(dst, "dst");
+        need(src, "src");
+        if (dst->nq != src->nq) throw Error(LQCS_ERR_INVALID_ARGUMENT, "state widths differ");
+        ensure_ready();
+        LQCS_HIP(hipMemcpyAsync(dst->d, src->d, amps_of(src->nq) * sizeof(amp_t), hipMemcpyDeviceToDevice,
+                                stream()));
+    });
+}
+
+lqcs_status lqcs_state_device_ptr(lqcs_dev_state* s, void** ptr) {
+    return guarded([&] {
+        need(s, "state");
+        need(ptr, "ptr");
+        *ptr = s->d;
+    });
+}
+
+lqcs_status lqcs_state_apply(lqcs_dev_state* s, const lqcs_program* prog, const lqcs_exec_options* opts) {
+    return guarded([&] {
+        need(s, "state");
+        need(prog, "program");
+        if (prog->n_ops && !prog->ops) throw Error(LQCS_ERR_INVALID_ARGUMENT, "ops is null");
+        ensure_ready();
+        arena_reset();
+        std::vector<DecodedOp> ops;
+        ops.reserve(prog->n_ops);
+        for (uint64_t k = 0; k < prog->n_ops; ++k) ops.push_back(decode(*prog, k, s->nq));
+        const int plan = opts ? opts->plan : LQCS_PLAN_AUTO;
+        const uint32_t L = opts ? static_cast<uint32_t>(std::max(0, opts->tile_qubits)) : 0;
+        const uint32_t low = opts ? static_cast<uint32_t>(std::max(0, opts->low_qubits)) : 0;
+        if (plan == LQCS_PLAN_LEARNED) {
+            apply_learned(s->d, s->nq, ops, L, low, 1);
+        } else {
+            apply_ops(s->d, s->nq, ops, plan, L, low, 1);
+        }
+    });
+}
+
+lqcs_status lqcs_last_plan_stats(uint64_t* passes, uint64_t* launches) {
+    if (passes) *passes = g_last_passes;
+    if (launches) *launches = g_last_launches;
+    return LQCS_OK;
+}
+
+lqcs_status lqcs_state_norm_sq(const lqcs_dev_state* s, double* out) {
+    return guarded([&] {
+        need(s, "state");
+        need(out, "out");
+        ensure_ready();
+        arena_reset();
+        double* d = static_cast<double*>(arena(sizeof(double)));
+        reduce_norm_sq(s->d, s->nq, 1, d);
+        LQCS_HIP(hipMemcpyAsync(out, d, sizeof(double), hipMemcpyDeviceToHost, stream()));
+        sync();
+    });
+}
+
+lqcs_status lqcs_state_probabilities(const lqcs_dev_state* s, double* out) {
+    return guarded([&] {
+        need(s, "state");
+        need(out, "out");
+        ensure_ready();
+        arena_reset();
+        double* d = static_cast<double*>(arena(sizeof(double) * amps_of(s->nq)));
+        launch_probabilities(s->d, s->nq, d);
+        LQCS_HIP(hipMemcpyAsync(out, d, sizeof(double) * amps_of(s->nq), hipMemcpyDeviceToHost, stream()));
+        sync();
+    });
+}
+
+lqcs_status lqcs_state_expectation_pauli(const lqcs_dev_state* s, uint64_t x_mask, uint64_t zy_mask,
+                                         uint32_t n_y, double* out) {
+    const double one = 1.0;
+    return lqcs_state_expectation_terms(s, &x_mask, &zy_mask, &n_y, &one, 1, out);
+}
+
+lqcs_status lqcs_state_expectation_terms(const lqcs_dev_state* s, const uint64_t* x_masks,
+                                         const uint64_t* zy_masks, const uint32_t* n_ys,
+                                         const double* coeffs, uint64_t n_terms, double* out) {
+    return guarded([&] {
+        need(s, "state");
+        need(out, "out");
+        *out = 0.0;
+        if (n_terms == 0) return;
+        need(x_masks, "x_masks");
+        need(zy_masks, "zy_masks");
+        need(n_ys, "n_ys");
+        need(coeffs, "coeffs");
+        const uint64_t full = amps_of(s->nq) - 1;
+        for (uint64_t t = 0; t < n_terms; ++t)
+            if ((x_masks[t] | zy_masks[t]) & ~full) throw Error(LQCS_ERR_OUT_OF_RANGE, "pauli mask out of range");
+        ensure_ready();
+        arena_reset();
+        std::vector<double> vals(2 * n_terms);
+        constexpr uint64_t kChunk = 60000;  // grid.y limit
+        for (uint64_t t0 = 0; t0 < n_terms; t0 += kChunk) {
+            const uint32_t cnt = static_cast<uint32_t>(std::min(kChunk, n_terms - t0));
+            const uint64_t* dx = stage(x_masks + t0, cnt);
+            const uint64_t* dz = stage(zy_masks + t0, cnt);
+            double* d = static_cast<double*>(arena(sizeof(double) * 2 * cnt));
+            reduce_pauli_terms(s->d, s->nq, dx, dz, cnt, d);
+            LQCS_HIP(hipMemcpyAsync(vals.data() + 2 * t0, d, sizeof(double) * 2 * cnt, hipMemcpyDeviceToHost,
+                                    stream()));
+        }
+        sync();
+        // Combine as algorithms::expectation does: e += coeff * Re(i^ny * sum).
+        double e = 0.0;
+        for (uint64_t t = 0; t < n_terms; ++t) {
+            const double re = vals[2 * t], im = vals[2 * t + 1];
+            double v;
+            switch (n_ys[t] % 4) {
+                case 0: v = re; break;
+                case 1: v = -im; break;
+                case 2: v = -re; break;
+                default: v = im; break;
+            }
+            e += coeffs[t] * v;
+        }
+        *out = e;
+    });
+}
+
+lqcs_status lqcs_state_sample(const lqcs_dev_state* s, const double* draws, uint64_t n_draws,
+                              uint64_t* out_indices) {
+    return guarded([&] {
+        need(s, "state");
+        if (n_draws == 0) return;
+        need(draws, "draws");
+        need(out_indices, "out_indices");
+        ensure_ready();
+        arena_reset();
+        const double* dd = stage(draws, n_draws);
+        uint64_t* di = static_cast<uint64_t*>(arena(sizeof(uint64_t) * n_draws));
+        constexpr uint64_t kMaxGrid = 1u << 30;
+        for (uint64_t o = 0; o < n_draws; o += kMaxGrid) {
+            const uint64_t cnt = std::min(kMaxGrid, n_draws - o);
+            launch_sample(s->d, s->nq, dd + o, cnt, di + o);
+        }
+        LQCS_HIP(hipMemcpyAsync(out_indices, di, sizeof(uint64_t) * n_draws, hipMemcpyDeviceToHost, stream()));
+        sync();
+    });
+}
+
+lqcs_status lqcs_state_collapse(lqcs_dev_state* s, uint32_t qubit, double u, int* outcome) {
+    return guarded([&] {
+        need(s, "state");
+        if (qubit >= s->nq) throw Error(LQCS_ERR_OUT_OF_RANGE, "qubit out of range");
+        ensure_ready();
+        arena_reset();
+        double* p1 = static_cast<double*>(arena(sizeof(double)));
+        const double* du = stage(&u, 1);
+        uint8_t* creg = static_cast<uint8_t*>(arena(1));
+        reduce_p1(s->d, s->nq, qubit, 1, p1);
+        launch_collapse(s->d, s->nq, qubit, p1, du, 1, 0, 0, creg, 1, 0, 1);
+        uint8_t h = 0;
+        LQCS_HIP(hipMemcpyAsync(&h, creg, 1, hipMemcpyDeviceToHost, stream()));
+        sync();
+        if (outcome) *outcome = h;
+    });
+}
+
+lqcs_status lqcs_state_reset_qubit(lqcs_dev_state* s, uint32_t qubit, double u) {
+    return guarded([&] {
+        need(s, "state");
+        if (qubit >= s->nq) throw Error(LQCS_ERR_OUT_OF_RANGE, "qubit out of range");
+        ensure_ready();
+        arena_reset();
+        double* p1 = static_cast<double*>(arena(sizeof(double)));
+        const double* du = stage(&u, 1);
+        reduce_p1(s->d, s->nq, qubit, 1, p1);
+        launch_collapse(s->d, s->nq, qubit, p1, du, 1, 0, 1, nullptr, 0, -1, 1);
+    });
+}
+
+lqcs_status lqcs_state_kraus_probability(const lqcs_dev_state* s, uint32_t qubit, const double* k,
+                                         double* out) {
+    return guarded([&] {
+        need(s, "state");
+        need(k, "k");
+        need(out, "out");
+        if (qubit >= s->nq) throw Error(LQCS_ERR_OUT_OF_RANGE, "qubit out of range");
+        ensure_ready();
+        arena_reset();
+        double* d = static_cast<double*>(arena(4 * sizeof(double)));
+        reduce_qstats(s->d, s->nq, qubit, 1, d);
+        double st[4];
+        LQCS_HIP(hipMemcpyAsync(st, d, sizeof st, hipMemcpyDeviceToHost, stream()));
+        sync();
+        *out = kraus_branch_probability(k, st);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+// Engine internals shared by the C-ABI translation units.
+#pragma once
+
+#include "common.hpp"
+#include "tile.hpp"
+
+#include <vector>
+
+#include "../../../include/lqcs_device.h"
+
+namespace lqcs_dev {
+
+void init_device(int device);
+void ensure_ready();
+void sync();
+
+// One op of the stream with its tables copied out of the program pools.
+struct DecodedOp {
+    uint32_t kind = 0;
+    std::vector<uint32_t> controls, targets;
+    std::vector<amp_t> m;        // matrix (row-major) or d0,d1
+    std::vector<uint64_t> perm;
+    uint64_t cmask = 0;          // control bits
+    uint64_t touched = 0;        // every qubit the op reads or writes
+    uint64_t x = 0, zy = 0;      // Pauli rotation
+    uint32_t n_y = 0;
+    double theta = 0.0;
+    bool diagonal = false;       // exact zeros off the diagonal
+};
+
+DecodedOp decode(const lqcs_program& p, uint64_t k, uint32_t nq);
+void run_single_op(amp_t* s, uint32_t nq, const DecodedOp& d, uint32_t batch);
+
+struct PlanStep {
+    bool tiled = false;
+    uint64_t smask = 0;            // tile qubits (tiled)
+    std::vector<TileOp> tile_ops;  // tiled
+    uint32_t op = 0;               // stand-alone op index
+};
+struct Plan {
+    std::vector<PlanStep> steps;
+};
+
+struct TileChoice {
+    uint32_t L = 0;    // 0: no tiling
+    uint32_t low = 0;
+};
+TileChoice default_tile(uint32_t nq);
+
+Plan plan_greedy(const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t low);
+void execute_plan(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan,
+                  const std::vector<DecodedOp>& ops, uint32_t batch);
+void apply_ops(amp_t* s, uint32_t nq, const std::vector<DecodedOp>& ops, int plan_kind,
+               uint32_t L, uint32_t low, uint32_t batch);
+// Cost-model guided planning (planner.hip); falls back to greedy.
+void apply_learned(amp_t* s, uint32_t nq, const std::vector<DecodedOp>& ops, uint32_t L,
+                   uint32_t low, uint32_t batch);
+void set_last_stats(uint64_t passes, uint64_t launches);
+
+// ||K psi||^2 on one qubit from its 2x2 statistics (S00, S11, Re S01, Im S01)
+// where S01 = sum conj(a0) a1; k = 2x2 row-major complex (8 doubles).
+inline double kraus_branch_probability(const double* k, const double* st) {
+    const double k0r = k[0], k0i = k[1], k1r = k[2], k1i = k[3];
+    const double k2r = k[4], k2i = k[5], k3r = k[6], k3i = k[7];
+    const double c00 = k0r * k0r + k0i * k0i + k2r * k2r + k2i * k2i;
+    const double c11 = k1r * k1r + k1i * k1i + k3r * k3r + k3i * k3i;
+    // conj(k0) k1 + conj(k2) k3
+    const double xr = k0r * k1r + k0i * k1i + k2r * k3r + k2i * k3i;
+    const double xi = k0r * k1i - k0i * k1r + k2r * k3i - k2i * k3r;
+    return c00 * st[0] + c11 * st[1] + 2.0 * (xr * st[2] - xi * st[3]);
+}
+
+}  // namespace lqcs_dev

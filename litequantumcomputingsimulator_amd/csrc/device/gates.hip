@@ -1,0 +1,373 @@
+// Gate-by-gate kernels: one HBM pass per operation, one kernel per entry of
+// the reference kernel seam (src/backend/kernels/kernels.hpp:13-62). They are
+// the fallback for ops the tiled passes do not take and the baseline the
+// planner is measured against.
+//
+// Every kernel streams complex128 amplitudes with 16-byte loads; a thread owns
+// kItems independent groups so each wave keeps several loads in flight.
+// blockIdx.y selects a register in a batch (stride = 2^n amplitudes).
+#include "common.hpp"
+
+#include <vector>
+
+namespace lqcs_dev {
+namespace {
+
+constexpr int kItems = 2;
+
+__device__ __forceinline__ uint64_t group_index(int k) {
+    return (uint64_t(blockIdx.x) * kItems + k) * blockDim.x + threadIdx.x;
+}
+__device__ __forceinline__ amp_t* batch_base(amp_t* s, uint32_t nq) {
+    return s + (uint64_t(blockIdx.y) << nq);
+}
+
+// gate_kernels.cpp:26-39 -- pairs (i0, i0 | 2^t)
+__global__ void __launch_bounds__(kBlock) k_single(amp_t* __restrict__ s, uint32_t nq, uint32_t t,
+                                                   Mat2 m) {
+    amp_t* b = batch_base(s, nq);
+    const uint64_t groups = uint64_t{1} << (nq - 1);
+    amp_t a0[kItems], a1[kItems];
+    uint64_t i0[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t g = group_index(k);
+        i0[k] = insert_zero_bit(g < groups ? g : 0, t);
+        a0[k] = ld(b + i0[k]);
+        a1[k] = ld(b + (i0[k] | (uint64_t{1} << t)));
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        if (group_index(k) >= groups) continue;
+        st(b + i0[k], cmac2(m.m[0], a0[k], m.m[1], a1[k]));
+        st(b + (i0[k] | (uint64_t{1} << t)), cmac2(m.m[2], a0[k], m.m[3], a1[k]));
+    }
+}
+
+// :41-47 -- phase multiply, every amplitude
+__global__ void __launch_bounds__(kBlock) k_diag_single(amp_t* __restrict__ s, uint32_t nq,
+                                                        uint32_t t, amp_t d0, amp_t d1) {
+    amp_t* b = batch_base(s, nq);
+    const uint64_t n = uint64_t{1} << nq;
+    amp_t a[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = group_index(k);
+        a[k] = ld(b + (i < n ? i : 0));
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = group_index(k);
+        if (i >= n) continue;
+        st(b + i, cmul(a[k], ((i >> t) & 1) ? d1 : d0));
+    }
+}
+
+// :49-69 -- only the control=1 half is touched
+__global__ void __launch_bounds__(kBlock) k_controlled_single(amp_t* __restrict__ s, uint32_t nq,
+                                                              uint32_t c, uint32_t t, Mat2 m) {
+    amp_t* b = batch_base(s, nq);
+    const unsigned lo = c < t ? c : t, hi = c < t ? t : c;
+    const uint64_t groups = uint64_t{1} << (nq - 2);
+    amp_t a0[kItems], a1[kItems];
+    uint64_t i0[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t g = group_index(k);
+        i0[k] = insert_two_zero_bits(g < groups ? g : 0, lo, hi) | (uint64_t{1} << c);
+        a0[k] = ld(b + i0[k]);
+        a1[k] = ld(b + (i0[k] | (uint64_t{1} << t)));
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        if (group_index(k) >= groups) continue;
+        st(b + i0[k], cmac2(m.m[0], a0[k], m.m[1], a1[k]));
+        st(b + (i0[k] | (uint64_t{1} << t)), cmac2(m.m[2], a0[k], m.m[3], a1[k]));
+    }
+}
+
+// :71-84 -- exchange |a=1,b=0> <-> |a=0,b=1>
+__global__ void __launch_bounds__(kBlock) k_swap(amp_t* __restrict__ s, uint32_t nq, uint32_t qa,
+                                                 uint32_t qb) {
+    amp_t* b = batch_base(s, nq);
+    const unsigned lo = qa < qb ? qa : qb, hi = qa < qb ? qb : qa;
+    const uint64_t groups = uint64_t{1} << (nq - 2);
+    amp_t x[kItems], y[kItems];
+    uint64_t base[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t g = group_index(k);
+        base[k] = insert_two_zero_bits(g < groups ? g : 0, lo, hi);
+        x[k] = ld(b + (base[k] | (uint64_t{1} << qa)));
+        y[k] = ld(b + (base[k] | (uint64_t{1} << qb)));
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        if (group_index(k) >= groups) continue;
+        st(b + (base[k] | (uint64_t{1} << qa)), y[k]);
+        st(b + (base[k] | (uint64_t{1} << qb)), x[k]);
+    }
+}
+
+// :86-106 -- dense 4x4, local bit 0 <-> q0
+__global__ void __launch_bounds__(kBlock) k_two_qubit(amp_t* __restrict__ s, uint32_t nq,
+                                                      uint32_t q0, uint32_t q1, Mat4 m) {
+    amp_t* b = batch_base(s, nq);
+    const unsigned lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
+    const uint64_t groups = uint64_t{1} << (nq - 2);
+    const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= groups) return;
+    const uint64_t base = insert_two_zero_bits(g, lo, hi);
+    const uint64_t idx[4] = {base, base | (uint64_t{1} << q0), base | (uint64_t{1} << q1),
+                             base | (uint64_t{1} << q0) | (uint64_t{1} << q1)};
+    amp_t a[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) a[r] = ld(b + idx[r]);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        amp_t acc = cmac2(m.m[4 * r], a[0], m.m[4 * r + 1], a[1]);
+        acc = cadd(acc, cmac2(m.m[4 * r + 2], a[2], m.m[4 * r + 3], a[3]));
+        st(b + idx[r], acc);
+    }
+}
+
+// Controlled k-target layout (:118-139): fixed positions sorted ascending.
+struct Layout {
+    uint32_t fixed[LQCS_LAYOUT_MAX];
+    uint32_t n_fixed;
+    uint32_t nt;
+    uint64_t c_mask;
+};
+
+__device__ __forceinline__ uint64_t expand(uint64_t g, const Layout& L) {
+    for (uint32_t i = 0; i < L.n_fixed; ++i) g = insert_zero_bit(g, L.fixed[i]);
+    return g | L.c_mask;
+}
+
+// :143-167 for small k: one thread per group, amplitudes in registers,
+// matrix staged in LDS by the block.
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_cu_small(amp_t* __restrict__ s, uint32_t nq, Layout L,
+                                                     const uint64_t* __restrict__ offs,
+                                                     const amp_t* __restrict__ mat) {
+    constexpr int D = 1 << K;
+    __shared__ amp_t sm[D * D];
+    for (int i = threadIdx.x; i < D * D; i += blockDim.x) sm[i] = mat[i];
+    __syncthreads();
+    amp_t* b = batch_base(s, nq);
+    const uint64_t groups = uint64_t{1} << (nq - L.n_fixed);
+    const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= groups) return;
+    const uint64_t base = expand(g, L);
+    amp_t a[D];
+#pragma unroll
+    for (int l = 0; l < D; ++l) a[l] = ld(b + (base | offs[l]));
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+        amp_t acc = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int c = 0; c < D; ++c) acc = cadd(acc, cmul(sm[r * D + c], a[c]));
+        st(b + (base | offs[r]), acc);
+    }
+}
+
+// :143-167 for larger k: one block per group, the group staged in LDS.
+__global__ void __launch_bounds__(kBlock) k_cu_large(amp_t* __restrict__ s, uint32_t nq, Layout L,
+                                                     const uint64_t* __restrict__ offs,
+                                                     const amp_t* __restrict__ mat) {
+    extern __shared__ amp_t grp[];
+    const uint32_t D = 1u << L.nt;
+    amp_t* b = batch_base(s, nq);
+    const uint64_t base = expand(blockIdx.x, L);
+    for (uint32_t l = threadIdx.x; l < D; l += blockDim.x) grp[l] = ld(b + (base | offs[l]));
+    __syncthreads();
+    amp_t out[LQCS_CU_LARGE_ROWS];
+    int k = 0;
+    for (uint32_t r = threadIdx.x; r < D; r += blockDim.x, ++k) {
+        amp_t acc = make_double2(0.0, 0.0);
+        for (uint32_t c = 0; c < D; ++c) acc = cadd(acc, cmul(mat[uint64_t(r) * D + c], grp[c]));
+        out[k] = acc;
+    }
+    __syncthreads();
+    k = 0;
+    for (uint32_t r = threadIdx.x; r < D; r += blockDim.x, ++k) st(b + (base | offs[r]), out[k]);
+}
+
+// :169-189 -- |l> -> |perm[l]> on the control subspace, staged through LDS.
+__global__ void __launch_bounds__(kBlock) k_cperm(amp_t* __restrict__ s, uint32_t nq, Layout L,
+                                                  const uint64_t* __restrict__ offs,
+                                                  const uint64_t* __restrict__ perm) {
+    extern __shared__ amp_t grp[];
+    const uint32_t D = 1u << L.nt;
+    amp_t* b = batch_base(s, nq);
+    const uint64_t base = expand(blockIdx.x, L);
+    for (uint32_t l = threadIdx.x; l < D; l += blockDim.x) grp[l] = ld(b + (base | offs[l]));
+    __syncthreads();
+    for (uint32_t l = threadIdx.x; l < D; l += blockDim.x) st(b + (base | offs[perm[l]]), grp[l]);
+}
+
+// :191-225 -- exp(-i theta/2 P): pairs (i, i^x) with i's lowest x bit clear
+__global__ void __launch_bounds__(kBlock) k_pauli_pairs(amp_t* __restrict__ s, uint32_t nq,
+                                                        uint64_t x, uint64_t zy, uint32_t lsb,
+                                                        amp_t c, amp_t yphase) {
+    amp_t* b = batch_base(s, nq);
+    const uint64_t groups = uint64_t{1} << (nq - 1);
+    amp_t ai[kItems], aj[kItems];
+    uint64_t ii[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t g = group_index(k);
+        ii[k] = insert_zero_bit(g < groups ? g : 0, lsb);
+        ai[k] = ld(b + ii[k]);
+        aj[k] = ld(b + (ii[k] ^ x));
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        if (group_index(k) >= groups) continue;
+        const uint64_t i = ii[k], j = i ^ x;
+        const double si = (__popcll(i & zy) & 1) ? -1.0 : 1.0;
+        const double sj = (__popcll(j & zy) & 1) ? -1.0 : 1.0;
+        st(b + i, cmac2(c, ai[k], cscale(yphase, sj), aj[k]));
+        st(b + j, cmac2(cscale(yphase, si), ai[k], c, aj[k]));
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_pauli_diag(amp_t* __restrict__ s, uint32_t nq,
+                                                       uint64_t zy, amp_t c, amp_t yphase) {
+    amp_t* b = batch_base(s, nq);
+    const uint64_t n = uint64_t{1} << nq;
+    const amp_t dp = cadd(c, yphase), dm = cadd(c, cscale(yphase, -1.0));
+    amp_t a[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = group_index(k);
+        a[k] = ld(b + (i < n ? i : 0));
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = group_index(k);
+        if (i >= n) continue;
+        st(b + i, cmul(a[k], (__popcll(i & zy) & 1) ? dm : dp));
+    }
+}
+
+__global__ void k_fill_basis(amp_t* __restrict__ s, uint32_t nq, uint64_t index) {
+    amp_t* b = batch_base(s, nq);
+    const uint64_t n = uint64_t{1} << nq;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * blockDim.x)
+        b[i] = make_double2(i == index ? 1.0 : 0.0, 0.0);
+}
+
+Layout make_layout(const uint32_t* ctl, uint32_t nc, const uint32_t* tgt, uint32_t nt) {
+    Layout L{};
+    if (nc + nt > LQCS_LAYOUT_MAX) throw Error(3, "operation touches too many qubits");
+    L.n_fixed = 0;
+    for (uint32_t i = 0; i < nc; ++i) L.fixed[L.n_fixed++] = ctl[i];
+    for (uint32_t i = 0; i < nt; ++i) L.fixed[L.n_fixed++] = tgt[i];
+    for (uint32_t i = 1; i < L.n_fixed; ++i)
+        for (uint32_t j = i; j > 0 && L.fixed[j - 1] > L.fixed[j]; --j) {
+            const uint32_t tmp = L.fixed[j];
+            L.fixed[j] = L.fixed[j - 1];
+            L.fixed[j - 1] = tmp;
+        }
+    L.c_mask = 0;
+    for (uint32_t i = 0; i < nc; ++i) L.c_mask |= uint64_t{1} << ctl[i];
+    L.nt = nt;
+    return L;
+}
+
+std::vector<uint64_t> target_offsets(const uint32_t* tgt, uint32_t nt) {
+    std::vector<uint64_t> off(size_t{1} << nt, 0);
+    for (uint64_t l = 0; l < off.size(); ++l)
+        for (uint32_t j = 0; j < nt; ++j)
+            if ((l >> j) & 1) off[l] |= uint64_t{1} << tgt[j];
+    return off;
+}
+
+}  // namespace
+
+void launch_single(amp_t* s, uint32_t nq, uint32_t t, const Mat2& m, uint32_t batch) {
+    k_single<<<dim3(grid_for(uint64_t{1} << (nq - 1), kItems), batch), kBlock, 0, stream()>>>(s, nq, t, m);
+}
+
+void launch_diag_single(amp_t* s, uint32_t nq, uint32_t t, amp_t d0, amp_t d1, uint32_t batch) {
+    k_diag_single<<<dim3(grid_for(uint64_t{1} << nq, kItems), batch), kBlock, 0, stream()>>>(s, nq, t, d0, d1);
+}
+
+void launch_controlled_single(amp_t* s, uint32_t nq, uint32_t c, uint32_t t, const Mat2& m,
+                              uint32_t batch) {
+    k_controlled_single<<<dim3(grid_for(uint64_t{1} << (nq - 2), kItems), batch), kBlock, 0, stream()>>>(
+        s, nq, c, t, m);
+}
+
+void launch_swap(amp_t* s, uint32_t nq, uint32_t a, uint32_t b, uint32_t batch) {
+    k_swap<<<dim3(grid_for(uint64_t{1} << (nq - 2), kItems), batch), kBlock, 0, stream()>>>(s, nq, a, b);
+}
+
+void launch_two_qubit(amp_t* s, uint32_t nq, uint32_t q0, uint32_t q1, const Mat4& m,
+                      uint32_t batch) {
+    k_two_qubit<<<dim3(grid_for(uint64_t{1} << (nq - 2)), batch), kBlock, 0, stream()>>>(s, nq, q0, q1, m);
+}
+
+void launch_controlled_unitary(amp_t* s, uint32_t nq, const uint32_t* ctl, uint32_t nc,
+                               const uint32_t* tgt, uint32_t nt, const amp_t* m_host,
+                               uint32_t batch) {
+    const Layout L = make_layout(ctl, nc, tgt, nt);
+    const std::vector<uint64_t> off = target_offsets(tgt, nt);
+    const uint64_t D = uint64_t{1} << nt;
+    const uint64_t groups = uint64_t{1} << (nq - L.n_fixed);
+    const uint64_t* d_off = stage(off.data(), off.size());
+    const amp_t* d_mat = stage(m_host, D * D);
+    switch (nt) {
+        case 1: k_cu_small<1><<<dim3(grid_for(groups), batch), kBlock, 0, stream()>>>(s, nq, L, d_off, d_mat); return;
+        case 2: k_cu_small<2><<<dim3(grid_for(groups), batch), kBlock, 0, stream()>>>(s, nq, L, d_off, d_mat); return;
+        case 3: k_cu_small<3><<<dim3(grid_for(groups), batch), kBlock, 0, stream()>>>(s, nq, L, d_off, d_mat); return;
+        case 4: k_cu_small<4><<<dim3(grid_for(groups), batch), kBlock, 0, stream()>>>(s, nq, L, d_off, d_mat); return;
+        default: break;
+    }
+    if (D > uint64_t(kBlock) * LQCS_CU_LARGE_ROWS)
+        throw Error(3, "controlled unitary wider than the engine supports");
+    k_cu_large<<<dim3(static_cast<unsigned>(groups), batch), kBlock, D * sizeof(amp_t), stream()>>>(
+        s, nq, L, d_off, d_mat);
+}
+
+void launch_controlled_permutation(amp_t* s, uint32_t nq, const uint32_t* ctl, uint32_t nc,
+                                   const uint32_t* tgt, uint32_t nt, const uint64_t* perm_host,
+                                   uint32_t batch) {
+    const Layout L = make_layout(ctl, nc, tgt, nt);
+    const std::vector<uint64_t> off = target_offsets(tgt, nt);
+    const uint64_t D = uint64_t{1} << nt;
+    if (D * sizeof(amp_t) > 64 * 1024) throw Error(3, "permutation wider than the engine supports");
+    const uint64_t groups = uint64_t{1} << (nq - L.n_fixed);
+    const uint64_t* d_off = stage(off.data(), off.size());
+    const uint64_t* d_perm = stage(perm_host, D);
+    k_cperm<<<dim3(static_cast<unsigned>(groups), batch), kBlock, D * sizeof(amp_t), stream()>>>(s, nq, L, d_off,
+                                                                                   d_perm);
+}
+
+void launch_pauli_rotation(amp_t* s, uint32_t nq, uint64_t x, uint64_t zy, uint32_t n_y,
+                           double theta, uint32_t batch) {
+    const double h = theta / 2;
+    const amp_t c = make_double2(std::cos(h), 0.0);
+    const amp_t mis = make_double2(0.0, -std::sin(h));
+    static const amp_t kIPow[4] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
+    const amp_t ip = kIPow[n_y % 4];
+    const amp_t yphase = make_double2(mis.x * ip.x - mis.y * ip.y, mis.x * ip.y + mis.y * ip.x);
+    if (x == 0) {
+        k_pauli_diag<<<dim3(grid_for(uint64_t{1} << nq, kItems), batch), kBlock, 0, stream()>>>(s, nq, zy, c,
+                                                                                   yphase);
+        return;
+    }
+    const uint32_t lsb = static_cast<uint32_t>(__builtin_ctzll(x));
+    k_pauli_pairs<<<dim3(grid_for(uint64_t{1} << (nq - 1), kItems), batch), kBlock, 0, stream()>>>(
+        s, nq, x, zy, lsb, c, yphase);
+}
+
+void launch_fill_basis(amp_t* s, uint32_t nq, uint64_t index, uint32_t batch) {
+    const uint64_t n = uint64_t{1} << nq;
+    const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(grid_for(n), 65536));
+    k_fill_basis<<<dim3(grid, batch), kBlock, 0, stream()>>>(s, nq, index);
+}
+
+}  // namespace lqcs_dev

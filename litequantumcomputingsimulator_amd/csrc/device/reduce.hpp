@@ -1,0 +1,23 @@
+// Launchers of the read-mostly passes (reduce.hip). Results land in device
+// memory; `batch` registers of 2^nq amplitudes lie back to back.
+#pragma once
+
+#include "common.hpp"
+
+namespace lqcs_dev {
+
+void reduce_norm_sq(const amp_t* s, uint32_t nq, uint32_t batch, double* d_out);
+// d_out[2*t .. 2*t+1] = (re, im) of sum_i sign_t(i) a_i conj(a_{i ^ x_t}).
+void reduce_pauli_terms(const amp_t* s, uint32_t nq, const uint64_t* d_x, const uint64_t* d_zy,
+                        uint32_t n_terms, double* d_out);
+void reduce_p1(const amp_t* s, uint32_t nq, uint32_t q, uint32_t batch, double* d_out);
+// d_out[4*r ..] = S00, S11, Re S01, Im S01 of qubit q in register r.
+void reduce_qstats(const amp_t* s, uint32_t nq, uint32_t q, uint32_t batch, double* d_out);
+void launch_collapse(amp_t* s, uint32_t nq, uint32_t q, const double* d_p1, const double* d_draws,
+                     uint64_t draw_stride, uint64_t draw_offset, int reset, uint8_t* d_creg,
+                     uint32_t creg_stride, int32_t clbit, uint32_t batch);
+void launch_probabilities(const amp_t* s, uint32_t nq, double* d_out);
+void launch_sample(const amp_t* s, uint32_t nq, const double* d_draws, uint64_t n_draws,
+                   uint64_t* d_out);
+
+}  // namespace lqcs_dev
