@@ -1,0 +1,19 @@
+// Engine knobs of the MI355X statevector engine (no counterpart in the
+// reference; defaults reproduce its behaviour).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+namespace lqcs {
+
+struct EngineOptions {
+    enum class Plan : int { GateByGate = 0, Greedy = 1, Learned = 2, Auto = 3 };
+    Plan plan = Plan::Auto;
+    int tile_qubits = 0;            // LDS tile width L (0 = engine default)
+    int low_qubits = 0;             // contiguous low qubits per tile (0 = default)
+    std::size_t max_qubits = 30;    // register cap; raise up to 34 on the device
+    std::uint32_t max_batch = 0;    // trajectories resident at once (0 = by free HBM)
+};
+
+}  // namespace lqcs

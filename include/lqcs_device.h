@@ -121,6 +121,14 @@ lqcs_status lqcs_state_apply(lqcs_dev_state* s, const lqcs_program* prog,
  * kernel launches it issued. */
 lqcs_status lqcs_last_plan_stats(uint64_t* passes, uint64_t* launches);
 
+/* Launch timing (no counterpart in the reference): while enabled, every
+ * tiled pass and every gate-by-gate kernel of lqcs_state_apply is bracketed
+ * by HIP events on the engine stream. lqcs_profile_read synchronises and
+ * returns launch counts and summed kernel milliseconds, then clears them. */
+lqcs_status lqcs_profile_enable(int enable);
+lqcs_status lqcs_profile_read(uint64_t* tile_launches, double* tile_ms, uint64_t* other_launches,
+                              double* other_ms);
+
 /* ---- reductions and measurement ---------------------------------------- */
 /* Statevector::norm (statevector.cpp:27-31) squared, before the sqrt. */
 lqcs_status lqcs_state_norm_sq(const lqcs_dev_state* s, double* out);

@@ -44,6 +44,28 @@ Context& ctx() {
 }
 
 thread_local std::string g_last_error;
+
+// Launch timing state (lqcs_profile_*).
+struct Profile {
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> tile, other;
+    std::vector<hipEvent_t> pool;
+};
+Profile& prof() {
+    static Profile p;
+    return p;
+}
+hipEvent_t take_event() {
+    Profile& p = prof();
+    if (!p.pool.empty()) {
+        hipEvent_t e = p.pool.back();
+        p.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    LQCS_HIP(hipEventCreate(&e));
+    return e;
+}
 thread_local uint64_t g_last_passes = 0, g_last_launches = 0;
 
 std::mutex& api_mutex() {
@@ -433,11 +455,22 @@ void execute_plan(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan,
     const TileOp* d_ops = flat.empty() ? nullptr : stage(flat.data(), flat.size());
     for (size_t i = 0; i < plan.steps.size(); ++i) {
         const PlanStep& st = plan.steps[i];
+        const bool timed = prof().on;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timed) {
+            e0 = take_event();
+            e1 = take_event();
+            LQCS_HIP(hipEventRecord(e0, stream()));
+        }
         if (st.tiled) {
             launch_tile_pass(s, nq, L, st.smask, d_ops + first[i],
                              static_cast<uint32_t>(st.tile_ops.size()), batch);
         } else {
             run_single_op(s, nq, ops[st.op], batch);
+        }
+        if (timed) {
+            LQCS_HIP(hipEventRecord(e1, stream()));
+            (st.tiled ? prof().tile : prof().other).emplace_back(e0, e1);
         }
     }
     g_last_passes = plan.steps.size();
@@ -459,7 +492,20 @@ void apply_ops(amp_t* s, uint32_t nq, const std::vector<DecodedOp>& ops, int pla
     if (L == 0) L = def.L;
     if (low == 0) low = def.low;
     if (plan_kind == LQCS_PLAN_GATE_BY_GATE || L == 0) {
-        for (const auto& d : ops) run_single_op(s, nq, d, batch);
+        for (const auto& d : ops) {
+            const bool timed = prof().on;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (timed) {
+                e0 = take_event();
+                e1 = take_event();
+                LQCS_HIP(hipEventRecord(e0, stream()));
+            }
+            run_single_op(s, nq, d, batch);
+            if (timed) {
+                LQCS_HIP(hipEventRecord(e1, stream()));
+                prof().other.emplace_back(e0, e1);
+            }
+        }
         g_last_passes = ops.size();
         g_last_launches = ops.size();
         return;
@@ -502,6 +548,23 @@ lqcs_status guarded(F&& f) {
     }
 }
 
+}  // namespace
+
+namespace lqcs_dev {
+lqcs_status run_guarded(const std::function<void()>& f) { return guarded(f); }
+void* device_alloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess)
+        throw Error(LQCS_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    return p;
+}
+void device_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+}  // namespace lqcs_dev
+
+namespace {
+
 void need(const void* p, const char* what) {
     if (!p) throw Error(LQCS_ERR_INVALID_ARGUMENT, std::string(what) + " is null");
 }
@@ -535,6 +598,38 @@ lqcs_status lqcs_dev_synchronize(void) {
     return guarded([&] {
         ensure_ready();
         sync();
+        LQCS_HIP(hipDeviceSynchronize());
+    });
+}
+
+lqcs_status lqcs_profile_enable(int enable) {
+    return guarded([&] {
+        ensure_ready();
+        prof().on = enable != 0;
+    });
+}
+
+lqcs_status lqcs_profile_read(uint64_t* tile_launches, double* tile_ms, uint64_t* other_launches,
+                              double* other_ms) {
+    return guarded([&] {
+        ensure_ready();
+        sync();
+        Profile& p = prof();
+        auto drain = [&](std::vector<std::pair<hipEvent_t, hipEvent_t>>& v, uint64_t* n, double* ms) {
+            double total = 0.0;
+            for (auto& [a, b] : v) {
+                float t = 0.0f;
+                LQCS_HIP(hipEventElapsedTime(&t, a, b));
+                total += t;
+                p.pool.push_back(a);
+                p.pool.push_back(b);
+            }
+            if (n) *n = v.size();
+            if (ms) *ms = total;
+            v.clear();
+        };
+        drain(p.tile, tile_launches, tile_ms);
+        drain(p.other, other_launches, other_ms);
     });
 }
 

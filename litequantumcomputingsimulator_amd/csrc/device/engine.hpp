@@ -4,6 +4,7 @@
 #include "common.hpp"
 #include "tile.hpp"
 
+#include <functional>
 #include <vector>
 
 #include "../../../include/lqcs_device.h"
@@ -56,6 +57,11 @@ void apply_ops(amp_t* s, uint32_t nq, const std::vector<DecodedOp>& ops, int pla
 void apply_learned(amp_t* s, uint32_t nq, const std::vector<DecodedOp>& ops, uint32_t L,
                    uint32_t low, uint32_t batch);
 void set_last_stats(uint64_t passes, uint64_t launches);
+// Runs f under the engine lock, mapping exceptions to an lqcs_status and the
+// thread's last-error text.
+lqcs_status run_guarded(const std::function<void()>& f);
+void* device_alloc(size_t bytes);  // hipMalloc with OOM mapped to LQCS_ERR_OUT_OF_MEMORY
+void device_free(void* p);
 
 // ||K psi||^2 on one qubit from its 2x2 statistics (S00, S11, Re S01, Im S01)
 // where S01 = sum conj(a0) a1; k = 2x2 row-major complex (8 doubles).

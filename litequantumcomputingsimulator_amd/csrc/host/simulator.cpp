@@ -1,0 +1,355 @@
+// Statevector container and simulator on the device engine. The public
+// behaviour (path selection, exceptions, RNG consumption, bitstring format)
+// follows src/backend/statevector_simulator.cpp and statevector.cpp of the
+// reference; every amplitude operation runs on the GPU through lqcs_device.h.
+#include <cstring>
+#include <map>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <utility>
+
+#include "lowering.hpp"
+#include "lqcs/backend/compiled.hpp"
+#include "lqcs/backend/statevector_simulator.hpp"
+#include "lqcs/core/random.hpp"
+#include "lqcs/transpiler/gate_fusion.hpp"
+
+namespace lqcs {
+
+using detail::check;
+
+// ---- Statevector ------------------------------------------------------------------
+Statevector::Statevector(std::size_t num_qubits) : Statevector(num_qubits, kMaxQubits) {}
+
+Statevector::Statevector(std::size_t num_qubits, std::size_t max_qubits) : num_qubits_(num_qubits) {
+    if (num_qubits == 0) throw std::invalid_argument("Statevector: num_qubits must be >= 1");
+    const std::size_t cap = std::min<std::size_t>(max_qubits, LQCS_DEVICE_MAX_QUBITS);
+    if (num_qubits > cap) {
+        const double gib = num_qubits >= 30 ? 16.0 * static_cast<double>(std::size_t{1} << (num_qubits - 30))
+                                            : 16.0 / static_cast<double>(std::size_t{1} << (30 - num_qubits));
+        throw std::invalid_argument("Statevector: " + std::to_string(num_qubits) + " qubits would need " +
+                                    std::to_string(gib) + " GiB (max " + std::to_string(cap) + " qubits)");
+    }
+    check(lqcs_state_create(static_cast<uint32_t>(num_qubits), &dev_));
+    fresh_ = Fresh::Device;
+}
+
+Statevector::~Statevector() {
+    if (dev_) lqcs_state_destroy(dev_);
+}
+
+Statevector::Statevector(const Statevector& o) : num_qubits_(o.num_qubits_) {
+    check(lqcs_state_create(static_cast<uint32_t>(num_qubits_), &dev_));
+    check(lqcs_state_copy(dev_, o.device()));
+    fresh_ = Fresh::Device;
+}
+
+Statevector& Statevector::operator=(const Statevector& o) {
+    if (this != &o) {
+        Statevector tmp(o);
+        *this = std::move(tmp);
+    }
+    return *this;
+}
+
+Statevector::Statevector(Statevector&& o) noexcept
+    : num_qubits_(o.num_qubits_), dev_(o.dev_), host_(std::move(o.host_)), fresh_(o.fresh_),
+      host_exported_(o.host_exported_) {
+    o.dev_ = nullptr;
+}
+
+Statevector& Statevector::operator=(Statevector&& o) noexcept {
+    if (this != &o) {
+        if (dev_) lqcs_state_destroy(dev_);
+        num_qubits_ = o.num_qubits_;
+        dev_ = o.dev_;
+        host_ = std::move(o.host_);
+        fresh_ = o.fresh_;
+        host_exported_ = o.host_exported_;
+        o.dev_ = nullptr;
+    }
+    return *this;
+}
+
+void Statevector::make_host_current() const {
+    if (host_.size() != size()) host_.resize(size());
+    if (fresh_ == Fresh::Device) {
+        check(lqcs_state_download(dev_, reinterpret_cast<double*>(host_.data())));
+        fresh_ = Fresh::Both;
+    }
+}
+
+complex_t* Statevector::data() {
+    make_host_current();
+    fresh_ = Fresh::Host;  // caller may write through the pointer
+    host_exported_ = true;
+    return host_.data();
+}
+
+const complex_t* Statevector::data() const {
+    make_host_current();
+    return host_.data();
+}
+
+lqcs_dev_state* Statevector::device() const {
+    if (fresh_ == Fresh::Host || host_exported_) {
+        check(lqcs_state_upload(dev_, reinterpret_cast<const double*>(host_.data())));
+        fresh_ = Fresh::Both;
+    }
+    return dev_;
+}
+
+void Statevector::mark_device_newer() { fresh_ = Fresh::Device; }
+
+void Statevector::set_basis(std::size_t index) {
+    if (index >= size()) throw std::out_of_range("set_basis: index out of range");
+    check(lqcs_state_set_basis(dev_, index));
+    host_exported_ = false;
+    fresh_ = Fresh::Device;
+}
+
+double Statevector::norm() const {
+    double s = 0.0;
+    check(lqcs_state_norm_sq(device(), &s));
+    return std::sqrt(s);
+}
+
+std::vector<double> Statevector::probabilities() const {
+    std::vector<double> p(size());
+    check(lqcs_state_probabilities(device(), p.data()));
+    return p;
+}
+
+double Statevector::expectation_value(std::string_view pauli) const {
+    const detail::PauliString p = detail::parse_pauli(pauli, num_qubits_);
+    double v = 0.0;
+    check(lqcs_state_expectation_pauli(device(), p.x, p.zy, p.n_y, &v));
+    return v;
+}
+
+// ---- StatevectorSimulator ------------------------------------------------------------
+namespace {
+
+lqcs_exec_options exec_options(const EngineOptions& e) {
+    lqcs_exec_options o{};
+    o.plan = static_cast<int32_t>(e.plan);
+    o.tile_qubits = e.tile_qubits;
+    o.low_qubits = e.low_qubits;
+    return o;
+}
+
+// evolve (statevector_simulator.cpp:197-207): Measure/Barrier skipped, Reset rejected.
+Statevector evolve(const QuantumCircuit& c, const EngineOptions& eng) {
+    detail::ProgramBuilder b;
+    for (const Instruction& in : c.instructions()) {
+        if (in.gate.type == GateType::Reset)
+            throw std::invalid_argument("run_statevector: Reset is non-deterministic; use run() instead");
+        if (in.gate.is_unitary()) b.add_instruction(in);
+    }
+    Statevector sv(c.num_qubits(), eng.max_qubits);
+    const lqcs_program prog = b.program();
+    const lqcs_exec_options o = exec_options(eng);
+    check(lqcs_state_apply(sv.device(), &prog, &o));
+    sv.mark_device_newer();
+    return sv;
+}
+
+std::string key_of(const std::vector<std::optional<qubit_t>>& src, uint64_t idx) {
+    std::string key(src.size(), '0');
+    for (std::size_t c = 0; c < src.size(); ++c)
+        if (src[c] && ((idx >> *src[c]) & 1u)) key[src.size() - 1 - c] = '1';
+    return key;
+}
+
+}  // namespace
+
+QuantumCircuit StatevectorSimulator::maybe_fuse(const QuantumCircuit& circuit) const {
+    if (!options_.fuse_gates) return circuit;
+    return SingleQubitGateFusion().run(circuit);
+}
+
+Statevector StatevectorSimulator::run_statevector(const QuantumCircuit& circuit) const {
+    if (!options_.noise.empty())
+        throw std::invalid_argument(
+            "run_statevector: noisy simulation is stochastic (one trajectory is not the quantum state); use "
+            "run() or DensityMatrixSimulator");
+    return evolve(maybe_fuse(circuit), options_.engine);
+}
+
+std::vector<std::uint64_t> StatevectorSimulator::sample_indices(const QuantumCircuit& circuit,
+                                                                std::size_t shots) const {
+    if (shots == 0) throw std::invalid_argument("run: shots must be >= 1");
+    const Statevector sv = evolve(maybe_fuse(circuit), options_.engine);
+    Rng rng(options_.seed);
+    std::vector<double> draws(shots);
+    for (double& u : draws) u = rng.uniform();
+    std::vector<std::uint64_t> idx(shots);
+    check(lqcs_state_sample(sv.device(), draws.data(), shots, idx.data()));
+    return idx;
+}
+
+Result StatevectorSimulator::run(const QuantumCircuit& circuit, std::size_t shots) {
+    if (shots == 0) throw std::invalid_argument("run: shots must be >= 1");
+    const bool noisy = !options_.noise.empty();
+    // Noise is attached to gates, so fusion would move it: off when noisy.
+    const QuantumCircuit opt = noisy ? circuit : maybe_fuse(circuit);
+
+    bool seen_measure = false, per_shot = noisy;
+    for (const Instruction& in : opt.instructions()) {
+        if (in.gate.type == GateType::Measure) seen_measure = true;
+        else if (in.gate.type == GateType::Reset) per_shot = true;
+        else if (seen_measure && in.gate.is_unitary()) per_shot = true;
+    }
+    if (!seen_measure)
+        throw std::invalid_argument("run: circuit has no measurements; use run_statevector() to inspect the final state");
+
+    Rng rng(options_.seed);
+    const std::size_t ncl = opt.num_clbits();
+    std::map<std::string, std::size_t> counts;
+
+    if (!per_shot) {
+        // run_sampling (:88-119): last Measure on a clbit wins.
+        std::vector<std::optional<qubit_t>> src(ncl);
+        for (const Instruction& in : opt.instructions())
+            if (in.gate.type == GateType::Measure) src[in.clbits[0]] = in.qubits[0];
+        const Statevector sv = evolve(opt, options_.engine);
+        std::vector<double> draws(shots);
+        for (double& u : draws) u = rng.uniform();
+        std::vector<uint64_t> idx(shots);
+        check(lqcs_state_sample(sv.device(), draws.data(), shots, idx.data()));
+        for (uint64_t i : idx) ++counts[key_of(src, i)];
+        return Result(std::move(counts), shots);
+    }
+
+    // Per-shot trajectories (:156-190).
+    detail::ProgramBuilder b;
+    std::vector<lqcs_traj_step> steps;
+    std::vector<uint32_t> noise_qubits;
+    uint64_t D = 0;
+    const uint32_t nch = static_cast<uint32_t>(options_.noise.channels().size());
+    for (const Instruction& in : opt.instructions()) {
+        lqcs_traj_step st{};
+        if (in.gate.type == GateType::Measure) {
+            st.kind = LQCS_TOP_MEASURE;
+            st.qubit = in.qubits[0];
+            st.clbit = in.clbits[0];
+            D += 1;
+        } else if (in.gate.type == GateType::Reset) {
+            st.kind = LQCS_TOP_RESET;
+            st.qubit = in.qubits[0];
+            D += 1;
+        } else if (in.gate.type == GateType::Barrier) {
+            continue;
+        } else {
+            st.kind = LQCS_TOP_UNITARY;
+            st.op_index = b.add_instruction(in);
+            st.noise_qubit_offset = static_cast<uint32_t>(noise_qubits.size());
+            st.n_noise_qubits = nch ? static_cast<uint32_t>(in.qubits.size()) : 0;
+            if (nch) noise_qubits.insert(noise_qubits.end(), in.qubits.begin(), in.qubits.end());
+            D += uint64_t(nch) * st.n_noise_qubits;
+        }
+        steps.push_back(st);
+    }
+    std::vector<double> draws(D * shots);
+    for (double& u : draws) u = rng.uniform();
+    std::vector<std::vector<double>> kraus(nch);
+    std::vector<lqcs_channel> chans(nch);
+    for (uint32_t c = 0; c < nch; ++c) {
+        for (const auto& k : options_.noise.channels()[c].ops)
+            for (const complex_t& z : k) {
+                kraus[c].push_back(z.real());
+                kraus[c].push_back(z.imag());
+            }
+        chans[c].n_ops = static_cast<uint32_t>(options_.noise.channels()[c].ops.size());
+        chans[c].ops = kraus[c].data();
+    }
+    std::vector<uint8_t> clbits(shots * ncl);
+    const lqcs_program prog = b.program();
+    check(lqcs_trajectories_run(static_cast<uint32_t>(opt.num_qubits()), static_cast<uint32_t>(ncl), &prog,
+                                steps.data(), steps.size(), noise_qubits.data(), chans.data(), nch, draws.data(),
+                                D, shots, options_.engine.max_batch, clbits.data()));
+    std::string key(ncl, '0');
+    for (std::size_t s = 0; s < shots; ++s) {
+        for (std::size_t c = 0; c < ncl; ++c) key[ncl - 1 - c] = clbits[s * ncl + c] ? '1' : '0';
+        ++counts[key];
+    }
+    return Result(std::move(counts), shots);
+}
+
+// ---- CompiledCircuit ---------------------------------------------------------------
+CompiledCircuit::CompiledCircuit(const QuantumCircuit& circuit, bool fuse_gates)
+    : prog_(std::make_shared<detail::ProgramBuilder>()), nq_(circuit.num_qubits()) {
+    const QuantumCircuit c = fuse_gates ? SingleQubitGateFusion().run(circuit) : circuit;
+    for (const Instruction& in : c.instructions()) {
+        if (in.gate.type == GateType::Reset)
+            throw std::invalid_argument("run_statevector: Reset is non-deterministic; use run() instead");
+        if (in.gate.is_unitary()) prog_->add_instruction(in);
+    }
+}
+
+CompiledCircuit::~CompiledCircuit() = default;
+
+std::size_t CompiledCircuit::num_ops() const { return prog_->size(); }
+
+void CompiledCircuit::apply(Statevector& sv, const EngineOptions& engine) const {
+    if (sv.num_qubits() != nq_) throw std::invalid_argument("apply: register width differs from the circuit");
+    const lqcs_program prog = prog_->program();
+    const lqcs_exec_options o = exec_options(engine);
+    check(lqcs_state_apply(sv.device(), &prog, &o));
+    sv.mark_device_newer();
+}
+
+void synchronize() { check(lqcs_dev_synchronize()); }
+
+void profile_enable(bool on) { check(lqcs_profile_enable(on ? 1 : 0)); }
+
+LaunchProfile profile_read() {
+    LaunchProfile p;
+    check(lqcs_profile_read(&p.tile_launches, &p.tile_ms, &p.other_launches, &p.other_ms));
+    return p;
+}
+
+// ---- greedy single-qubit fusion (src/transpiler/gate_fusion.cpp:32-68) ----------------
+QuantumCircuit SingleQubitGateFusion::run(const QuantumCircuit& circuit) const {
+    using M2 = std::array<complex_t, 4>;
+    const std::size_t nq = circuit.num_qubits();
+    QuantumCircuit out(nq, circuit.num_clbits());
+    std::vector<std::optional<M2>> pending(nq);
+    auto near_identity = [](const M2& m) {
+        return std::abs(m[0] - 1.0) < 1e-12 && std::abs(m[1]) < 1e-12 && std::abs(m[2]) < 1e-12 &&
+               std::abs(m[3] - 1.0) < 1e-12;
+    };
+    auto flush = [&](qubit_t q) {
+        if (!pending[q]) return;
+        if (!near_identity(*pending[q]))
+            out.append({Gate{GateType::Unitary, {}, 0, {pending[q]->begin(), pending[q]->end()}}, {q}, {}});
+        pending[q].reset();
+    };
+    for (const Instruction& in : circuit.instructions()) {
+        const Gate& g = in.gate;
+        if (g.is_unitary() && g.n_controls == 0 && g.base_qubits() == 1) {
+            const auto m = g.base_matrix();
+            const M2 a{m[0], m[1], m[2], m[3]};
+            std::optional<M2>& acc = pending[in.qubits[0]];
+            if (!acc) {
+                acc = a;
+            } else {  // the later gate multiplies from the left
+                const M2& b = *acc;
+                acc = M2{a[0] * b[0] + a[1] * b[2], a[0] * b[1] + a[1] * b[3], a[2] * b[0] + a[3] * b[2],
+                         a[2] * b[1] + a[3] * b[3]};
+            }
+            continue;
+        }
+        if (g.type == GateType::Barrier) {
+            for (qubit_t q = 0; q < nq; ++q) flush(q);
+        } else {
+            for (qubit_t q : in.qubits) flush(q);
+        }
+        out.append(in);
+    }
+    for (qubit_t q = 0; q < nq; ++q) flush(q);
+    return out;
+}
+
+}  // namespace lqcs

@@ -1,0 +1,374 @@
+// _pylqcs: the Python face of the engine, with the names and call shapes of
+// the reference's pylqcs bindings (python/src/bindings.cpp:31-444) for the
+// statevector path, plus engine extensions (adjoint gradients, sample
+// indices, plan statistics). Long-running calls release the GIL.
+#include <pybind11/complex.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <algorithm>
+#include <optional>
+#include <vector>
+
+#include "lqcs/lqcs.hpp"
+#include "lqcs_device.h"
+
+namespace py = pybind11;
+using namespace py::literals;
+using namespace lqcs;
+
+namespace {
+
+using CArray = py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast>;
+
+std::vector<complex_t> to_matrix(const CArray& a) {
+    const auto* p = static_cast<const complex_t*>(a.data());
+    return {p, p + a.size()};
+}
+
+algorithms::Hamiltonian to_h(const std::vector<std::pair<double, std::string>>& terms) {
+    algorithms::Hamiltonian h;
+    for (const auto& [c, p] : terms) h.push_back({c, p});
+    return h;
+}
+
+EngineOptions::Plan plan_of(const std::string& s) {
+    if (s == "gate_by_gate") return EngineOptions::Plan::GateByGate;
+    if (s == "greedy") return EngineOptions::Plan::Greedy;
+    if (s == "learned") return EngineOptions::Plan::Learned;
+    if (s == "auto") return EngineOptions::Plan::Auto;
+    throw std::invalid_argument("plan must be gate_by_gate, greedy, learned or auto");
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_pylqcs, m) {
+    m.doc() = "MI355X statevector engine with the pylqcs API";
+
+    py::enum_<GateType>(m, "GateType")
+        .value("I", GateType::I).value("X", GateType::X).value("Y", GateType::Y).value("Z", GateType::Z)
+        .value("H", GateType::H).value("S", GateType::S).value("Sdg", GateType::Sdg).value("T", GateType::T)
+        .value("Tdg", GateType::Tdg).value("SX", GateType::SX).value("RX", GateType::RX)
+        .value("RY", GateType::RY).value("RZ", GateType::RZ).value("P", GateType::P).value("U", GateType::U)
+        .value("SWAP", GateType::SWAP).value("iSWAP", GateType::iSWAP).value("RXX", GateType::RXX)
+        .value("RYY", GateType::RYY).value("RZZ", GateType::RZZ)
+        .value("PauliRotation", GateType::PauliRotation).value("Unitary", GateType::Unitary)
+        .value("Permutation", GateType::Permutation).value("Measure", GateType::Measure)
+        .value("Reset", GateType::Reset).value("Barrier", GateType::Barrier);
+
+    py::class_<Gate>(m, "Gate")
+        .def(py::init<GateType, std::vector<double>, std::size_t>(), "type"_a,
+             "params"_a = std::vector<double>{}, "n_controls"_a = 0)
+        .def_readonly("type", &Gate::type)
+        .def_readonly("params", &Gate::params)
+        .def_readonly("n_controls", &Gate::n_controls)
+        .def("num_qubits", &Gate::num_qubits)
+        .def("name", &Gate::name)
+        .def("inverse", &Gate::inverse)
+        .def("matrix", [](const Gate& g) {
+            const auto v = g.matrix();
+            const py::ssize_t d = py::ssize_t{1} << g.num_qubits();
+            py::array_t<std::complex<double>> a({d, d});
+            std::copy(v.begin(), v.end(), static_cast<std::complex<double>*>(a.mutable_data()));
+            return a;
+        })
+        .def("__repr__", [](const Gate& g) { return "<Gate " + g.name() + ">"; });
+
+    auto ri = py::return_value_policy::reference_internal;
+    py::class_<QuantumCircuit>(m, "QuantumCircuit")
+        .def(py::init<std::size_t, std::size_t>(), "num_qubits"_a, "num_clbits"_a = 0)
+        .def("i", &QuantumCircuit::i, ri).def("x", &QuantumCircuit::x, ri).def("y", &QuantumCircuit::y, ri)
+        .def("z", &QuantumCircuit::z, ri).def("h", &QuantumCircuit::h, ri).def("s", &QuantumCircuit::s, ri)
+        .def("sdg", &QuantumCircuit::sdg, ri).def("t", &QuantumCircuit::t, ri)
+        .def("tdg", &QuantumCircuit::tdg, ri).def("sx", &QuantumCircuit::sx, ri)
+        .def("rx", &QuantumCircuit::rx, ri).def("ry", &QuantumCircuit::ry, ri).def("rz", &QuantumCircuit::rz, ri)
+        .def("p", &QuantumCircuit::p, ri).def("u", &QuantumCircuit::u, ri)
+        .def("cx", &QuantumCircuit::cx, ri).def("cy", &QuantumCircuit::cy, ri).def("cz", &QuantumCircuit::cz, ri)
+        .def("ch", &QuantumCircuit::ch, ri).def("cp", &QuantumCircuit::cp, ri)
+        .def("crx", &QuantumCircuit::crx, ri).def("cry", &QuantumCircuit::cry, ri)
+        .def("crz", &QuantumCircuit::crz, ri).def("ccx", &QuantumCircuit::ccx, ri)
+        .def("cswap", &QuantumCircuit::cswap, ri)
+        .def("mcx", [](QuantumCircuit& qc, std::vector<qubit_t> c, qubit_t t) -> QuantumCircuit& { return qc.mcx(c, t); },
+             ri, "controls"_a, "target"_a)
+        .def("mcp",
+             [](QuantumCircuit& qc, double lam, std::vector<qubit_t> c, qubit_t t) -> QuantumCircuit& {
+                 return qc.mcp(lam, c, t);
+             },
+             ri, "lam"_a, "controls"_a, "target"_a)
+        .def("swap", &QuantumCircuit::swap, ri).def("iswap", &QuantumCircuit::iswap, ri)
+        .def("rxx", &QuantumCircuit::rxx, ri).def("ryy", &QuantumCircuit::ryy, ri)
+        .def("rzz", &QuantumCircuit::rzz, ri)
+        .def("rp",
+             [](QuantumCircuit& qc, double th, std::string_view p, std::vector<qubit_t> q) -> QuantumCircuit& {
+                 return qc.rp(th, p, q);
+             },
+             ri, "theta"_a, "pauli"_a, "qubits"_a)
+        .def("unitary",
+             [](QuantumCircuit& qc, const CArray& mat, std::vector<qubit_t> q) -> QuantumCircuit& {
+                 return qc.unitary(to_matrix(mat), q);
+             },
+             ri, "matrix"_a, "qubits"_a)
+        .def("permutation",
+             [](QuantumCircuit& qc, std::vector<std::size_t> table, std::vector<qubit_t> q) -> QuantumCircuit& {
+                 return qc.permutation(table, q);
+             },
+             ri, "table"_a, "qubits"_a)
+        .def("controlled_permutation",
+             [](QuantumCircuit& qc, std::vector<std::size_t> table, std::vector<qubit_t> controls,
+                std::vector<qubit_t> targets) -> QuantumCircuit& {
+                 std::vector<qubit_t> qs = controls;
+                 qs.insert(qs.end(), targets.begin(), targets.end());
+                 return qc.append({Gate{GateType::Permutation, {}, controls.size(), {}, table}, qs, {}});
+             },
+             ri, "table"_a, "controls"_a, "targets"_a)
+        .def("controlled_unitary",
+             [](QuantumCircuit& qc, const CArray& mat, std::vector<qubit_t> controls,
+                std::vector<qubit_t> targets) -> QuantumCircuit& {
+                 std::vector<qubit_t> qs = controls;
+                 qs.insert(qs.end(), targets.begin(), targets.end());
+                 return qc.append({Gate{GateType::Unitary, {}, controls.size(), to_matrix(mat)}, qs, {}});
+             },
+             ri, "matrix"_a, "controls"_a, "targets"_a)
+        .def("measure", &QuantumCircuit::measure, ri)
+        .def("measure_all", &QuantumCircuit::measure_all, ri)
+        .def("reset", &QuantumCircuit::reset, ri)
+        .def("barrier", &QuantumCircuit::barrier, ri)
+        .def("compose",
+             [](QuantumCircuit& qc, const QuantumCircuit& o, std::vector<qubit_t> map) -> QuantumCircuit& {
+                 return qc.compose(o, map);
+             },
+             ri, "other"_a, "qubit_map"_a = std::vector<qubit_t>{})
+        .def("inverse", &QuantumCircuit::inverse)
+        .def("power", &QuantumCircuit::power)
+        .def("num_qubits", &QuantumCircuit::num_qubits)
+        .def("num_clbits", &QuantumCircuit::num_clbits)
+        .def("num_instructions", &QuantumCircuit::num_instructions)
+        .def("depth", &QuantumCircuit::depth)
+        .def("draw", &QuantumCircuit::draw)
+        .def("__str__", &QuantumCircuit::draw)
+        .def("__repr__", [](const QuantumCircuit& qc) {
+            return "<QuantumCircuit qubits=" + std::to_string(qc.num_qubits()) +
+                   " clbits=" + std::to_string(qc.num_clbits()) +
+                   " instructions=" + std::to_string(qc.num_instructions()) + ">";
+        });
+
+    py::class_<Statevector>(m, "Statevector")
+        .def(py::init<std::size_t>(), "num_qubits"_a)
+        .def("num_qubits", &Statevector::num_qubits)
+        .def("norm", &Statevector::norm)
+        .def("__len__", &Statevector::size)
+        .def("__getitem__",
+             [](const Statevector& sv, std::size_t i) {
+                 if (i >= sv.size()) throw py::index_error();
+                 return static_cast<std::complex<double>>(sv[i]);
+             })
+        .def("probabilities",
+             [](const Statevector& sv) {
+                 const auto p = sv.probabilities();
+                 py::array_t<double> a(static_cast<py::ssize_t>(p.size()));
+                 std::copy(p.begin(), p.end(), a.mutable_data());
+                 return a;
+             })
+        .def("expectation_value", &Statevector::expectation_value, "pauli"_a)
+        .def("set_basis", &Statevector::set_basis, "index"_a)
+        .def("to_numpy", [](py::object self) {
+            auto& sv = self.cast<Statevector&>();
+            return py::array_t<std::complex<double>>({static_cast<py::ssize_t>(sv.size())},
+                                                     {static_cast<py::ssize_t>(sizeof(complex_t))},
+                                                     reinterpret_cast<std::complex<double>*>(sv.data()), self);
+        });
+
+    py::class_<Result>(m, "Result")
+        .def("counts", &Result::counts)
+        .def("probabilities", &Result::probabilities)
+        .def("shots", &Result::shots)
+        .def("__repr__", [](const Result& r) {
+            std::string s = "<Result shots=" + std::to_string(r.shots()) + " {";
+            bool first = true;
+            for (const auto& [k, v] : r.counts()) {
+                s += (first ? "'" : ", '") + k + "': " + std::to_string(v);
+                first = false;
+            }
+            return s + "}>";
+        });
+
+    py::class_<KrausChannel>(m, "KrausChannel")
+        .def(py::init([](const std::vector<CArray>& ops) {
+                 std::vector<std::array<complex_t, 4>> k;
+                 for (const auto& a : ops) {
+                     if (a.size() != 4) throw std::invalid_argument("KrausChannel: operators must be 2x2");
+                     const auto v = to_matrix(a);
+                     k.push_back({v[0], v[1], v[2], v[3]});
+                 }
+                 return KrausChannel(std::move(k));
+             }),
+             "ops"_a)
+        .def_static("depolarizing", &KrausChannel::depolarizing, "p"_a)
+        .def_static("amplitude_damping", &KrausChannel::amplitude_damping, "gamma"_a)
+        .def_static("phase_damping", &KrausChannel::phase_damping, "gamma"_a)
+        .def_static("bit_flip", &KrausChannel::bit_flip, "p"_a)
+        .def_static("phase_flip", &KrausChannel::phase_flip, "p"_a)
+        .def("ops", [](const KrausChannel& k) {
+            std::vector<std::vector<std::complex<double>>> out;
+            for (const auto& o : k.ops) out.push_back({o[0], o[1], o[2], o[3]});
+            return out;
+        });
+
+    py::class_<NoiseModel>(m, "NoiseModel")
+        .def(py::init<>())
+        .def("add_all_qubit_channel", &NoiseModel::add_all_qubit_channel, ri, "channel"_a)
+        .def("empty", &NoiseModel::empty);
+
+    py::class_<StatevectorSimulator>(m, "StatevectorSimulator")
+        .def(py::init([](std::optional<std::uint64_t> seed, bool fuse_gates, int num_threads, NoiseModel noise,
+                         const std::string& plan, int tile_qubits, int low_qubits, std::size_t max_qubits,
+                         std::uint32_t max_batch) {
+                 StatevectorSimulator::Options o;
+                 o.seed = seed;
+                 o.fuse_gates = fuse_gates;
+                 o.num_threads = num_threads;
+                 o.noise = std::move(noise);
+                 o.engine.plan = plan_of(plan);
+                 o.engine.tile_qubits = tile_qubits;
+                 o.engine.low_qubits = low_qubits;
+                 o.engine.max_qubits = max_qubits;
+                 o.engine.max_batch = max_batch;
+                 return StatevectorSimulator(std::move(o));
+             }),
+             "seed"_a = py::none(), "fuse_gates"_a = true, "num_threads"_a = 0, "noise"_a = NoiseModel{},
+             "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0, "max_qubits"_a = 30, "max_batch"_a = 0)
+        .def("run", &StatevectorSimulator::run, "circuit"_a, "shots"_a = 1024,
+             py::call_guard<py::gil_scoped_release>())
+        .def("run_statevector", &StatevectorSimulator::run_statevector, "circuit"_a,
+             py::call_guard<py::gil_scoped_release>())
+        .def("sample_indices",
+             [](const StatevectorSimulator& s, const QuantumCircuit& c, std::size_t shots) {
+                 std::vector<std::uint64_t> v;
+                 {
+                     py::gil_scoped_release nogil;
+                     v = s.sample_indices(c, shots);
+                 }
+                 py::array_t<std::uint64_t> a(static_cast<py::ssize_t>(v.size()));
+                 std::copy(v.begin(), v.end(), a.mutable_data());
+                 return a;
+             },
+             "circuit"_a, "shots"_a = 1024)
+        .def("name", &StatevectorSimulator::name);
+
+    py::class_<ParametricCircuit>(m, "ParametricCircuit")
+        .def(py::init<std::size_t, std::size_t>(), "num_qubits"_a, "num_clbits"_a = 0)
+        .def("x", &ParametricCircuit::x, ri).def("y", &ParametricCircuit::y, ri)
+        .def("z", &ParametricCircuit::z, ri).def("h", &ParametricCircuit::h, ri)
+        .def("s", &ParametricCircuit::s, ri).def("sdg", &ParametricCircuit::sdg, ri)
+        .def("t", &ParametricCircuit::t, ri).def("sx", &ParametricCircuit::sx, ri)
+        .def("cx", &ParametricCircuit::cx, ri).def("cy", &ParametricCircuit::cy, ri)
+        .def("cz", &ParametricCircuit::cz, ri).def("swap", &ParametricCircuit::swap, ri)
+        .def("ccx", &ParametricCircuit::ccx, ri).def("barrier", &ParametricCircuit::barrier, ri)
+        .def("rx", &ParametricCircuit::rx, "qubit"_a)
+        .def("ry", &ParametricCircuit::ry, "qubit"_a)
+        .def("rz", &ParametricCircuit::rz, "qubit"_a)
+        .def("rp", [](ParametricCircuit& pc, std::string_view p, std::vector<qubit_t> q) { return pc.rp(p, q); },
+             "pauli"_a, "qubits"_a)
+        .def("bind", [](const ParametricCircuit& pc, std::vector<double> v) { return pc.bind(v); }, "values"_a)
+        .def("num_parameters", &ParametricCircuit::num_parameters)
+        .def("num_qubits", &ParametricCircuit::num_qubits)
+        .def("num_clbits", &ParametricCircuit::num_clbits);
+
+    auto alg = m.def_submodule("algorithms", "quantum algorithms on the statevector engine");
+    alg.def("qft", &algorithms::qft, "n"_a, "do_swaps"_a = true);
+    alg.def("qft_inverse", &algorithms::qft_inverse, "n"_a, "do_swaps"_a = true);
+    alg.def(
+        "expectation",
+        [](const Statevector& sv, const std::vector<std::pair<double, std::string>>& terms) {
+            return algorithms::expectation(sv, to_h(terms));
+        },
+        "statevector"_a, "hamiltonian"_a);
+    alg.def(
+        "parameter_shift_gradient",
+        [](const std::vector<std::pair<double, std::string>>& terms, const ParametricCircuit& ansatz,
+           std::vector<double> params) {
+            const auto h = to_h(terms);
+            py::gil_scoped_release nogil;
+            return algorithms::parameter_shift_gradient(h, ansatz, params);
+        },
+        "hamiltonian"_a, "ansatz"_a, "params"_a);
+    alg.def(
+        "adjoint_gradient",
+        [](const std::vector<std::pair<double, std::string>>& terms, const ParametricCircuit& ansatz,
+           std::vector<double> params) {
+            const auto h = to_h(terms);
+            double e = 0.0;
+            std::vector<double> g;
+            {
+                py::gil_scoped_release nogil;
+                g = algorithms::adjoint_gradient(h, ansatz, params, &e);
+            }
+            return py::make_tuple(e, g);
+        },
+        "hamiltonian"_a, "ansatz"_a, "params"_a, "returns (energy, gradient)");
+    alg.def(
+        "adjoint_gradient_batched",
+        [](const std::vector<std::vector<std::pair<double, std::string>>>& hams,
+           const std::vector<const ParametricCircuit*>& ansatze, const std::vector<std::vector<double>>& params) {
+            if (hams.size() != ansatze.size() || params.size() != ansatze.size())
+                throw std::invalid_argument("adjoint_gradient_batched: list lengths differ");
+            std::vector<algorithms::Hamiltonian> hs;
+            for (const auto& t : hams) hs.push_back(to_h(t));
+            std::vector<algorithms::GradientJob> jobs;
+            for (std::size_t i = 0; i < ansatze.size(); ++i) jobs.push_back({ansatze[i], &hs[i], params[i]});
+            std::vector<algorithms::GradientResult> r;
+            {
+                py::gil_scoped_release nogil;
+                r = algorithms::adjoint_gradient_batched(jobs);
+            }
+            py::list out;
+            for (const auto& x : r) out.append(py::make_tuple(x.energy, x.grad));
+            return out;
+        },
+        "hamiltonians"_a, "ansatze"_a, "params"_a, "list of (energy, gradient)");
+
+    py::class_<CompiledCircuit>(m, "CompiledCircuit")
+        .def(py::init<const QuantumCircuit&, bool>(), "circuit"_a, "fuse_gates"_a = true)
+        .def("apply",
+             [](const CompiledCircuit& c, Statevector& sv, const std::string& plan, int tile_qubits,
+                int low_qubits) {
+                 EngineOptions e;
+                 e.plan = plan_of(plan);
+                 e.tile_qubits = tile_qubits;
+                 e.low_qubits = low_qubits;
+                 e.max_qubits = LQCS_DEVICE_MAX_QUBITS;
+                 py::gil_scoped_release nogil;
+                 c.apply(sv, e);
+             },
+             "statevector"_a, "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0)
+        .def("num_qubits", &CompiledCircuit::num_qubits)
+        .def("num_ops", &CompiledCircuit::num_ops);
+    m.def("statevector", [](std::size_t n) { return Statevector(n, LQCS_DEVICE_MAX_QUBITS); }, "num_qubits"_a,
+          "register of up to MAX_DEVICE_QUBITS qubits (beyond the 30-qubit reference cap)");
+    m.def("synchronize", [] {
+        py::gil_scoped_release nogil;
+        synchronize();
+    });
+    m.def("profile_enable", &profile_enable, "on"_a);
+    m.def("profile_read", [] {
+        const LaunchProfile p = profile_read();
+        return py::dict("tile_launches"_a = p.tile_launches, "tile_ms"_a = p.tile_ms,
+                        "other_launches"_a = p.other_launches, "other_ms"_a = p.other_ms);
+    });
+
+    m.def("device_info", [] {
+        char name[256] = {0};
+        int cus = 0;
+        std::uint64_t hbm = 0;
+        if (lqcs_dev_info(name, sizeof name, &cus, &hbm) != LQCS_OK)
+            throw std::runtime_error(lqcs_dev_last_error());
+        return py::dict("name"_a = std::string(name), "compute_units"_a = cus, "hbm_bytes"_a = hbm);
+    });
+    m.def("last_plan_stats", [] {
+        std::uint64_t passes = 0, launches = 0;
+        lqcs_last_plan_stats(&passes, &launches);
+        return py::dict("passes"_a = passes, "launches"_a = launches);
+    });
+    m.attr("MAX_DEVICE_QUBITS") = LQCS_DEVICE_MAX_QUBITS;
+}

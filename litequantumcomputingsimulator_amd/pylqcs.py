@@ -1,0 +1,79 @@
+"""pylqcs: Python face of the engine (the reference's python/pylqcs/__init__.py
+re-exports, backed here by the HIP engine), plus a torch autograd bridge."""
+from __future__ import annotations
+
+from . import _load
+
+_native = _load()
+
+GateType = _native.GateType
+Gate = _native.Gate
+QuantumCircuit = _native.QuantumCircuit
+Statevector = _native.Statevector
+Result = _native.Result
+KrausChannel = _native.KrausChannel
+NoiseModel = _native.NoiseModel
+StatevectorSimulator = _native.StatevectorSimulator
+ParametricCircuit = _native.ParametricCircuit
+algorithms = _native.algorithms
+device_info = _native.device_info
+last_plan_stats = _native.last_plan_stats
+MAX_DEVICE_QUBITS = _native.MAX_DEVICE_QUBITS
+CompiledCircuit = _native.CompiledCircuit
+statevector = _native.statevector
+synchronize = _native.synchronize
+profile_enable = _native.profile_enable
+profile_read = _native.profile_read
+
+__all__ = [
+    "GateType", "Gate", "QuantumCircuit", "Statevector", "Result", "KrausChannel",
+    "NoiseModel", "StatevectorSimulator", "ParametricCircuit", "algorithms",
+    "device_info", "last_plan_stats", "MAX_DEVICE_QUBITS", "CompiledCircuit",
+    "statevector", "synchronize", "profile_enable", "profile_read", "ExpectationFunction",
+    "expectation",
+]
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+class _Expectation:
+    """Lazily built torch.autograd.Function: forward = <H>(theta) on the
+    engine, backward = adjoint gradient from the same call."""
+
+    _fn = None
+
+    @classmethod
+    def get(cls):
+        if cls._fn is None:
+            torch = _torch()
+
+            class ExpectationFunction(torch.autograd.Function):
+                @staticmethod
+                def forward(ctx, theta, ansatz, hamiltonian):
+                    params = theta.detach().to("cpu", torch.float64).tolist()
+                    energy, grad = algorithms.adjoint_gradient(hamiltonian, ansatz, params)
+                    ctx.save_for_backward(torch.tensor(grad, dtype=torch.float64, device=theta.device))
+                    return torch.tensor(energy, dtype=theta.dtype, device=theta.device)
+
+                @staticmethod
+                def backward(ctx, grad_out):
+                    (g,) = ctx.saved_tensors
+                    return grad_out * g.to(grad_out.dtype), None, None
+
+            cls._fn = ExpectationFunction
+        return cls._fn
+
+
+def ExpectationFunction():
+    """The torch.autograd.Function class (built on first use)."""
+    return _Expectation.get()
+
+
+def expectation(theta, ansatz, hamiltonian):
+    """Differentiable <psi(theta)|H|psi(theta)> for a ParametricCircuit;
+    hamiltonian = [(coeff, pauli), ...]."""
+    return _Expectation.get().apply(theta, ansatz, hamiltonian)

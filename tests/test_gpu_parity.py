@@ -1,0 +1,243 @@
+"""Parity of the HIP engine (through pylqcs -> C++20 host -> C-ABI) with the
+CPU oracle on the same seeded inputs. Amplitudes: max-abs <= 1e-12.
+Sampled indices: identical except draws whose scaled value lies within a
+rounding margin of a cumulative boundary ("ambiguous", expected none)."""
+import math
+
+import numpy as np
+import pytest
+
+from litequantumcomputingsimulator_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def random_circuit(n, n_gates, seed):
+    """16 gate kinds as in the reference's tests/test_equivalence.cpp:47-93
+    (kinds and arities; the draw sequence is numpy's)."""
+    rng = np.random.default_rng(seed)
+    c = O.Circuit(n)
+
+    def pick(k):
+        return [int(q) for q in rng.permutation(n)[:k]]
+
+    for _ in range(n_gates):
+        k = int(rng.integers(0, 16))
+        a = float(rng.uniform(0, 6.28))
+        if k == 0: c.h(pick(1)[0])
+        elif k == 1: c.x(pick(1)[0])
+        elif k == 2: c.t(pick(1)[0])
+        elif k == 3: c.sx(pick(1)[0])
+        elif k == 4: c.rx(a, pick(1)[0])
+        elif k == 5: c.u(a, float(rng.uniform(0, 6.28)), float(rng.uniform(0, 6.28)), pick(1)[0])
+        elif k == 6: c.cx(*pick(2))
+        elif k == 7: c.cp(a, *pick(2))
+        elif k == 8: c.swap(*pick(2))
+        elif k == 9: c.iswap(*pick(2))
+        elif k == 10: c.rxx(a, *pick(2))
+        elif k == 11: c.rzz(a, *pick(2))
+        elif k == 12: c.ccx(*pick(3))
+        elif k == 13: c.cswap(*pick(3))
+        elif k == 14:
+            q = pick(3)
+            c.mcp(a, q[:2], q[2])
+        else:
+            c.permutation([int(v) for v in rng.permutation(4)], pick(2))
+    return c
+
+
+def product_state(lq, circ, fuse=True, **eng):
+    sim = lq.StatevectorSimulator(fuse_gates=fuse, **eng)
+    return np.array(sim.run_statevector(circ.to_product(lq)).to_numpy())
+
+
+PLANS = [dict(plan="gate_by_gate"), dict(plan="greedy"), dict(plan="greedy", tile_qubits=6, low_qubits=2)]
+
+
+@pytest.mark.parametrize("seed", [11, 22, 33])
+@pytest.mark.parametrize("eng", PLANS)
+def test_random_circuits_match_oracle(lq, seed, eng):
+    n = 7
+    c = random_circuit(n, 60, seed)
+    want = O.run_statevector(c, fuse=True)
+    got = product_state(lq, c, True, **eng)
+    assert np.max(np.abs(got - want)) <= TOL
+
+
+@pytest.mark.parametrize("n", [6, 9, 12, 14])
+def test_extra_gates_match_oracle(lq, n):
+    rng = np.random.default_rng(n)
+    c = O.Circuit(n)
+    for q in range(n):
+        c.h(q)
+    c.rp(0.7, "XYZ", [0, n - 1, 2]).rp(-1.1, "ZZ", [1, 3]).rp(0.4, "YIX", [n - 2, 0, 4])
+    c.ch(0, n - 1).cry(0.3, n - 1, 1).crz(0.9, 2, 3).crx(0.2, 1, 0).cy(3, 4)
+    c.ryy(0.5, 0, n - 1).rzz(0.25, 1, 2).rxx(0.75, n - 1, 3)
+    c.mcx([0, 1, 2], n - 1).mcp(0.6, [n - 1, 1], 0).s(1).sdg(2).tdg(3).y(4).z(5).i(0).p(0.3, 2).ry(0.8, 1)
+    m = np.linalg.qr(rng.standard_normal((8, 8)) + 1j * rng.standard_normal((8, 8)))[0]
+    c.unitary(m, [0, n - 1, 2])
+    m2 = np.linalg.qr(rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4)))[0]
+    c.controlled_unitary(m2, [n - 1], [0, 1])
+    c.controlled_permutation([1, 2, 3, 0], [n - 1], [2, 3])
+    c.permutation([int(v) for v in rng.permutation(8)], [1, n - 1, 3])
+    want = O.run_statevector(c, fuse=False)
+    for eng in PLANS[:2]:
+        got = product_state(lq, c, False, **eng)
+        assert np.max(np.abs(got - want)) <= TOL, eng
+
+
+def test_qft20_from_basis_matches_oracle_and_dft(lq):
+    n = 20
+    c = O.Circuit(n)
+    x = W.qft_from_basis(c, n, seed=7)
+    want = O.run_statevector(c, fuse=False)
+    got = product_state(lq, c, False)
+    assert np.max(np.abs(got - want)) <= TOL
+    k = np.arange(1 << n)
+    dft = np.exp(2j * np.pi * ((x * k) % (1 << n)) / (1 << n)) / math.sqrt(1 << n)
+    assert np.max(np.abs(got - dft)) <= TOL
+
+
+@pytest.mark.parametrize("n,layers", [(16, 3), (20, 4)])
+def test_random_layers_greedy_vs_oracle(lq, n, layers):
+    c = O.Circuit(n)
+    W.random_u3_cx_layers(c, n, layers, seed=n)
+    want = O.run_statevector(c, fuse=True)
+    for eng in (dict(plan="greedy"), dict(plan="greedy", tile_qubits=10, low_qubits=3),
+                dict(plan="gate_by_gate")):
+        got = product_state(lq, c, True, **eng)
+        assert np.max(np.abs(got - want)) <= TOL, eng
+
+
+def _ambiguous(cum, draws, margin):
+    total = cum[-1]
+    r = draws * total
+    j = np.searchsorted(cum, r, side="right")
+    lo = np.where(j > 0, cum[np.maximum(j - 1, 0)], 0.0)
+    hi = np.where(j < len(cum), cum[np.minimum(j, len(cum) - 1)], np.inf)
+    return (np.abs(r - lo) < margin) | (np.abs(hi - r) < margin)
+
+
+def test_qft20_sampling_indices_match_oracle(lq):
+    n, shots, seed = 20, 1024, 2024
+    c = O.Circuit(n)
+    W.qft_from_basis(c, n, seed=3)
+    c.measure_all()
+    want_idx, cum = O.run_sampling(c, seed, shots, fuse=False)
+    sim = lq.StatevectorSimulator(seed=seed, fuse_gates=False)
+    got_idx = np.array(sim.sample_indices(c.to_product(lq), shots))
+    draws = O.uniform_stream(seed, shots)
+    amb = _ambiguous(cum, draws, 1e-12)
+    assert int(amb.sum()) == 0
+    assert np.array_equal(got_idx[~amb], want_idx[~amb])
+
+
+@pytest.mark.parametrize("fuse", [True, False])
+def test_run_counts_match_oracle(lq, fuse):
+    c = O.Circuit(3, 3)
+    c.h(0).t(0).cx(0, 1).rx(0.4, 2).sx(2).cx(1, 2).measure_all()
+    want = O.run_counts(c, seed=5, shots=512, fuse=fuse)
+    got = lq.StatevectorSimulator(seed=5, fuse_gates=fuse).run(c.to_product(lq), 512).counts()
+    assert dict(got) == want
+
+
+def test_midcircuit_counts_match_oracle(lq):
+    c = O.Circuit(2, 2)
+    c.h(0).t(0).measure(0, 0).h(1).cx(0, 1).reset(0).h(0).measure(1, 1)
+    want = O.run_counts(c, seed=9, shots=256)
+    got = lq.StatevectorSimulator(seed=9).run(c.to_product(lq), 256).counts()
+    assert dict(got) == want
+
+
+def test_noisy_counts_match_oracle(lq):
+    c = O.Circuit(3, 3)
+    c.h(0).cx(0, 1).t(1).cx(1, 2).u(0.3, 0.2, 0.1, 2).measure_all()
+    chans = [O.depolarizing(0.1), O.amplitude_damping(0.2)]
+    want = O.run_counts(c, seed=99, shots=400, channels=chans)
+    noise = lq.NoiseModel()
+    noise.add_all_qubit_channel(lq.KrausChannel.depolarizing(0.1))
+    noise.add_all_qubit_channel(lq.KrausChannel.amplitude_damping(0.2))
+    got = lq.StatevectorSimulator(seed=99, noise=noise).run(c.to_product(lq), 400).counts()
+    assert dict(got) == want
+
+
+def test_expectations_match_oracle(lq):
+    n = 10
+    c = random_circuit(n, 50, 5)
+    st = O.run_statevector(c)
+    sv = lq.StatevectorSimulator().run_statevector(c.to_product(lq))
+    h = W.random_hamiltonian(n, 30, seed=1)
+    for coeff, p in h[:10]:
+        assert abs(sv.expectation_value(p) - O.expectation_value(st, n, p)) <= TOL
+    assert abs(lq.algorithms.expectation(sv, h) - O.expectation(st, n, h)) <= 1e-11
+    assert abs(sv.norm() - 1.0) <= 1e-12
+
+
+def _ansatz(lq, n, n_rot, seed):
+    rots, theta = W.random_pauli_ansatz(n, n_rot, seed)
+    pc = lq.ParametricCircuit(n)
+    t = O.Circuit(n)
+    for q in range(n):
+        pc.h(q)
+        t.h(q)
+    param_inst = []
+    for p, qs in rots:
+        pc.rp(p, qs)
+        param_inst.append(len(t.insts))
+        t.rp(0.0, p, qs)
+    pc.cx(0, 1)
+    t.cx(0, 1)
+    pc.ry(2)
+    param_inst.append(len(t.insts))
+    t.ry(0.0, 2)
+    theta = list(theta) + [0.37]
+    return pc, t, param_inst, theta
+
+
+def test_gradients_match_oracle(lq):
+    n = 8
+    pc, t, pinst, theta = _ansatz(lq, n, 12, seed=4)
+    h = W.random_hamiltonian(n, 15, seed=2)
+    want = O.parameter_shift_gradient(t, pinst, theta, h)
+    ps = np.array(lq.algorithms.parameter_shift_gradient(h, pc, theta))
+    e, adj = lq.algorithms.adjoint_gradient(h, pc, theta)
+    assert np.max(np.abs(ps - want)) <= 1e-10
+    assert np.max(np.abs(np.array(adj) - want)) <= 1e-10
+    assert abs(e - O.bound_energy(t, pinst, theta, h)) <= 1e-10
+
+
+def test_batched_adjoint_matches_single(lq):
+    jobs = []
+    for k, n in enumerate([5, 7, 6, 9]):
+        pc, t, pinst, theta = _ansatz(lq, n, 10, seed=10 + k)
+        h = W.random_hamiltonian(n, 8, seed=20 + k)
+        jobs.append((pc, t, pinst, theta, h))
+    out = lq.algorithms.adjoint_gradient_batched([j[4] for j in jobs], [j[0] for j in jobs],
+                                                 [j[3] for j in jobs])
+    for (pc, t, pinst, theta, h), (e, g) in zip(jobs, out):
+        want = O.parameter_shift_gradient(t, pinst, theta, h)
+        assert np.max(np.abs(np.array(g) - want)) <= 1e-10
+        assert abs(e - O.bound_energy(t, pinst, theta, h)) <= 1e-10
+
+
+def test_errors_keep_reference_types(lq):
+    with pytest.raises(ValueError):
+        lq.Statevector(31)
+    with pytest.raises(ValueError):
+        lq.Statevector(0)
+    qc = lq.QuantumCircuit(2)
+    qc.h(0)
+    with pytest.raises(ValueError):
+        lq.StatevectorSimulator().run(qc, 10)  # no measurement
+    qm = lq.QuantumCircuit(1, 1)
+    qm.measure(0, 0)
+    with pytest.raises(ValueError):
+        lq.StatevectorSimulator().run(qm, 0)
+    sv = lq.Statevector(2)
+    with pytest.raises(ValueError):
+        sv.expectation_value("Z")
+    with pytest.raises(IndexError):
+        _ = sv[4]
