@@ -367,6 +367,24 @@ def density_probabilities(circ: Circuit, channels=()) -> np.ndarray:
     return np.array([rho[(i << circ.n_qubits) | i].real for i in range(d)])
 
 
+def density_matrix(circ: Circuit, channels=()) -> np.ndarray:
+    """rho as a 2^n x 2^n matrix, rho[r, c] = <r|rho|c>."""
+    lib = oracle_lib()
+    c, keep = circ.to_orc()
+    nz, keep2 = noise_struct(channels)
+    vec = _cbuf(1 << (2 * circ.n_qubits))
+    lib.orc_run_density_matrix(C.byref(c), C.byref(nz) if channels else None, vec.ctypes.data_as(C.POINTER(OrcC)))
+    d = 1 << circ.n_qubits
+    return vec.reshape(d, d).T.copy()  # vector index (col << n) | row
+
+
+def mt19937_64_outputs(seed: int, n: int) -> np.ndarray:
+    lib = oracle_lib()
+    st = (C.c_uint64 * 313)()
+    lib.orc_rng_seed(st, C.c_uint64(seed))
+    return np.array([lib.orc_rng_next(st) for _ in range(n)], dtype=np.uint64)
+
+
 def uniform_stream(seed: int, n: int) -> np.ndarray:
     lib = oracle_lib()
     out = np.zeros(n, dtype=np.float64)
