@@ -294,7 +294,7 @@ void run_single_op(amp_t* s, uint32_t nq, const DecodedOp& d, uint32_t batch) {
 }
 
 // ---- tiled planning ------------------------------------------------------------
-namespace {
+// ---- tile-pass encoding (shared with planner.hip)
 
 // Qubits whose values must vary inside the tile for this op; ~0 when the op
 // cannot run inside a tile at all.
@@ -319,11 +319,11 @@ uint64_t tile_requirement(const DecodedOp& d) {
     }
 }
 
-uint32_t local_pos(uint64_t smask, uint32_t q) {
+static uint32_t local_pos(uint64_t smask, uint32_t q) {
     return static_cast<uint32_t>(__builtin_popcountll(smask & ((uint64_t{1} << q) - 1)));
 }
 
-uint32_t to_local_mask(uint64_t smask, uint64_t global) {
+static uint32_t to_local_mask(uint64_t smask, uint64_t global) {
     uint32_t m = 0;
     for (uint64_t g = global & smask; g; g &= g - 1) m |= 1u << local_pos(smask, __builtin_ctzll(g));
     return m;
@@ -392,7 +392,7 @@ TileOp make_tile_op(const DecodedOp& d, uint64_t smask) {
     return t;
 }
 
-}  // namespace
+
 
 Plan plan_greedy(const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t low) {
     Plan plan;
@@ -738,6 +738,39 @@ lqcs_status lqcs_state_apply(lqcs_dev_state* s, const lqcs_program* prog, const 
             apply_learned(s->d, s->nq, ops, L, low, 1);
         } else {
             apply_ops(s->d, s->nq, ops, plan, L, low, 1);
+        }
+    });
+}
+
+lqcs_status lqcs_plan_describe(uint32_t n_qubits, const lqcs_program* prog, const lqcs_exec_options* opts,
+                               uint64_t* n_steps, uint64_t* ops_per_step, uint64_t* tile_masks,
+                               uint64_t max_steps) {
+    return guarded([&] {
+        need(prog, "program");
+        need(n_steps, "n_steps");
+        if (n_qubits == 0 || n_qubits > LQCS_DEVICE_MAX_QUBITS) throw Error(LQCS_ERR_INVALID_ARGUMENT, "bad width");
+        std::vector<DecodedOp> ops;
+        for (uint64_t k = 0; k < prog->n_ops; ++k) ops.push_back(decode(*prog, k, n_qubits));
+        const int kind = opts ? opts->plan : LQCS_PLAN_AUTO;
+        const TileChoice def = default_tile(n_qubits);
+        uint32_t L = opts && opts->tile_qubits > 0 ? static_cast<uint32_t>(opts->tile_qubits) : def.L;
+        uint32_t low = opts && opts->low_qubits > 0 ? static_cast<uint32_t>(opts->low_qubits) : def.low;
+        Plan plan;
+        if (kind == LQCS_PLAN_GATE_BY_GATE || L == 0) {
+            for (uint32_t i = 0; i < ops.size(); ++i) {
+                PlanStep st;
+                st.op = i;
+                plan.steps.push_back(st);
+            }
+        } else if (kind == LQCS_PLAN_LEARNED) {
+            plan = plan_gain(ops, n_qubits, L, low);
+        } else {
+            plan = plan_greedy(ops, n_qubits, L, low);
+        }
+        *n_steps = plan.steps.size();
+        for (uint64_t i = 0; i < plan.steps.size() && i < max_steps; ++i) {
+            if (ops_per_step) ops_per_step[i] = plan.steps[i].tiled ? plan.steps[i].tile_ops.size() : 1;
+            if (tile_masks) tile_masks[i] = plan.steps[i].tiled ? plan.steps[i].smask : 0;
         }
     });
 }

@@ -48,6 +48,10 @@ struct TileChoice {
 };
 TileChoice default_tile(uint32_t nq);
 
+// Qubits that must vary inside a tile for this op (~0: op cannot be tiled).
+uint64_t tile_requirement(const DecodedOp& d);
+TileOp make_tile_op(const DecodedOp& d, uint64_t smask);
+Plan plan_gain(const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t low);
 Plan plan_greedy(const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t low);
 void execute_plan(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan,
                   const std::vector<DecodedOp>& ops, uint32_t batch);
