@@ -128,6 +128,10 @@ lqcs_status lqcs_plan_describe(uint32_t n_qubits, const lqcs_program* prog, cons
                                uint64_t* n_steps, uint64_t* ops_per_step, uint64_t* tile_masks,
                                uint64_t max_steps);
 
+/* Kernel used for tiled passes: 1 = register-resident tile (default),
+ * 0 = LDS tile. A switch for comparisons; results agree to rounding. */
+lqcs_status lqcs_set_tile_kernel(int kernel);
+
 /* Launch timing (no counterpart in the reference): while enabled, every
  * tiled pass and every gate-by-gate kernel of lqcs_state_apply is bracketed
  * by HIP events on the engine stream. lqcs_profile_read synchronises and

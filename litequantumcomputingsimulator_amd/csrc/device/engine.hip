@@ -44,6 +44,8 @@ Context& ctx() {
 }
 
 thread_local std::string g_last_error;
+// Register tiles on (default) or off (LDS tiles only), for comparisons.
+bool g_use_rtile = true;
 
 // Launch timing state (lqcs_profile_*).
 struct Profile {
@@ -443,16 +445,35 @@ Plan plan_greedy(const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uin
 
 void execute_plan(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan,
                   const std::vector<DecodedOp>& ops, uint32_t batch) {
-    size_t total = 0;
-    for (const auto& st : plan.steps) total += st.tile_ops.size();
+    // Register-tile programs where possible, LDS tile programs otherwise.
     std::vector<TileOp> flat;
-    flat.reserve(total);
-    std::vector<size_t> first;
+    std::vector<RPhase> rph;
+    std::vector<RegOp> rops;
+    struct Where {
+        int kind;  // 0 stand-alone, 1 LDS tile, 2 register tile
+        size_t first, count;
+    };
+    std::vector<Where> where;
     for (const auto& st : plan.steps) {
-        first.push_back(flat.size());
-        flat.insert(flat.end(), st.tile_ops.begin(), st.tile_ops.end());
+        if (!st.tiled) {
+            where.push_back({0, 0, 0});
+            continue;
+        }
+        RTileProgram rp;
+        if (g_use_rtile && build_rtile(st.tile_ops, L, st.smask, rp)) {
+            const size_t op0 = rops.size();
+            for (auto& ph : rp.phases) ph.first_op += static_cast<uint32_t>(op0);
+            where.push_back({2, rph.size(), rp.phases.size()});
+            rph.insert(rph.end(), rp.phases.begin(), rp.phases.end());
+            rops.insert(rops.end(), rp.ops.begin(), rp.ops.end());
+        } else {
+            where.push_back({1, flat.size(), st.tile_ops.size()});
+            flat.insert(flat.end(), st.tile_ops.begin(), st.tile_ops.end());
+        }
     }
     const TileOp* d_ops = flat.empty() ? nullptr : stage(flat.data(), flat.size());
+    const RPhase* d_ph = rph.empty() ? nullptr : stage(rph.data(), rph.size());
+    const RegOp* d_rops = rops.empty() ? nullptr : stage(rops.data(), rops.size());
     for (size_t i = 0; i < plan.steps.size(); ++i) {
         const PlanStep& st = plan.steps[i];
         const bool timed = prof().on;
@@ -462,9 +483,11 @@ void execute_plan(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan,
             e1 = take_event();
             LQCS_HIP(hipEventRecord(e0, stream()));
         }
-        if (st.tiled) {
-            launch_tile_pass(s, nq, L, st.smask, d_ops + first[i],
-                             static_cast<uint32_t>(st.tile_ops.size()), batch);
+        const Where& w = where[i];
+        if (w.kind == 2) {
+            launch_rtile(s, nq, st.smask, d_ph + w.first, static_cast<uint32_t>(w.count), d_rops, batch);
+        } else if (w.kind == 1) {
+            launch_tile_pass(s, nq, L, st.smask, d_ops + w.first, static_cast<uint32_t>(w.count), batch);
         } else {
             run_single_op(s, nq, ops[st.op], batch);
         }
@@ -599,6 +622,13 @@ lqcs_status lqcs_dev_synchronize(void) {
         ensure_ready();
         sync();
         LQCS_HIP(hipDeviceSynchronize());
+    });
+}
+
+lqcs_status lqcs_set_tile_kernel(int kernel) {
+    return guarded([&] {
+        if (kernel != 0 && kernel != 1) throw Error(LQCS_ERR_INVALID_ARGUMENT, "tile kernel must be 0 (LDS) or 1 (register)");
+        g_use_rtile = kernel == 1;
     });
 }
 

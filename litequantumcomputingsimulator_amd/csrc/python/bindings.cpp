@@ -351,6 +351,10 @@ PYBIND11_MODULE(_pylqcs, m) {
         synchronize();
     });
     m.def("profile_enable", &profile_enable, "on"_a);
+    m.def("set_tile_kernel", [](const std::string& k) {
+        if (k != "register" && k != "lds") throw std::invalid_argument("tile kernel must be 'register' or 'lds'");
+        if (lqcs_set_tile_kernel(k == "register" ? 1 : 0) != LQCS_OK) throw std::runtime_error(lqcs_dev_last_error());
+    }, "kernel"_a);
     m.def("profile_read", [] {
         const LaunchProfile p = profile_read();
         return py::dict("tile_launches"_a = p.tile_launches, "tile_ms"_a = p.tile_ms,

@@ -24,12 +24,13 @@ statevector = _native.statevector
 synchronize = _native.synchronize
 profile_enable = _native.profile_enable
 profile_read = _native.profile_read
+set_tile_kernel = _native.set_tile_kernel
 
 __all__ = [
     "GateType", "Gate", "QuantumCircuit", "Statevector", "Result", "KrausChannel",
     "NoiseModel", "StatevectorSimulator", "ParametricCircuit", "algorithms",
     "device_info", "last_plan_stats", "MAX_DEVICE_QUBITS", "CompiledCircuit",
-    "statevector", "synchronize", "profile_enable", "profile_read", "ExpectationFunction",
+    "statevector", "synchronize", "profile_enable", "profile_read", "set_tile_kernel", "ExpectationFunction",
     "expectation",
 ]
 

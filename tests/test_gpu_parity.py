@@ -241,3 +241,19 @@ def test_errors_keep_reference_types(lq):
         sv.expectation_value("Z")
     with pytest.raises(IndexError):
         _ = sv[4]
+
+
+@pytest.mark.parametrize("kernel", ["register", "lds"])
+@pytest.mark.parametrize("plan,low", [("greedy", 4), ("learned", 3), ("learned", 2), ("greedy", 5)])
+def test_tile_kernels_and_planners_match_oracle(lq, kernel, plan, low):
+    n = 17
+    c = O.Circuit(n)
+    W.random_u3_cx_cp(c, n, 5, seed=low)
+    c.rp(0.3, "XZY", [0, 9, 16]).rp(0.9, "ZIZ", [2, 11, 4]).ccx(3, 15, 7).swap(1, 14).ryy(0.4, 5, 12)
+    want = O.run_statevector(c, fuse=True)
+    lq.set_tile_kernel(kernel)
+    try:
+        got = product_state(lq, c, True, plan=plan, tile_qubits=12, low_qubits=low)
+    finally:
+        lq.set_tile_kernel("register")
+    assert np.max(np.abs(got - want)) <= TOL
