@@ -172,6 +172,14 @@ PYBIND11_MODULE(_pylqcs, m) {
              })
         .def("expectation_value", &Statevector::expectation_value, "pauli"_a)
         .def("set_basis", &Statevector::set_basis, "index"_a)
+        .def("copy_from",
+             [](Statevector& dst, const Statevector& src) {
+                 if (dst.num_qubits() != src.num_qubits()) throw std::invalid_argument("copy_from: widths differ");
+                 if (lqcs_state_copy(dst.device(), src.device()) != LQCS_OK)
+                     throw std::runtime_error(lqcs_dev_last_error());
+                 dst.mark_device_newer();
+             },
+             "source"_a)
         .def("to_numpy", [](py::object self) {
             auto& sv = self.cast<Statevector&>();
             return py::array_t<std::complex<double>>({static_cast<py::ssize_t>(sv.size())},
