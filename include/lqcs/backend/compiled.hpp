@@ -5,6 +5,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <memory>
+#include <vector>
 
 #include "../circuit/quantum_circuit.hpp"
 #include "engine.hpp"
@@ -25,6 +26,13 @@ public:
 
     // sv <- U sv, in place on the device.
     void apply(Statevector& sv, const EngineOptions& engine = {}) const;
+
+    // Plan the engine would run (no device needed): per HBM pass, the ops it
+    // applies, its tile qubit mask (0 = stand-alone op) and register phases.
+    struct PassInfo {
+        std::uint64_t ops, tile_mask, phases;
+    };
+    std::vector<PassInfo> describe(const EngineOptions& engine = {}) const;
 
     std::size_t num_qubits() const { return nq_; }
     std::size_t num_ops() const;

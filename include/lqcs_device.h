@@ -122,11 +122,12 @@ lqcs_status lqcs_state_apply(lqcs_dev_state* s, const lqcs_program* prog,
 lqcs_status lqcs_last_plan_stats(uint64_t* passes, uint64_t* launches);
 
 /* Plan of lqcs_state_apply without running it (no device needed): number of
- * steps (HBM passes), ops per step and each step's tile qubit mask (0 for a
- * stand-alone gate-by-gate step); arrays hold max_steps entries. */
+ * steps (HBM passes), ops per step, each step's tile qubit mask (0 for a
+ * stand-alone gate-by-gate step) and its register-tile phase count (0 when
+ * the step runs on another kernel); arrays hold max_steps entries. */
 lqcs_status lqcs_plan_describe(uint32_t n_qubits, const lqcs_program* prog, const lqcs_exec_options* opts,
                                uint64_t* n_steps, uint64_t* ops_per_step, uint64_t* tile_masks,
-                               uint64_t max_steps);
+                               uint64_t* phases_per_step, uint64_t max_steps);
 
 /* Kernel used for tiled passes: 1 = register-resident tile (default),
  * 0 = LDS tile. A switch for comparisons; results agree to rounding. */

@@ -774,7 +774,7 @@ lqcs_status lqcs_state_apply(lqcs_dev_state* s, const lqcs_program* prog, const 
 
 lqcs_status lqcs_plan_describe(uint32_t n_qubits, const lqcs_program* prog, const lqcs_exec_options* opts,
                                uint64_t* n_steps, uint64_t* ops_per_step, uint64_t* tile_masks,
-                               uint64_t max_steps) {
+                               uint64_t* phases_per_step, uint64_t max_steps) {
     return guarded([&] {
         need(prog, "program");
         need(n_steps, "n_steps");
@@ -801,6 +801,13 @@ lqcs_status lqcs_plan_describe(uint32_t n_qubits, const lqcs_program* prog, cons
         for (uint64_t i = 0; i < plan.steps.size() && i < max_steps; ++i) {
             if (ops_per_step) ops_per_step[i] = plan.steps[i].tiled ? plan.steps[i].tile_ops.size() : 1;
             if (tile_masks) tile_masks[i] = plan.steps[i].tiled ? plan.steps[i].smask : 0;
+            if (phases_per_step) {
+                RTileProgram rp;
+                phases_per_step[i] = (plan.steps[i].tiled && g_use_rtile &&
+                                      build_rtile(plan.steps[i].tile_ops, L, plan.steps[i].smask, rp))
+                                         ? rp.phases.size()
+                                         : 0;
+            }
         }
     });
 }

@@ -200,12 +200,7 @@ __device__ __forceinline__ void apply_reg_op(amp_t (&v)[kRR], const RegOp& op, u
     }
 }
 
-// Global offset of register r in the natural layout: r's 4 bits placed on
-// the global positions of tile bits 8..11.
-__device__ __forceinline__ uint64_t nat_gofs(int r, uint64_t hi) { return pdep64(uint64_t(r), hi); }
-
 __global__ void __launch_bounds__(kRT) k_rtile(amp_t* __restrict__ s, uint32_t nq, uint64_t smask,
-                                              uint64_t smask_nat_lo, uint64_t smask_nat_hi,
                                               const RPhase* __restrict__ phases, uint32_t n_phases,
                                               const RegOp* __restrict__ ops) {
     __shared__ amp_t lds[1 << kRL];
@@ -213,44 +208,32 @@ __global__ void __launch_bounds__(kRT) k_rtile(amp_t* __restrict__ s, uint32_t n
     const uint64_t full = (uint64_t{1} << nq) - 1;
     const uint64_t base = pdep64(blockIdx.x, full & ~smask);
     amp_t v[kRR];
-    // Coalesced load in the natural layout (thread id = tile bits 0..7,
-    // register = tile bits 8..11), straight into LDS.
     {
-        const uint64_t gt = base | pdep64(threadIdx.x, smask_nat_lo);
+        const RPhase& p0 = phases[0];
+        const uint64_t gt = base | pdep64(threadIdx.x, p0.tmask_global);
 #pragma unroll
-        for (int r = 0; r < kRR; ++r) v[r] = b[gt | nat_gofs(r, smask_nat_hi)];
-#pragma unroll
-        for (int r = 0; r < kRR; ++r) lds[swz(threadIdx.x | (uint32_t(r) << 8))] = v[r];
+        for (int r = 0; r < kRR; ++r) v[r] = b[gt | p0.gofs[r]];
     }
-    __syncthreads();
     for (uint32_t p = 0; p < n_phases; ++p) {
         const RPhase& ph = phases[p];
         const uint32_t jt = pdep32(threadIdx.x, ph.tmask);
         if (p > 0) {
             const RPhase& pp = phases[p - 1];
             const uint32_t jtp = pdep32(threadIdx.x, pp.tmask);
-            __syncthreads();  // every thread has read the previous layout
+            if (p > 1) __syncthreads();  // previous transpose's reads are done
 #pragma unroll
             for (int r = 0; r < kRR; ++r) lds[swz(jtp | pp.jr[r])] = v[r];
             __syncthreads();
-        }
 #pragma unroll
-        for (int r = 0; r < kRR; ++r) v[r] = lds[swz(jt | ph.jr[r])];
+            for (int r = 0; r < kRR; ++r) v[r] = lds[swz(jt | ph.jr[r])];
+        }
         for (uint32_t o = ph.first_op; o < ph.first_op + ph.n_ops; ++o) apply_reg_op(v, ops[o], jt, ph, base);
     }
-    // Back to the natural layout through LDS, then a coalesced store.
     {
         const RPhase& pl = phases[n_phases - 1];
-        const uint32_t jt = pdep32(threadIdx.x, pl.tmask);
-        __syncthreads();
+        const uint64_t gt = base | pdep64(threadIdx.x, pl.tmask_global);
 #pragma unroll
-        for (int r = 0; r < kRR; ++r) lds[swz(jt | pl.jr[r])] = v[r];
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < kRR; ++r) v[r] = lds[swz(threadIdx.x | (uint32_t(r) << 8))];
-        const uint64_t gt = base | pdep64(threadIdx.x, smask_nat_lo);
-#pragma unroll
-        for (int r = 0; r < kRR; ++r) b[gt | nat_gofs(r, smask_nat_hi)] = v[r];
+        for (int r = 0; r < kRR; ++r) b[gt | pl.gofs[r]] = v[r];
     }
 }
 
@@ -393,14 +376,8 @@ bool build_rtile(const std::vector<TileOp>& tops, uint32_t L, uint64_t smask, RT
 void launch_rtile(amp_t* s, uint32_t nq, uint64_t smask, const RPhase* d_phases, uint32_t n_phases,
                   const RegOp* d_ops, uint32_t batch) {
     const uint64_t tiles = uint64_t{1} << (nq - kRL);
-    uint64_t lo = 0, m = smask;  // lowest 8 tile qubits <-> thread id, next 4 <-> register
-    for (int i = 0; i < 8; ++i) {
-        const uint64_t bit = m & (~m + 1);
-        lo |= bit;
-        m ^= bit;
-    }
-    k_rtile<<<dim3(static_cast<unsigned>(tiles), batch), kRT, 0, stream()>>>(s, nq, smask, lo, m, d_phases,
-                                                                            n_phases, d_ops);
+    k_rtile<<<dim3(static_cast<unsigned>(tiles), batch), kRT, 0, stream()>>>(s, nq, smask, d_phases, n_phases,
+                                                                            d_ops);
 }
 
 }  // namespace lqcs_dev

@@ -300,6 +300,18 @@ void CompiledCircuit::apply(Statevector& sv, const EngineOptions& engine) const 
     sv.mark_device_newer();
 }
 
+std::vector<CompiledCircuit::PassInfo> CompiledCircuit::describe(const EngineOptions& engine) const {
+    const lqcs_program prog = prog_->program();
+    const lqcs_exec_options o = exec_options(engine);
+    uint64_t n = 0;
+    check(lqcs_plan_describe(static_cast<uint32_t>(nq_), &prog, &o, &n, nullptr, nullptr, nullptr, 0));
+    std::vector<uint64_t> ops(n), masks(n), phases(n);
+    check(lqcs_plan_describe(static_cast<uint32_t>(nq_), &prog, &o, &n, ops.data(), masks.data(), phases.data(), n));
+    std::vector<PassInfo> out(n);
+    for (uint64_t i = 0; i < n; ++i) out[i] = {ops[i], masks[i], phases[i]};
+    return out;
+}
+
 void synchronize() { check(lqcs_dev_synchronize()); }
 
 void profile_enable(bool on) { check(lqcs_profile_enable(on ? 1 : 0)); }

@@ -351,7 +351,19 @@ PYBIND11_MODULE(_pylqcs, m) {
              },
              "statevector"_a, "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0)
         .def("num_qubits", &CompiledCircuit::num_qubits)
-        .def("num_ops", &CompiledCircuit::num_ops);
+        .def("num_ops", &CompiledCircuit::num_ops)
+        .def("describe",
+             [](const CompiledCircuit& c, const std::string& plan, int tile_qubits, int low_qubits) {
+                 EngineOptions e;
+                 e.plan = plan_of(plan);
+                 e.tile_qubits = tile_qubits;
+                 e.low_qubits = low_qubits;
+                 py::list out;
+                 for (const auto& p : c.describe(e))
+                     out.append(py::dict("ops"_a = p.ops, "tile_mask"_a = p.tile_mask, "phases"_a = p.phases));
+                 return out;
+             },
+             "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0);
     m.def("statevector", [](std::size_t n) { return Statevector(n, LQCS_DEVICE_MAX_QUBITS); }, "num_qubits"_a,
           "register of up to MAX_DEVICE_QUBITS qubits (beyond the 30-qubit reference cap)");
     m.def("synchronize", [] {
