@@ -44,72 +44,123 @@ __device__ __forceinline__ uint32_t pdep32(uint32_t v, uint32_t mask) {
 }
 
 // LDS slot of tile index j: low 3 bits XOR-folded with higher bits so that
-// lanes differing only in high bits still spread over the LDS banks.
-__device__ __forceinline__ uint32_t swz(uint32_t j) { return j ^ (((j >> 3) ^ (j >> 6) ^ (j >> 9)) & 7u); }
+// lanes differing only in high bits still spread over the LDS banks. The map
+// is linear over XOR, so slot(jt | jr) = slot(jt) ^ slot(jr) for disjoint jt,
+// jr; the host precomputes slot(jr) per register.
+__host__ __device__ __forceinline__ uint32_t swz(uint32_t j) {
+    return j ^ (((j >> 3) ^ (j >> 6) ^ (j >> 9)) & 7u);
+}
 
+using Regs = amp_t[kRR];
+
+// Select on values already loaded (a select between an array element and a
+// computed value would otherwise become a select of addresses and push the
+// register arrays to scratch).
+__device__ __forceinline__ amp_t pick(bool c, amp_t p, amp_t q) {
+    return make_double2(c ? p.x : q.x, c ? p.y : q.y);
+}
+
+__device__ __forceinline__ void copy_regs(const Regs& a, Regs& o) {
+#pragma unroll
+    for (int r = 0; r < kRR; ++r) o[r] = a[r];
+}
+
+// Every op reads register set `a` and writes all of set `o` (ping-pong), so
+// results land in their final registers without copies.
 template <int B>
-__device__ __forceinline__ void r_dense1(amp_t (&v)[kRR], const RegOp& op) {
+__device__ __forceinline__ void d1_all(const Regs& a, Regs& o, const RegOp& op) {
+    const amp_t m0 = op.m[0], m1 = op.m[1], m2 = op.m[2], m3 = op.m[3];
+#pragma unroll
+    for (int r = 0; r < kRR; ++r) {
+        if (r & (1 << B)) continue;
+        const amp_t a0 = a[r], a1 = a[r | (1 << B)];
+        o[r] = cmac2(m0, a0, m1, a1);
+        o[r | (1 << B)] = cmac2(m2, a0, m3, a1);
+    }
+}
+
+// Controlled variant: pairs whose register bits miss a control, or lanes
+// whose thread bits miss one, keep their values (selects, no branches).
+template <int B>
+__device__ __forceinline__ void d1_masked(const Regs& a, Regs& o, const RegOp& op, bool tp) {
     const amp_t m0 = op.m[0], m1 = op.m[1], m2 = op.m[2], m3 = op.m[3];
     const uint32_t rc = op.rctl;
 #pragma unroll
     for (int r = 0; r < kRR; ++r) {
         if (r & (1 << B)) continue;
-        if ((uint32_t(r) & rc) != rc) continue;
-        const amp_t a0 = v[r], a1 = v[r | (1 << B)];
-        v[r] = cmac2(m0, a0, m1, a1);
-        v[r | (1 << B)] = cmac2(m2, a0, m3, a1);
+        const bool on = tp && ((uint32_t(r) & rc) == rc);
+        const amp_t a0 = a[r], a1 = a[r | (1 << B)];
+        const amp_t n0 = cmac2(m0, a0, m1, a1), n1 = cmac2(m2, a0, m3, a1);
+        o[r] = pick(on, n0, a0);
+        o[r | (1 << B)] = pick(on, n1, a1);
     }
 }
 
 template <int B>
-__device__ __forceinline__ void r_x(amp_t (&v)[kRR], const RegOp& op) {
+__device__ __forceinline__ void x_all(const Regs& a, Regs& o) {
+#pragma unroll
+    for (int r = 0; r < kRR; ++r) o[r] = a[r ^ (1 << B)];
+}
+
+// X on register bit B controlled by register bit C (compile-time pairs).
+template <int B, int C>
+__device__ __forceinline__ void x_rc(const Regs& a, Regs& o) {
+#pragma unroll
+    for (int r = 0; r < kRR; ++r) {
+        if (r & (1 << C)) o[r] = a[r ^ (1 << B)];
+        else o[r] = a[r];
+    }
+}
+
+template <int B>
+__device__ __forceinline__ void x_masked(const Regs& a, Regs& o, const RegOp& op, bool tp) {
     const uint32_t rc = op.rctl;
 #pragma unroll
     for (int r = 0; r < kRR; ++r) {
-        if (r & (1 << B)) continue;
-        if ((uint32_t(r) & rc) != rc) continue;
-        const amp_t t = v[r];
-        v[r] = v[r | (1 << B)];
-        v[r | (1 << B)] = t;
+        const bool on = tp && ((uint32_t(r) & rc) == rc);
+        const amp_t p = a[r ^ (1 << B)], q = a[r];
+        o[r] = pick(on, p, q);
     }
 }
 
 template <int B0, int B1>
-__device__ __forceinline__ void r_dense2(amp_t (&v)[kRR], const RegOp& op) {
+__device__ __forceinline__ void d2(const Regs& a, Regs& o, const RegOp& op, bool tp) {
     const uint32_t rc = op.rctl;
 #pragma unroll
     for (int r = 0; r < kRR; ++r) {
         if (r & ((1 << B0) | (1 << B1))) continue;
-        if ((uint32_t(r) & rc) != rc) continue;
         const int idx[4] = {r, r | (1 << B0), r | (1 << B1), r | (1 << B0) | (1 << B1)};
-        amp_t a[4];
+        const bool on = tp && ((uint32_t(r) & rc) == rc);
+        amp_t x[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) a[k] = v[idx[k]];
+        for (int k = 0; k < 4; ++k) x[k] = a[idx[k]];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const amp_t acc = cmac2(op.m[4 * k], a[0], op.m[4 * k + 1], a[1]);
-            v[idx[k]] = cadd(acc, cmac2(op.m[4 * k + 2], a[2], op.m[4 * k + 3], a[3]));
+            const amp_t acc = cmac2(op.m[4 * k], x[0], op.m[4 * k + 1], x[1]);
+            const amp_t n = cadd(acc, cmac2(op.m[4 * k + 2], x[2], op.m[4 * k + 3], x[3]));
+            o[idx[k]] = pick(on, n, x[k]);
         }
     }
 }
 
-__device__ __forceinline__ void r_diag(amp_t (&v)[kRR], const RegOp& op, uint32_t jt, const RPhase& ph,
-                                       uint64_t base) {
-    const uint32_t g0 = (base & op.dg0) ? 1u : 0u, g1 = (base & op.dg1) ? 2u : 0u;
-    const amp_t d0 = op.m[0], d1 = op.m[1], d2 = op.m[2], d3 = op.m[3];
+__device__ __forceinline__ void r_diag(const Regs& a, Regs& o, const RegOp& op, uint32_t jt, const RPhase& ph,
+                                       uint64_t base, bool tp) {
+    const uint32_t g = ((base & op.dg0) ? 1u : 0u) | ((base & op.dg1) ? 2u : 0u);
+    const amp_t d0 = op.m[0], d1 = op.m[1], d2v = op.m[2], d3 = op.m[3];
     const uint32_t rc = op.rctl;
 #pragma unroll
     for (int r = 0; r < kRR; ++r) {
-        if ((uint32_t(r) & rc) != rc) continue;
+        const bool on = tp && ((uint32_t(r) & rc) == rc);
         const uint32_t j = jt | ph.jr[r];
-        const uint32_t sel = g0 | g1 | ((j & op.dl0) ? 1u : 0u) | ((j & op.dl1) ? 2u : 0u);
-        const amp_t d = sel == 0 ? d0 : sel == 1 ? d1 : sel == 2 ? d2 : d3;
-        v[r] = cmul(v[r], d);
+        const uint32_t sel = g | ((j & op.dl0) ? 1u : 0u) | ((j & op.dl1) ? 2u : 0u);
+        const amp_t d = sel == 0 ? d0 : sel == 1 ? d1 : sel == 2 ? d2v : d3;
+        const amp_t q = a[r];
+        o[r] = pick(on, cmul(q, d), q);
     }
 }
 
 template <int XR>
-__device__ __forceinline__ void r_pauli(amp_t (&v)[kRR], const RegOp& op, uint32_t jt, const RPhase& ph,
+__device__ __forceinline__ void r_pauli(const Regs& a, Regs& o, const RegOp& op, uint32_t jt, const RPhase& ph,
                                         uint64_t base) {
     constexpr int low = XR & -XR;
     const amp_t c = op.m[0], yp = op.m[1];
@@ -121,81 +172,118 @@ __device__ __forceinline__ void r_pauli(amp_t (&v)[kRR], const RegOp& op, uint32
         const uint32_t j1 = jt | ph.jr[r], j2 = jt | ph.jr[r2];
         const double s1 = ((__popc(j1 & op.zl) ^ gp) & 1u) ? -1.0 : 1.0;
         const double s2 = ((__popc(j2 & op.zl) ^ gp) & 1u) ? -1.0 : 1.0;
-        const amp_t a1 = v[r], a2 = v[r2];
-        v[r] = cmac2(c, a1, cscale(yp, s2), a2);
-        v[r2] = cmac2(cscale(yp, s1), a1, c, a2);
+        const amp_t a1 = a[r], a2 = a[r2];
+        o[r] = cmac2(c, a1, cscale(yp, s2), a2);
+        o[r2] = cmac2(cscale(yp, s1), a1, c, a2);
     }
 }
 
-__device__ __forceinline__ void r_pauli_diag(amp_t (&v)[kRR], const RegOp& op, uint32_t jt, const RPhase& ph,
-                                             uint64_t base) {
+__device__ __forceinline__ void r_pauli_diag(const Regs& a, Regs& o, const RegOp& op, uint32_t jt,
+                                             const RPhase& ph, uint64_t base) {
     const amp_t c = op.m[0], yp = op.m[1];
     const amp_t dp = cadd(c, yp), dm = cadd(c, cscale(yp, -1.0));
     const uint32_t gp = __popcll(base & op.zg) & 1u;
 #pragma unroll
     for (int r = 0; r < kRR; ++r) {
         const uint32_t j = jt | ph.jr[r];
-        v[r] = cmul(v[r], ((__popc(j & op.zl) ^ gp) & 1u) ? dm : dp);
+        o[r] = cmul(a[r], ((__popc(j & op.zl) ^ gp) & 1u) ? dm : dp);
     }
 }
 
-__device__ __forceinline__ void apply_reg_op(amp_t (&v)[kRR], const RegOp& op, uint32_t jt, const RPhase& ph,
-                                          uint64_t base) {
-    if ((base & op.gctrl) != op.gctrl) return;
-    if ((jt & op.tctl) != op.tctl) return;
+__device__ __forceinline__ void apply_reg_op(const Regs& a, Regs& o, const RegOp& op, uint32_t jt,
+                                             const RPhase& ph, uint64_t base) {
+    if ((base & op.gctrl) != op.gctrl) {  // tile-uniform control not met
+        copy_regs(a, o);
+        return;
+    }
+    const bool free = (op.rctl | op.tctl) == 0;
+    const bool tp = (jt & op.tctl) == op.tctl;
     switch (op.kind) {
         case RK_DENSE1:
+            if (free) {
+                switch (op.b0) {
+                    case 0: d1_all<0>(a, o, op); return;
+                    case 1: d1_all<1>(a, o, op); return;
+                    case 2: d1_all<2>(a, o, op); return;
+                    default: d1_all<3>(a, o, op); return;
+                }
+            }
             switch (op.b0) {
-                case 0: r_dense1<0>(v, op); return;
-                case 1: r_dense1<1>(v, op); return;
-                case 2: r_dense1<2>(v, op); return;
-                default: r_dense1<3>(v, op); return;
+                case 0: d1_masked<0>(a, o, op, tp); return;
+                case 1: d1_masked<1>(a, o, op, tp); return;
+                case 2: d1_masked<2>(a, o, op, tp); return;
+                default: d1_masked<3>(a, o, op, tp); return;
             }
         case RK_X:
+            if (free) {
+                switch (op.b0) {
+                    case 0: x_all<0>(a, o); return;
+                    case 1: x_all<1>(a, o); return;
+                    case 2: x_all<2>(a, o); return;
+                    default: x_all<3>(a, o); return;
+                }
+            }
+            if (op.tctl == 0 && __builtin_popcount(op.rctl) == 1) {
+                switch (op.b0 * 4 + __builtin_ctz(op.rctl)) {
+                    case 1: x_rc<0, 1>(a, o); return;
+                    case 2: x_rc<0, 2>(a, o); return;
+                    case 3: x_rc<0, 3>(a, o); return;
+                    case 4: x_rc<1, 0>(a, o); return;
+                    case 6: x_rc<1, 2>(a, o); return;
+                    case 7: x_rc<1, 3>(a, o); return;
+                    case 8: x_rc<2, 0>(a, o); return;
+                    case 9: x_rc<2, 1>(a, o); return;
+                    case 11: x_rc<2, 3>(a, o); return;
+                    case 12: x_rc<3, 0>(a, o); return;
+                    case 13: x_rc<3, 1>(a, o); return;
+                    default: x_rc<3, 2>(a, o); return;
+                }
+            }
             switch (op.b0) {
-                case 0: r_x<0>(v, op); return;
-                case 1: r_x<1>(v, op); return;
-                case 2: r_x<2>(v, op); return;
-                default: r_x<3>(v, op); return;
+                case 0: x_masked<0>(a, o, op, tp); return;
+                case 1: x_masked<1>(a, o, op, tp); return;
+                case 2: x_masked<2>(a, o, op, tp); return;
+                default: x_masked<3>(a, o, op, tp); return;
             }
         case RK_DENSE2:
             switch (op.b0 * 4 + op.b1) {
-                case 1: r_dense2<0, 1>(v, op); return;
-                case 2: r_dense2<0, 2>(v, op); return;
-                case 3: r_dense2<0, 3>(v, op); return;
-                case 4: r_dense2<1, 0>(v, op); return;
-                case 6: r_dense2<1, 2>(v, op); return;
-                case 7: r_dense2<1, 3>(v, op); return;
-                case 8: r_dense2<2, 0>(v, op); return;
-                case 9: r_dense2<2, 1>(v, op); return;
-                case 11: r_dense2<2, 3>(v, op); return;
-                case 12: r_dense2<3, 0>(v, op); return;
-                case 13: r_dense2<3, 1>(v, op); return;
-                default: r_dense2<3, 2>(v, op); return;
+                case 1: d2<0, 1>(a, o, op, tp); return;
+                case 2: d2<0, 2>(a, o, op, tp); return;
+                case 3: d2<0, 3>(a, o, op, tp); return;
+                case 4: d2<1, 0>(a, o, op, tp); return;
+                case 6: d2<1, 2>(a, o, op, tp); return;
+                case 7: d2<1, 3>(a, o, op, tp); return;
+                case 8: d2<2, 0>(a, o, op, tp); return;
+                case 9: d2<2, 1>(a, o, op, tp); return;
+                case 11: d2<2, 3>(a, o, op, tp); return;
+                case 12: d2<3, 0>(a, o, op, tp); return;
+                case 13: d2<3, 1>(a, o, op, tp); return;
+                default: d2<3, 2>(a, o, op, tp); return;
             }
         case RK_DIAG:
-            r_diag(v, op, jt, ph, base);
+            r_diag(a, o, op, jt, ph, base, tp);
             return;
         case RK_PAULI:
             switch (op.xr) {
-                case 0: r_pauli_diag(v, op, jt, ph, base); return;
-                case 1: r_pauli<1>(v, op, jt, ph, base); return;
-                case 2: r_pauli<2>(v, op, jt, ph, base); return;
-                case 3: r_pauli<3>(v, op, jt, ph, base); return;
-                case 4: r_pauli<4>(v, op, jt, ph, base); return;
-                case 5: r_pauli<5>(v, op, jt, ph, base); return;
-                case 6: r_pauli<6>(v, op, jt, ph, base); return;
-                case 7: r_pauli<7>(v, op, jt, ph, base); return;
-                case 8: r_pauli<8>(v, op, jt, ph, base); return;
-                case 9: r_pauli<9>(v, op, jt, ph, base); return;
-                case 10: r_pauli<10>(v, op, jt, ph, base); return;
-                case 11: r_pauli<11>(v, op, jt, ph, base); return;
-                case 12: r_pauli<12>(v, op, jt, ph, base); return;
-                case 13: r_pauli<13>(v, op, jt, ph, base); return;
-                case 14: r_pauli<14>(v, op, jt, ph, base); return;
-                default: r_pauli<15>(v, op, jt, ph, base); return;
+                case 0: r_pauli_diag(a, o, op, jt, ph, base); return;
+                case 1: r_pauli<1>(a, o, op, jt, ph, base); return;
+                case 2: r_pauli<2>(a, o, op, jt, ph, base); return;
+                case 3: r_pauli<3>(a, o, op, jt, ph, base); return;
+                case 4: r_pauli<4>(a, o, op, jt, ph, base); return;
+                case 5: r_pauli<5>(a, o, op, jt, ph, base); return;
+                case 6: r_pauli<6>(a, o, op, jt, ph, base); return;
+                case 7: r_pauli<7>(a, o, op, jt, ph, base); return;
+                case 8: r_pauli<8>(a, o, op, jt, ph, base); return;
+                case 9: r_pauli<9>(a, o, op, jt, ph, base); return;
+                case 10: r_pauli<10>(a, o, op, jt, ph, base); return;
+                case 11: r_pauli<11>(a, o, op, jt, ph, base); return;
+                case 12: r_pauli<12>(a, o, op, jt, ph, base); return;
+                case 13: r_pauli<13>(a, o, op, jt, ph, base); return;
+                case 14: r_pauli<14>(a, o, op, jt, ph, base); return;
+                default: r_pauli<15>(a, o, op, jt, ph, base); return;
             }
         default:
+            copy_regs(a, o);
             return;
     }
 }
@@ -207,27 +295,38 @@ __global__ void __launch_bounds__(kRT) k_rtile(amp_t* __restrict__ s, uint32_t n
     amp_t* b = s + (uint64_t(blockIdx.y) << nq);
     const uint64_t full = (uint64_t{1} << nq) - 1;
     const uint64_t base = pdep64(blockIdx.x, full & ~smask);
-    amp_t v[kRR];
+    Regs v, w;
     {
         const RPhase& p0 = phases[0];
         const uint64_t gt = base | pdep64(threadIdx.x, p0.tmask_global);
 #pragma unroll
         for (int r = 0; r < kRR; ++r) v[r] = b[gt | p0.gofs[r]];
     }
+    uint32_t sjt_prev = 0;
     for (uint32_t p = 0; p < n_phases; ++p) {
         const RPhase& ph = phases[p];
         const uint32_t jt = pdep32(threadIdx.x, ph.tmask);
+        const uint32_t sjt = swz(jt);
         if (p > 0) {
             const RPhase& pp = phases[p - 1];
-            const uint32_t jtp = pdep32(threadIdx.x, pp.tmask);
-            if (p > 1) __syncthreads();  // previous transpose's reads are done
+            if (p > 1) __syncthreads();  // every thread has read the previous layout
 #pragma unroll
-            for (int r = 0; r < kRR; ++r) lds[swz(jtp | pp.jr[r])] = v[r];
+            for (int r = 0; r < kRR; ++r) lds[sjt_prev ^ pp.sjr[r]] = v[r];
             __syncthreads();
 #pragma unroll
-            for (int r = 0; r < kRR; ++r) v[r] = lds[swz(jt | ph.jr[r])];
+            for (int r = 0; r < kRR; ++r) v[r] = lds[sjt ^ ph.sjr[r]];
         }
-        for (uint32_t o = ph.first_op; o < ph.first_op + ph.n_ops; ++o) apply_reg_op(v, ops[o], jt, ph, base);
+        sjt_prev = sjt;
+        uint32_t o = ph.first_op;
+        const uint32_t end = ph.first_op + ph.n_ops;
+        for (; o + 1 < end; o += 2) {
+            apply_reg_op(v, w, ops[o], jt, ph, base);
+            apply_reg_op(w, v, ops[o + 1], jt, ph, base);
+        }
+        if (o < end) {
+            apply_reg_op(v, w, ops[o], jt, ph, base);
+            copy_regs(w, v);
+        }
     }
     {
         const RPhase& pl = phases[n_phases - 1];
@@ -311,7 +410,8 @@ bool build_rtile(const std::vector<TileOp>& tops, uint32_t L, uint64_t smask, RT
             uint32_t j = 0;
             for (int i = 0; i < 4; ++i)
                 if (r & (1 << i)) j |= 1u << rb[i];
-            ph.jr[r] = static_cast<uint16_t>(j);
+            ph.jr[r] = j;
+            ph.sjr[r] = swz(j);
             ph.gofs[r] = pdep_host(j, smask);
         }
         auto ridx = [&](uint32_t pos) {

@@ -50,6 +50,7 @@ struct RPhase {
     uint32_t pad;
     uint64_t tmask_global;   // their global qubit positions
     uint32_t jr[16];         // tile index of register r (thread bits zero)
+    uint32_t sjr[16];        // its LDS slot contribution (XOR-linear swizzle)
     uint64_t gofs[16];       // global offset of register r
 };
 
