@@ -44,6 +44,9 @@ Context& ctx() {
 }
 
 thread_local std::string g_last_error;
+}  // namespace
+extern bool g_rtile_persistent;
+namespace {
 // Register tiles on (default) or off (LDS tiles only), for comparisons.
 bool g_use_rtile = true;
 
@@ -627,8 +630,10 @@ lqcs_status lqcs_dev_synchronize(void) {
 
 lqcs_status lqcs_set_tile_kernel(int kernel) {
     return guarded([&] {
-        if (kernel != 0 && kernel != 1) throw Error(LQCS_ERR_INVALID_ARGUMENT, "tile kernel must be 0 (LDS) or 1 (register)");
-        g_use_rtile = kernel == 1;
+        if (kernel < 0 || kernel > 2)
+            throw Error(LQCS_ERR_INVALID_ARGUMENT, "tile kernel must be 0 (LDS), 1 (register) or 2 (register, persistent)");
+        g_use_rtile = kernel >= 1;
+        g_rtile_persistent = kernel == 2;
     });
 }
 

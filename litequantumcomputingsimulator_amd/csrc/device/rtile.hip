@@ -17,6 +17,11 @@
 #include <cstring>
 
 namespace lqcs_dev {
+
+// Persistent register tiles (default) and their workgroups per CU.
+bool g_rtile_persistent = true;
+uint32_t g_rtile_blocks_per_cu = 2;
+
 namespace {
 
 constexpr int kRL = 12;   // tile bits
@@ -336,6 +341,71 @@ __global__ void __launch_bounds__(kRT) k_rtile(amp_t* __restrict__ s, uint32_t n
     }
 }
 
+// Persistent form: each workgroup walks tiles blockIdx.x, +gridDim.x, ...
+// and issues the global loads of its next tile before computing the current
+// one, so HBM reads overlap the in-register gate work.
+__global__ void __launch_bounds__(kRT) k_rtile_persist(amp_t* __restrict__ s, uint32_t nq, uint64_t smask,
+                                                      const RPhase* __restrict__ phases, uint32_t n_phases,
+                                                      const RegOp* __restrict__ ops, uint32_t tiles) {
+    __shared__ amp_t lds[1 << kRL];
+    amp_t* b = s + (uint64_t(blockIdx.y) << nq);
+    const uint64_t full = (uint64_t{1} << nq) - 1;
+    const uint64_t outside = full & ~smask;
+    const RPhase& p0 = phases[0];
+    const RPhase& pl = phases[n_phases - 1];
+    const uint64_t gt0 = pdep64(threadIdx.x, p0.tmask_global);
+    const uint64_t gtl = pdep64(threadIdx.x, pl.tmask_global);
+    Regs v, w, nx;
+    uint32_t tile = blockIdx.x;
+    if (tile >= tiles) return;
+    {
+        const uint64_t gt = pdep64(tile, outside) | gt0;
+#pragma unroll
+        for (int r = 0; r < kRR; ++r) v[r] = b[gt | p0.gofs[r]];
+    }
+    for (; tile < tiles; tile += gridDim.x) {
+        const uint64_t base = pdep64(tile, outside);
+        const uint32_t next = tile + gridDim.x;
+        if (next < tiles) {
+            const uint64_t gt = pdep64(next, outside) | gt0;
+#pragma unroll
+            for (int r = 0; r < kRR; ++r) nx[r] = b[gt | p0.gofs[r]];
+        }
+        uint32_t sjt_prev = 0;
+        for (uint32_t p = 0; p < n_phases; ++p) {
+            const RPhase& ph = phases[p];
+            const uint32_t jt = pdep32(threadIdx.x, ph.tmask);
+            const uint32_t sjt = swz(jt);
+            if (p > 0) {
+                const RPhase& pp = phases[p - 1];
+                __syncthreads();  // every thread is done with the LDS tile
+#pragma unroll
+                for (int r = 0; r < kRR; ++r) lds[sjt_prev ^ pp.sjr[r]] = v[r];
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < kRR; ++r) v[r] = lds[sjt ^ ph.sjr[r]];
+            }
+            sjt_prev = sjt;
+            uint32_t o = ph.first_op;
+            const uint32_t end = ph.first_op + ph.n_ops;
+            for (; o + 1 < end; o += 2) {
+                apply_reg_op(v, w, ops[o], jt, ph, base);
+                apply_reg_op(w, v, ops[o + 1], jt, ph, base);
+            }
+            if (o < end) {
+                apply_reg_op(v, w, ops[o], jt, ph, base);
+                copy_regs(w, v);
+            }
+        }
+        {
+            const uint64_t gt = base | gtl;
+#pragma unroll
+            for (int r = 0; r < kRR; ++r) b[gt | pl.gofs[r]] = v[r];
+        }
+        if (next < tiles) copy_regs(nx, v);
+    }
+}
+
 uint64_t pdep_host(uint64_t v, uint64_t mask) {
     uint64_t r = 0;
     for (uint64_t bit = 1; mask; bit <<= 1) {
@@ -476,6 +546,12 @@ bool build_rtile(const std::vector<TileOp>& tops, uint32_t L, uint64_t smask, RT
 void launch_rtile(amp_t* s, uint32_t nq, uint64_t smask, const RPhase* d_phases, uint32_t n_phases,
                   const RegOp* d_ops, uint32_t batch) {
     const uint64_t tiles = uint64_t{1} << (nq - kRL);
+    const uint64_t resident = uint64_t(num_cus()) * g_rtile_blocks_per_cu;
+    if (g_rtile_persistent && tiles > resident) {
+        k_rtile_persist<<<dim3(static_cast<unsigned>(resident), batch), kRT, 0, stream()>>>(
+            s, nq, smask, d_phases, n_phases, d_ops, static_cast<uint32_t>(tiles));
+        return;
+    }
     k_rtile<<<dim3(static_cast<unsigned>(tiles), batch), kRT, 0, stream()>>>(s, nq, smask, d_phases, n_phases,
                                                                             d_ops);
 }
