@@ -129,9 +129,10 @@ lqcs_status lqcs_plan_describe(uint32_t n_qubits, const lqcs_program* prog, cons
                                uint64_t* n_steps, uint64_t* ops_per_step, uint64_t* tile_masks,
                                uint64_t* phases_per_step, uint64_t max_steps);
 
-/* Kernel used for tiled passes: 2 = register-resident tile, persistent with
- * next-tile prefetch (default); 1 = register tile, one workgroup per tile;
- * 0 = LDS tile. A switch for comparisons; results agree to rounding. */
+/* Kernel used for tiled passes: 1 = register-resident tile, one workgroup
+ * per tile (default); 2 = register tile, streaming (each workgroup walks
+ * tiles and prefetches the next one); 0 = LDS tile. A switch for
+ * comparisons; results agree to rounding. */
 lqcs_status lqcs_set_tile_kernel(int kernel);
 
 /* Launch timing (no counterpart in the reference): while enabled, every

@@ -45,7 +45,9 @@ Context& ctx() {
 
 thread_local std::string g_last_error;
 }  // namespace
-extern bool g_rtile_persistent;
+
+extern bool g_rtile_stream;  // rtile.hip
+
 namespace {
 // Register tiles on (default) or off (LDS tiles only), for comparisons.
 bool g_use_rtile = true;
@@ -451,32 +453,33 @@ void execute_plan(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan,
     // Register-tile programs where possible, LDS tile programs otherwise.
     std::vector<TileOp> flat;
     std::vector<RPhase> rph;
-    std::vector<RegOp> rops;
+    std::vector<amp_t> rwords;
     struct Where {
         int kind;  // 0 stand-alone, 1 LDS tile, 2 register tile
-        size_t first, count;
+        size_t first, count, words;
+        bool restage;
     };
     std::vector<Where> where;
     for (const auto& st : plan.steps) {
         if (!st.tiled) {
-            where.push_back({0, 0, 0});
+            where.push_back({0, 0, 0, 0, false});
             continue;
         }
         RTileProgram rp;
-        if (g_use_rtile && build_rtile(st.tile_ops, L, st.smask, rp)) {
-            const size_t op0 = rops.size();
-            for (auto& ph : rp.phases) ph.first_op += static_cast<uint32_t>(op0);
-            where.push_back({2, rph.size(), rp.phases.size()});
+        if (g_use_rtile && build_rtile(st.tile_ops, L, st.smask, nq, rp)) {
+            bool restage = false;
+            for (size_t p = 1; p < rp.phases.size(); ++p) restage |= rp.phases[p].seg_words != 0;
+            where.push_back({2, rph.size(), rp.phases.size(), rwords.size(), restage});
             rph.insert(rph.end(), rp.phases.begin(), rp.phases.end());
-            rops.insert(rops.end(), rp.ops.begin(), rp.ops.end());
+            rwords.insert(rwords.end(), rp.words.begin(), rp.words.end());
         } else {
-            where.push_back({1, flat.size(), st.tile_ops.size()});
+            where.push_back({1, flat.size(), st.tile_ops.size(), 0, false});
             flat.insert(flat.end(), st.tile_ops.begin(), st.tile_ops.end());
         }
     }
     const TileOp* d_ops = flat.empty() ? nullptr : stage(flat.data(), flat.size());
     const RPhase* d_ph = rph.empty() ? nullptr : stage(rph.data(), rph.size());
-    const RegOp* d_rops = rops.empty() ? nullptr : stage(rops.data(), rops.size());
+    const amp_t* d_rwords = rwords.empty() ? nullptr : stage(rwords.data(), rwords.size());
     for (size_t i = 0; i < plan.steps.size(); ++i) {
         const PlanStep& st = plan.steps[i];
         const bool timed = prof().on;
@@ -488,7 +491,8 @@ void execute_plan(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan,
         }
         const Where& w = where[i];
         if (w.kind == 2) {
-            launch_rtile(s, nq, st.smask, d_ph + w.first, static_cast<uint32_t>(w.count), d_rops, batch);
+            launch_rtile(s, nq, st.smask, d_ph + w.first, static_cast<uint32_t>(w.count),
+                         d_rwords ? d_rwords + w.words : nullptr, w.restage, batch);
         } else if (w.kind == 1) {
             launch_tile_pass(s, nq, L, st.smask, d_ops + w.first, static_cast<uint32_t>(w.count), batch);
         } else {
@@ -631,9 +635,10 @@ lqcs_status lqcs_dev_synchronize(void) {
 lqcs_status lqcs_set_tile_kernel(int kernel) {
     return guarded([&] {
         if (kernel < 0 || kernel > 2)
-            throw Error(LQCS_ERR_INVALID_ARGUMENT, "tile kernel must be 0 (LDS), 1 (register) or 2 (register, persistent)");
+            throw Error(LQCS_ERR_INVALID_ARGUMENT,
+                        "tile kernel must be 0 (LDS), 1 (register) or 2 (register, streaming)");
         g_use_rtile = kernel >= 1;
-        g_rtile_persistent = kernel == 2;
+        g_rtile_stream = kernel == 2;
     });
 }
 
@@ -809,7 +814,7 @@ lqcs_status lqcs_plan_describe(uint32_t n_qubits, const lqcs_program* prog, cons
             if (phases_per_step) {
                 RTileProgram rp;
                 phases_per_step[i] = (plan.steps[i].tiled && g_use_rtile &&
-                                      build_rtile(plan.steps[i].tile_ops, L, plan.steps[i].smask, rp))
+                                      build_rtile(plan.steps[i].tile_ops, L, plan.steps[i].smask, n_qubits, rp))
                                          ? rp.phases.size()
                                          : 0;
             }

@@ -8,8 +8,15 @@
 // into phases; between phases the tile is transposed through LDS into the
 // next phase's layout (one LDS write + read of the tile, two barriers).
 //
-// Global load and store use the first and last phases' layouts, padded with
-// high tile bits so that lanes walk the contiguous low-qubit runs.
+// Global load and store use the first and last phases' layouts. Their
+// register bits avoid the tile's contiguous low qubits, so that the lanes of
+// a wave walk contiguous runs of amplitudes: a layout whose lanes step over
+// high qubits touches a separate cache line (and page) per lane and runs the
+// pass several times slower.
+//
+// Op records (a 32-byte header plus matrix entries) are staged per segment
+// into a second LDS array and read from there with uniform addresses, so an
+// op costs one LDS round trip instead of a chain of scalar-cache misses.
 #include "common.hpp"
 #include "engine.hpp"
 #include "tile.hpp"
@@ -18,8 +25,8 @@
 
 namespace lqcs_dev {
 
-// Persistent register tiles (default) and their workgroups per CU.
-bool g_rtile_persistent = true;
+// Streaming register tiles (next-tile prefetch) and their workgroups per CU.
+bool g_rtile_stream = false;
 uint32_t g_rtile_blocks_per_cu = 2;
 
 namespace {
@@ -27,6 +34,7 @@ namespace {
 constexpr int kRL = 12;   // tile bits
 constexpr int kRT = 256;  // threads
 constexpr int kRR = 16;   // registers per thread
+constexpr uint32_t kStage = kRTileWords / kRT;  // op words each thread stages
 
 __device__ __forceinline__ uint64_t pdep64(uint64_t v, uint64_t mask) {
     uint64_t r = 0;
@@ -58,6 +66,28 @@ __host__ __device__ __forceinline__ uint32_t swz(uint32_t j) {
 
 using Regs = amp_t[kRR];
 
+// An op's uniform fields, decoded from its staged header.
+struct Hdr {
+    uint32_t kind, b0, b1, xr, rctl, tctl, la, lb, gctrl, ga, gb;
+};
+
+__device__ __forceinline__ uint32_t rfl(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+}
+
+__device__ __forceinline__ double rfl(double v) {
+    const uint64_t u = static_cast<uint64_t>(__double_as_longlong(v));
+    const uint64_t lo = rfl(static_cast<uint32_t>(u)), hi = rfl(static_cast<uint32_t>(u >> 32));
+    return __longlong_as_double(static_cast<long long>(lo | (hi << 32)));
+}
+
+// A matrix entry read from the staged op words (same address in every lane)
+// and moved to scalar registers, where the FMAs take it as an operand.
+__device__ __forceinline__ amp_t uni(const amp_t* p) {
+    const amp_t v = *p;
+    return make_double2(rfl(v.x), rfl(v.y));
+}
+
 // Select on values already loaded (a select between an array element and a
 // computed value would otherwise become a select of addresses and push the
 // register arrays to scratch).
@@ -71,10 +101,11 @@ __device__ __forceinline__ void copy_regs(const Regs& a, Regs& o) {
 }
 
 // Every op reads register set `a` and writes all of set `o` (ping-pong), so
-// results land in their final registers without copies.
+// results land in their final registers without copies. `m` points at the
+// op's matrix entries in LDS.
 template <int B>
-__device__ __forceinline__ void d1_all(const Regs& a, Regs& o, const RegOp& op) {
-    const amp_t m0 = op.m[0], m1 = op.m[1], m2 = op.m[2], m3 = op.m[3];
+__device__ __forceinline__ void d1_all(const Regs& a, Regs& o, const amp_t* m) {
+    const amp_t m0 = uni(m), m1 = uni(m + 1), m2 = uni(m + 2), m3 = uni(m + 3);
 #pragma unroll
     for (int r = 0; r < kRR; ++r) {
         if (r & (1 << B)) continue;
@@ -87,9 +118,8 @@ __device__ __forceinline__ void d1_all(const Regs& a, Regs& o, const RegOp& op) 
 // Controlled variant: pairs whose register bits miss a control, or lanes
 // whose thread bits miss one, keep their values (selects, no branches).
 template <int B>
-__device__ __forceinline__ void d1_masked(const Regs& a, Regs& o, const RegOp& op, bool tp) {
-    const amp_t m0 = op.m[0], m1 = op.m[1], m2 = op.m[2], m3 = op.m[3];
-    const uint32_t rc = op.rctl;
+__device__ __forceinline__ void d1_masked(const Regs& a, Regs& o, const amp_t* m, uint32_t rc, bool tp) {
+    const amp_t m0 = uni(m), m1 = uni(m + 1), m2 = uni(m + 2), m3 = uni(m + 3);
 #pragma unroll
     for (int r = 0; r < kRR; ++r) {
         if (r & (1 << B)) continue;
@@ -118,8 +148,7 @@ __device__ __forceinline__ void x_rc(const Regs& a, Regs& o) {
 }
 
 template <int B>
-__device__ __forceinline__ void x_masked(const Regs& a, Regs& o, const RegOp& op, bool tp) {
-    const uint32_t rc = op.rctl;
+__device__ __forceinline__ void x_masked(const Regs& a, Regs& o, uint32_t rc, bool tp) {
 #pragma unroll
     for (int r = 0; r < kRR; ++r) {
         const bool on = tp && ((uint32_t(r) & rc) == rc);
@@ -129,8 +158,7 @@ __device__ __forceinline__ void x_masked(const Regs& a, Regs& o, const RegOp& op
 }
 
 template <int B0, int B1>
-__device__ __forceinline__ void d2(const Regs& a, Regs& o, const RegOp& op, bool tp) {
-    const uint32_t rc = op.rctl;
+__device__ __forceinline__ void d2(const Regs& a, Regs& o, const amp_t* m, uint32_t rc, bool tp) {
 #pragma unroll
     for (int r = 0; r < kRR; ++r) {
         if (r & ((1 << B0) | (1 << B1))) continue;
@@ -141,23 +169,22 @@ __device__ __forceinline__ void d2(const Regs& a, Regs& o, const RegOp& op, bool
         for (int k = 0; k < 4; ++k) x[k] = a[idx[k]];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const amp_t acc = cmac2(op.m[4 * k], x[0], op.m[4 * k + 1], x[1]);
-            const amp_t n = cadd(acc, cmac2(op.m[4 * k + 2], x[2], op.m[4 * k + 3], x[3]));
+            const amp_t acc = cmac2(uni(m + 4 * k), x[0], uni(m + 4 * k + 1), x[1]);
+            const amp_t n = cadd(acc, cmac2(uni(m + 4 * k + 2), x[2], uni(m + 4 * k + 3), x[3]));
             o[idx[k]] = pick(on, n, x[k]);
         }
     }
 }
 
-__device__ __forceinline__ void r_diag(const Regs& a, Regs& o, const RegOp& op, uint32_t jt, const RPhase& ph,
-                                       uint64_t base, bool tp) {
-    const uint32_t g = ((base & op.dg0) ? 1u : 0u) | ((base & op.dg1) ? 2u : 0u);
-    const amp_t d0 = op.m[0], d1 = op.m[1], d2v = op.m[2], d3 = op.m[3];
-    const uint32_t rc = op.rctl;
+__device__ __forceinline__ void r_diag(const Regs& a, Regs& o, const Hdr& h, const amp_t* m, uint32_t jt,
+                                       const RPhase& ph, uint32_t tile, bool tp) {
+    const uint32_t g = ((tile & h.ga) ? 1u : 0u) | ((tile & h.gb) ? 2u : 0u);
+    const amp_t d0 = uni(m), d1 = uni(m + 1), d2v = uni(m + 2), d3 = uni(m + 3);
 #pragma unroll
     for (int r = 0; r < kRR; ++r) {
-        const bool on = tp && ((uint32_t(r) & rc) == rc);
+        const bool on = tp && ((uint32_t(r) & h.rctl) == h.rctl);
         const uint32_t j = jt | ph.jr[r];
-        const uint32_t sel = g | ((j & op.dl0) ? 1u : 0u) | ((j & op.dl1) ? 2u : 0u);
+        const uint32_t sel = g | ((j & h.la) ? 1u : 0u) | ((j & h.lb) ? 2u : 0u);
         const amp_t d = sel == 0 ? d0 : sel == 1 ? d1 : sel == 2 ? d2v : d3;
         const amp_t q = a[r];
         o[r] = pick(on, cmul(q, d), q);
@@ -165,71 +192,71 @@ __device__ __forceinline__ void r_diag(const Regs& a, Regs& o, const RegOp& op, 
 }
 
 template <int XR>
-__device__ __forceinline__ void r_pauli(const Regs& a, Regs& o, const RegOp& op, uint32_t jt, const RPhase& ph,
-                                        uint64_t base) {
+__device__ __forceinline__ void r_pauli(const Regs& a, Regs& o, const Hdr& h, const amp_t* m, uint32_t jt,
+                                        const RPhase& ph, uint32_t tile) {
     constexpr int low = XR & -XR;
-    const amp_t c = op.m[0], yp = op.m[1];
-    const uint32_t gp = __popcll(base & op.zg) & 1u;
+    const amp_t c = uni(m), yp = uni(m + 1);
+    const uint32_t gp = __popc(tile & h.ga) & 1u;
 #pragma unroll
     for (int r = 0; r < kRR; ++r) {
         if (r & low) continue;
         const int r2 = r ^ XR;
         const uint32_t j1 = jt | ph.jr[r], j2 = jt | ph.jr[r2];
-        const double s1 = ((__popc(j1 & op.zl) ^ gp) & 1u) ? -1.0 : 1.0;
-        const double s2 = ((__popc(j2 & op.zl) ^ gp) & 1u) ? -1.0 : 1.0;
+        const double s1 = ((__popc(j1 & h.la) ^ gp) & 1u) ? -1.0 : 1.0;
+        const double s2 = ((__popc(j2 & h.la) ^ gp) & 1u) ? -1.0 : 1.0;
         const amp_t a1 = a[r], a2 = a[r2];
         o[r] = cmac2(c, a1, cscale(yp, s2), a2);
         o[r2] = cmac2(cscale(yp, s1), a1, c, a2);
     }
 }
 
-__device__ __forceinline__ void r_pauli_diag(const Regs& a, Regs& o, const RegOp& op, uint32_t jt,
-                                             const RPhase& ph, uint64_t base) {
-    const amp_t c = op.m[0], yp = op.m[1];
+__device__ __forceinline__ void r_pauli_diag(const Regs& a, Regs& o, const Hdr& h, const amp_t* m, uint32_t jt,
+                                             const RPhase& ph, uint32_t tile) {
+    const amp_t c = uni(m), yp = uni(m + 1);
     const amp_t dp = cadd(c, yp), dm = cadd(c, cscale(yp, -1.0));
-    const uint32_t gp = __popcll(base & op.zg) & 1u;
+    const uint32_t gp = __popc(tile & h.ga) & 1u;
 #pragma unroll
     for (int r = 0; r < kRR; ++r) {
         const uint32_t j = jt | ph.jr[r];
-        o[r] = cmul(a[r], ((__popc(j & op.zl) ^ gp) & 1u) ? dm : dp);
+        o[r] = cmul(a[r], ((__popc(j & h.la) ^ gp) & 1u) ? dm : dp);
     }
 }
 
-__device__ __forceinline__ void apply_reg_op(const Regs& a, Regs& o, const RegOp& op, uint32_t jt,
-                                             const RPhase& ph, uint64_t base) {
-    if ((base & op.gctrl) != op.gctrl) {  // tile-uniform control not met
+__device__ __forceinline__ void apply_reg_op(const Regs& a, Regs& o, const Hdr& h, const amp_t* m, uint32_t jt,
+                                             const RPhase& ph, uint32_t tile) {
+    if ((tile & h.gctrl) != h.gctrl) {  // tile-uniform control not met
         copy_regs(a, o);
         return;
     }
-    const bool free = (op.rctl | op.tctl) == 0;
-    const bool tp = (jt & op.tctl) == op.tctl;
-    switch (op.kind) {
+    const bool free = (h.rctl | h.tctl) == 0;
+    const bool tp = (jt & h.tctl) == h.tctl;
+    switch (h.kind) {
         case RK_DENSE1:
             if (free) {
-                switch (op.b0) {
-                    case 0: d1_all<0>(a, o, op); return;
-                    case 1: d1_all<1>(a, o, op); return;
-                    case 2: d1_all<2>(a, o, op); return;
-                    default: d1_all<3>(a, o, op); return;
+                switch (h.b0) {
+                    case 0: d1_all<0>(a, o, m); return;
+                    case 1: d1_all<1>(a, o, m); return;
+                    case 2: d1_all<2>(a, o, m); return;
+                    default: d1_all<3>(a, o, m); return;
                 }
             }
-            switch (op.b0) {
-                case 0: d1_masked<0>(a, o, op, tp); return;
-                case 1: d1_masked<1>(a, o, op, tp); return;
-                case 2: d1_masked<2>(a, o, op, tp); return;
-                default: d1_masked<3>(a, o, op, tp); return;
+            switch (h.b0) {
+                case 0: d1_masked<0>(a, o, m, h.rctl, tp); return;
+                case 1: d1_masked<1>(a, o, m, h.rctl, tp); return;
+                case 2: d1_masked<2>(a, o, m, h.rctl, tp); return;
+                default: d1_masked<3>(a, o, m, h.rctl, tp); return;
             }
         case RK_X:
             if (free) {
-                switch (op.b0) {
+                switch (h.b0) {
                     case 0: x_all<0>(a, o); return;
                     case 1: x_all<1>(a, o); return;
                     case 2: x_all<2>(a, o); return;
                     default: x_all<3>(a, o); return;
                 }
             }
-            if (op.tctl == 0 && __builtin_popcount(op.rctl) == 1) {
-                switch (op.b0 * 4 + __builtin_ctz(op.rctl)) {
+            if (h.tctl == 0 && __builtin_popcount(h.rctl) == 1) {
+                switch (h.b0 * 4 + __builtin_ctz(h.rctl)) {
                     case 1: x_rc<0, 1>(a, o); return;
                     case 2: x_rc<0, 2>(a, o); return;
                     case 3: x_rc<0, 3>(a, o); return;
@@ -244,48 +271,48 @@ __device__ __forceinline__ void apply_reg_op(const Regs& a, Regs& o, const RegOp
                     default: x_rc<3, 2>(a, o); return;
                 }
             }
-            switch (op.b0) {
-                case 0: x_masked<0>(a, o, op, tp); return;
-                case 1: x_masked<1>(a, o, op, tp); return;
-                case 2: x_masked<2>(a, o, op, tp); return;
-                default: x_masked<3>(a, o, op, tp); return;
+            switch (h.b0) {
+                case 0: x_masked<0>(a, o, h.rctl, tp); return;
+                case 1: x_masked<1>(a, o, h.rctl, tp); return;
+                case 2: x_masked<2>(a, o, h.rctl, tp); return;
+                default: x_masked<3>(a, o, h.rctl, tp); return;
             }
         case RK_DENSE2:
-            switch (op.b0 * 4 + op.b1) {
-                case 1: d2<0, 1>(a, o, op, tp); return;
-                case 2: d2<0, 2>(a, o, op, tp); return;
-                case 3: d2<0, 3>(a, o, op, tp); return;
-                case 4: d2<1, 0>(a, o, op, tp); return;
-                case 6: d2<1, 2>(a, o, op, tp); return;
-                case 7: d2<1, 3>(a, o, op, tp); return;
-                case 8: d2<2, 0>(a, o, op, tp); return;
-                case 9: d2<2, 1>(a, o, op, tp); return;
-                case 11: d2<2, 3>(a, o, op, tp); return;
-                case 12: d2<3, 0>(a, o, op, tp); return;
-                case 13: d2<3, 1>(a, o, op, tp); return;
-                default: d2<3, 2>(a, o, op, tp); return;
+            switch (h.b0 * 4 + h.b1) {
+                case 1: d2<0, 1>(a, o, m, h.rctl, tp); return;
+                case 2: d2<0, 2>(a, o, m, h.rctl, tp); return;
+                case 3: d2<0, 3>(a, o, m, h.rctl, tp); return;
+                case 4: d2<1, 0>(a, o, m, h.rctl, tp); return;
+                case 6: d2<1, 2>(a, o, m, h.rctl, tp); return;
+                case 7: d2<1, 3>(a, o, m, h.rctl, tp); return;
+                case 8: d2<2, 0>(a, o, m, h.rctl, tp); return;
+                case 9: d2<2, 1>(a, o, m, h.rctl, tp); return;
+                case 11: d2<2, 3>(a, o, m, h.rctl, tp); return;
+                case 12: d2<3, 0>(a, o, m, h.rctl, tp); return;
+                case 13: d2<3, 1>(a, o, m, h.rctl, tp); return;
+                default: d2<3, 2>(a, o, m, h.rctl, tp); return;
             }
         case RK_DIAG:
-            r_diag(a, o, op, jt, ph, base, tp);
+            r_diag(a, o, h, m, jt, ph, tile, tp);
             return;
         case RK_PAULI:
-            switch (op.xr) {
-                case 0: r_pauli_diag(a, o, op, jt, ph, base); return;
-                case 1: r_pauli<1>(a, o, op, jt, ph, base); return;
-                case 2: r_pauli<2>(a, o, op, jt, ph, base); return;
-                case 3: r_pauli<3>(a, o, op, jt, ph, base); return;
-                case 4: r_pauli<4>(a, o, op, jt, ph, base); return;
-                case 5: r_pauli<5>(a, o, op, jt, ph, base); return;
-                case 6: r_pauli<6>(a, o, op, jt, ph, base); return;
-                case 7: r_pauli<7>(a, o, op, jt, ph, base); return;
-                case 8: r_pauli<8>(a, o, op, jt, ph, base); return;
-                case 9: r_pauli<9>(a, o, op, jt, ph, base); return;
-                case 10: r_pauli<10>(a, o, op, jt, ph, base); return;
-                case 11: r_pauli<11>(a, o, op, jt, ph, base); return;
-                case 12: r_pauli<12>(a, o, op, jt, ph, base); return;
-                case 13: r_pauli<13>(a, o, op, jt, ph, base); return;
-                case 14: r_pauli<14>(a, o, op, jt, ph, base); return;
-                default: r_pauli<15>(a, o, op, jt, ph, base); return;
+            switch (h.xr) {
+                case 0: r_pauli_diag(a, o, h, m, jt, ph, tile); return;
+                case 1: r_pauli<1>(a, o, h, m, jt, ph, tile); return;
+                case 2: r_pauli<2>(a, o, h, m, jt, ph, tile); return;
+                case 3: r_pauli<3>(a, o, h, m, jt, ph, tile); return;
+                case 4: r_pauli<4>(a, o, h, m, jt, ph, tile); return;
+                case 5: r_pauli<5>(a, o, h, m, jt, ph, tile); return;
+                case 6: r_pauli<6>(a, o, h, m, jt, ph, tile); return;
+                case 7: r_pauli<7>(a, o, h, m, jt, ph, tile); return;
+                case 8: r_pauli<8>(a, o, h, m, jt, ph, tile); return;
+                case 9: r_pauli<9>(a, o, h, m, jt, ph, tile); return;
+                case 10: r_pauli<10>(a, o, h, m, jt, ph, tile); return;
+                case 11: r_pauli<11>(a, o, h, m, jt, ph, tile); return;
+                case 12: r_pauli<12>(a, o, h, m, jt, ph, tile); return;
+                case 13: r_pauli<13>(a, o, h, m, jt, ph, tile); return;
+                case 14: r_pauli<14>(a, o, h, m, jt, ph, tile); return;
+                default: r_pauli<15>(a, o, h, m, jt, ph, tile); return;
             }
         default:
             copy_regs(a, o);
@@ -293,46 +320,112 @@ __device__ __forceinline__ void apply_reg_op(const Regs& a, Regs& o, const RegOp
     }
 }
 
-__global__ void __launch_bounds__(kRT) k_rtile(amp_t* __restrict__ s, uint32_t nq, uint64_t smask,
-                                              const RPhase* __restrict__ phases, uint32_t n_phases,
-                                              const RegOp* __restrict__ ops) {
-    __shared__ amp_t lds[1 << kRL];
-    amp_t* b = s + (uint64_t(blockIdx.y) << nq);
-    const uint64_t full = (uint64_t{1} << nq) - 1;
-    const uint64_t base = pdep64(blockIdx.x, full & ~smask);
-    Regs v, w;
-    {
-        const RPhase& p0 = phases[0];
-        const uint64_t gt = base | pdep64(threadIdx.x, p0.tmask_global);
+// Applies the staged op at word `off` and returns the next op's word.
+__device__ __forceinline__ uint32_t apply_staged(const Regs& a, Regs& o, const amp_t* words, uint32_t off,
+                                                 uint32_t jt, const RPhase& ph, uint32_t tile) {
+    const uint4* hp = reinterpret_cast<const uint4*>(words + off);
+    const uint4 h0 = hp[0], h1 = hp[1];
+    const uint32_t code = rfl(h0.x);
+    Hdr h;
+    h.kind = code & 15u;
+    h.b0 = (code >> 4) & 3u;
+    h.b1 = (code >> 6) & 3u;
+    h.xr = (code >> 8) & 15u;
+    h.rctl = (code >> 12) & 15u;
+    h.tctl = rfl(h0.y);
+    h.la = rfl(h0.z);
+    h.lb = rfl(h0.w);
+    h.gctrl = rfl(h1.x);
+    h.ga = rfl(h1.y);
+    h.gb = rfl(h1.z);
+    apply_reg_op(a, o, h, words + off + 2, jt, ph, tile);
+    return off + 2 + (code >> 16);
+}
+
+// s_waitcnt vmcnt(0) (gfx9 encoding: expcnt and lgkmcnt fields at their
+// maxima, i.e. not waited for).
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0xF70); }
+
+// Stages op words [off, off + n) of the pass into LDS (all threads).
+__device__ __forceinline__ void stage_words(amp_t* staged, const amp_t* words, uint32_t off, uint32_t n) {
 #pragma unroll
-        for (int r = 0; r < kRR; ++r) v[r] = b[gt | p0.gofs[r]];
+    for (uint32_t k = 0; k < kStage; ++k) {
+        const uint32_t i = threadIdx.x + k * kRT;
+        if (i < n) staged[i] = words[off + i];
     }
+}
+
+// Runs every phase of the pass on the tile held in v (phase 0's layout on
+// entry, the last phase's on exit). Segment 0 is already staged.
+__device__ __forceinline__ void run_phases(Regs& v, Regs& w, amp_t* lds, amp_t* staged,
+                                           const RPhase* __restrict__ phases, uint32_t n_phases,
+                                           const amp_t* __restrict__ words, uint32_t tile, bool& lds_used) {
     uint32_t sjt_prev = 0;
     for (uint32_t p = 0; p < n_phases; ++p) {
         const RPhase& ph = phases[p];
         const uint32_t jt = pdep32(threadIdx.x, ph.tmask);
         const uint32_t sjt = swz(jt);
-        if (p > 0) {
+        if (p > 0 && ph.transpose) {
             const RPhase& pp = phases[p - 1];
-            if (p > 1) __syncthreads();  // every thread has read the previous layout
+            if (lds_used) __syncthreads();  // every thread has read the previous layout
 #pragma unroll
             for (int r = 0; r < kRR; ++r) lds[sjt_prev ^ pp.sjr[r]] = v[r];
             __syncthreads();
 #pragma unroll
             for (int r = 0; r < kRR; ++r) v[r] = lds[sjt ^ ph.sjr[r]];
+            lds_used = true;
+        }
+        if (p > 0 && ph.seg_words) {
+            __syncthreads();  // every thread is done with the staged op words
+            stage_words(staged, words, ph.seg_off, ph.seg_words);
+            __syncthreads();
         }
         sjt_prev = sjt;
-        uint32_t o = ph.first_op;
-        const uint32_t end = ph.first_op + ph.n_ops;
-        for (; o + 1 < end; o += 2) {
-            apply_reg_op(v, w, ops[o], jt, ph, base);
-            apply_reg_op(w, v, ops[o + 1], jt, ph, base);
+        uint32_t off = ph.op_off;
+        uint32_t i = 0;
+        for (; i + 1 < ph.n_ops; i += 2) {
+            off = apply_staged(v, w, staged, off, jt, ph, tile);
+            off = apply_staged(w, v, staged, off, jt, ph, tile);
         }
-        if (o < end) {
-            apply_reg_op(v, w, ops[o], jt, ph, base);
+        if (i < ph.n_ops) {
+            apply_staged(v, w, staged, off, jt, ph, tile);
             copy_regs(w, v);
         }
     }
+}
+
+// One workgroup per tile.
+__global__ void __launch_bounds__(kRT) k_rtile(amp_t* __restrict__ s, uint32_t nq, uint64_t smask,
+                                              const RPhase* __restrict__ phases, uint32_t n_phases,
+                                              const amp_t* __restrict__ words) {
+    __shared__ amp_t lds[1 << kRL];
+    __shared__ amp_t staged[kRTileWords];
+    amp_t* b = s + (uint64_t(blockIdx.y) << nq);
+    const uint32_t tile = blockIdx.x;
+    const uint64_t base = pdep64(tile, ((uint64_t{1} << nq) - 1) & ~smask);
+    Regs v, w;
+    {
+        // Op words of the first segment, then the tile; the words go to LDS
+        // while the tile loads are still in flight.
+        const RPhase& p0 = phases[0];
+        amp_t stg[kStage];
+#pragma unroll
+        for (uint32_t k = 0; k < kStage; ++k) {
+            const uint32_t i = threadIdx.x + k * kRT;
+            stg[k] = i < p0.seg_words ? words[p0.seg_off + i] : make_double2(0.0, 0.0);
+        }
+        const uint64_t gt = base | pdep64(threadIdx.x, p0.tmask_global);
+#pragma unroll
+        for (int r = 0; r < kRR; ++r) v[r] = b[gt | p0.gofs[r]];
+#pragma unroll
+        for (uint32_t k = 0; k < kStage; ++k) {
+            const uint32_t i = threadIdx.x + k * kRT;
+            if (i < p0.seg_words) staged[i] = stg[k];
+        }
+        __syncthreads();
+    }
+    bool lds_used = false;
+    run_phases(v, w, lds, staged, phases, n_phases, words, tile, lds_used);
     {
         const RPhase& pl = phases[n_phases - 1];
         const uint64_t gt = base | pdep64(threadIdx.x, pl.tmask_global);
@@ -341,29 +434,39 @@ __global__ void __launch_bounds__(kRT) k_rtile(amp_t* __restrict__ s, uint32_t n
     }
 }
 
-// Persistent form: each workgroup walks tiles blockIdx.x, +gridDim.x, ...
-// and issues the global loads of its next tile before computing the current
-// one, so HBM reads overlap the in-register gate work.
-__global__ void __launch_bounds__(kRT) k_rtile_persist(amp_t* __restrict__ s, uint32_t nq, uint64_t smask,
-                                                      const RPhase* __restrict__ phases, uint32_t n_phases,
-                                                      const RegOp* __restrict__ ops, uint32_t tiles) {
+// Streaming form: each workgroup walks tiles blockIdx.x, + gridDim.x, ...
+// and issues the global loads of its next tile before running the phases
+// of the current one, so HBM reads overlap the in-register gate work.
+// `restage`: the pass has more than one op-word segment, so segment 0 is
+// staged again for every tile.
+__global__ void __launch_bounds__(kRT) k_rtile_stream(amp_t* __restrict__ s, uint32_t nq, uint64_t smask,
+                                                     const RPhase* __restrict__ phases, uint32_t n_phases,
+                                                     const amp_t* __restrict__ words, uint32_t tiles,
+                                                     uint32_t restage) {
     __shared__ amp_t lds[1 << kRL];
+    __shared__ amp_t staged[kRTileWords];
     amp_t* b = s + (uint64_t(blockIdx.y) << nq);
-    const uint64_t full = (uint64_t{1} << nq) - 1;
-    const uint64_t outside = full & ~smask;
+    const uint64_t outside = ((uint64_t{1} << nq) - 1) & ~smask;
     const RPhase& p0 = phases[0];
     const RPhase& pl = phases[n_phases - 1];
     const uint64_t gt0 = pdep64(threadIdx.x, p0.tmask_global);
     const uint64_t gtl = pdep64(threadIdx.x, pl.tmask_global);
-    Regs v, w, nx;
     uint32_t tile = blockIdx.x;
-    if (tile >= tiles) return;
+    Regs v, w, nx;
+    stage_words(staged, words, p0.seg_off, p0.seg_words);
     {
         const uint64_t gt = pdep64(tile, outside) | gt0;
 #pragma unroll
         for (int r = 0; r < kRR; ++r) v[r] = b[gt | p0.gofs[r]];
     }
-    for (; tile < tiles; tile += gridDim.x) {
+    __syncthreads();
+    // Explicit waits for the vector-memory counter (vmcnt 0) keep the
+    // compiler's own waits out of the phases: without them it cannot tell
+    // the next tile's loads from the current tile's and waits for both
+    // before the first gate.
+    wait_vmem();
+    bool lds_used = false;
+    for (;;) {
         const uint64_t base = pdep64(tile, outside);
         const uint32_t next = tile + gridDim.x;
         if (next < tiles) {
@@ -371,38 +474,21 @@ __global__ void __launch_bounds__(kRT) k_rtile_persist(amp_t* __restrict__ s, ui
 #pragma unroll
             for (int r = 0; r < kRR; ++r) nx[r] = b[gt | p0.gofs[r]];
         }
-        uint32_t sjt_prev = 0;
-        for (uint32_t p = 0; p < n_phases; ++p) {
-            const RPhase& ph = phases[p];
-            const uint32_t jt = pdep32(threadIdx.x, ph.tmask);
-            const uint32_t sjt = swz(jt);
-            if (p > 0) {
-                const RPhase& pp = phases[p - 1];
-                __syncthreads();  // every thread is done with the LDS tile
-#pragma unroll
-                for (int r = 0; r < kRR; ++r) lds[sjt_prev ^ pp.sjr[r]] = v[r];
-                __syncthreads();
-#pragma unroll
-                for (int r = 0; r < kRR; ++r) v[r] = lds[sjt ^ ph.sjr[r]];
-            }
-            sjt_prev = sjt;
-            uint32_t o = ph.first_op;
-            const uint32_t end = ph.first_op + ph.n_ops;
-            for (; o + 1 < end; o += 2) {
-                apply_reg_op(v, w, ops[o], jt, ph, base);
-                apply_reg_op(w, v, ops[o + 1], jt, ph, base);
-            }
-            if (o < end) {
-                apply_reg_op(v, w, ops[o], jt, ph, base);
-                copy_regs(w, v);
-            }
-        }
+        run_phases(v, w, lds, staged, phases, n_phases, words, tile, lds_used);
+        wait_vmem();  // the next tile has landed
         {
             const uint64_t gt = base | gtl;
 #pragma unroll
             for (int r = 0; r < kRR; ++r) b[gt | pl.gofs[r]] = v[r];
         }
-        if (next < tiles) copy_regs(nx, v);
+        if (next >= tiles) break;
+        if (restage) {
+            __syncthreads();  // every thread is done with the last segment
+            stage_words(staged, words, p0.seg_off, p0.seg_words);
+            __syncthreads();
+        }
+        copy_regs(nx, v);
+        tile = next;
     }
 }
 
@@ -411,6 +497,16 @@ uint64_t pdep_host(uint64_t v, uint64_t mask) {
     for (uint64_t bit = 1; mask; bit <<= 1) {
         const uint64_t low = mask & (~mask + 1);
         if (v & bit) r |= low;
+        mask ^= low;
+    }
+    return r;
+}
+
+uint32_t pext_host(uint64_t v, uint64_t mask) {
+    uint32_t r = 0, bit = 1;
+    for (; mask; bit <<= 1) {
+        const uint64_t low = mask & (~mask + 1);
+        if (v & low) r |= bit;
         mask ^= low;
     }
     return r;
@@ -433,46 +529,128 @@ void local_masks(const TileOp& t, uint32_t& touched, uint32_t& dense) {
     }
 }
 
+struct Group {
+    uint32_t reg = 0;  // register bits
+    std::vector<uint32_t> ops;
+};
+
+// First-fit grouping in program order: an op joins while its register bits
+// fit (at most 4, all within `allowed`) and no skipped earlier op shares a
+// bit with it. Returns the group and leaves the skipped ops in `pending`.
+Group take_forward(std::vector<uint32_t>& pending, const std::vector<uint32_t>& touched,
+                   const std::vector<uint32_t>& dense, uint32_t allowed) {
+    Group g;
+    uint32_t blocked = 0;
+    std::vector<uint32_t> rest;
+    for (uint32_t i : pending) {
+        if ((touched[i] & blocked) || (dense[i] & ~allowed) || popc(g.reg | dense[i]) > 4) {
+            blocked |= touched[i];
+            rest.push_back(i);
+            continue;
+        }
+        g.reg |= dense[i];
+        g.ops.push_back(i);
+    }
+    pending.swap(rest);
+    return g;
+}
+
+// The same from the end: ops that can run after everything left in `pending`.
+Group take_backward(std::vector<uint32_t>& pending, const std::vector<uint32_t>& touched,
+                    const std::vector<uint32_t>& dense, uint32_t allowed) {
+    Group g;
+    uint32_t blocked = 0;
+    std::vector<uint32_t> rest;
+    for (auto it = pending.rbegin(); it != pending.rend(); ++it) {
+        const uint32_t i = *it;
+        if ((touched[i] & blocked) || (dense[i] & ~allowed) || popc(g.reg | dense[i]) > 4) {
+            blocked |= touched[i];
+            rest.push_back(i);
+            continue;
+        }
+        g.reg |= dense[i];
+        g.ops.push_back(i);
+    }
+    std::reverse(g.ops.begin(), g.ops.end());
+    std::reverse(rest.begin(), rest.end());
+    pending.swap(rest);
+    return g;
+}
+
+uint32_t pad_high(uint32_t reg) {
+    for (int p = kRL - 1; p >= 0 && popc(reg) < 4; --p) reg |= 1u << p;
+    return reg;
+}
+
+uint32_t matrix_words(uint32_t kind) {
+    switch (kind) {
+        case RK_DENSE1: case RK_DIAG: return 4;
+        case RK_DENSE2: return 16;
+        case RK_PAULI: return 2;
+        default: return 0;
+    }
+}
+
 }  // namespace
 
-bool build_rtile(const std::vector<TileOp>& tops, uint32_t L, uint64_t smask, RTileProgram& out) {
+bool build_rtile(const std::vector<TileOp>& tops, uint32_t L, uint64_t smask, uint32_t nq, RTileProgram& out) {
     if (L != static_cast<uint32_t>(kRL)) return false;
     out.phases.clear();
-    out.ops.clear();
+    out.words.clear();
     const uint32_t all = (1u << kRL) - 1;
+    const uint64_t outside = ((nq >= 64 ? ~uint64_t{0} : (uint64_t{1} << nq) - 1)) & ~smask;
     std::vector<uint32_t> touched(tops.size()), dense(tops.size());
     for (size_t i = 0; i < tops.size(); ++i) {
         local_masks(tops[i], touched[i], dense[i]);
         if (popc(dense[i]) > 4) return false;
     }
+    // Register bits of the first and last phases avoid the tile's contiguous
+    // low qubits (tile bit p is qubit p for p below the run length), which
+    // then index the lanes and keep global accesses in contiguous runs.
+    const uint32_t run = static_cast<uint32_t>(__builtin_ctzll(~smask));
+    const uint32_t avoid = (1u << std::min<uint32_t>(run, 6)) - 1;
+    const uint32_t edge = all & ~avoid;
+
     std::vector<uint32_t> pending(tops.size());
     for (uint32_t i = 0; i < tops.size(); ++i) pending[i] = i;
-    std::vector<std::pair<uint32_t, std::vector<uint32_t>>> groups;  // (register bits, ops)
-    while (!pending.empty()) {
-        uint32_t C = 0, blocked = 0;
-        std::vector<uint32_t> taken, rest;
-        for (uint32_t i : pending) {
-            if ((touched[i] & blocked) || popc(C | dense[i]) > 4) {
-                blocked |= touched[i];
-                rest.push_back(i);
-                continue;
-            }
-            C |= dense[i];
-            taken.push_back(i);
-        }
-        groups.emplace_back(C, std::move(taken));
-        pending.swap(rest);
-    }
-    if (groups.empty()) groups.emplace_back(0u, std::vector<uint32_t>{});
-    for (auto& g : groups) {  // pad with the highest free tile bits
-        for (int p = kRL - 1; p >= 0 && popc(g.first) < 4; --p) g.first |= (1u << p) & ~g.first & all;
-    }
+    std::vector<Group> groups;
+    groups.push_back(take_forward(pending, touched, dense, edge));
+    Group last;
+    if (!pending.empty()) last = take_backward(pending, touched, dense, edge);
+    while (!pending.empty()) groups.push_back(take_forward(pending, touched, dense, all));
+    if (!last.ops.empty()) groups.push_back(std::move(last));
+    for (auto& g : groups) g.reg = pad_high(g.reg);
+    if (groups.size() > 1 && groups.front().ops.empty() && (groups[1].reg & avoid) == 0)
+        groups.erase(groups.begin());  // the second layout loads as well
+    if (groups.back().reg & avoid) groups.push_back(Group{pad_high(0), {}});  // store layout
+    // Neighbours with one layout run as one phase.
+    std::vector<Group> merged;
     for (auto& g : groups) {
+        if (!merged.empty() && merged.back().reg == g.reg) {
+            merged.back().ops.insert(merged.back().ops.end(), g.ops.begin(), g.ops.end());
+        } else {
+            merged.push_back(std::move(g));
+        }
+    }
+
+    // Encode: per phase its layout, per op a header and its matrix words.
+    // A phase whose words exceed the staging area continues in further
+    // phases of the same layout (no transpose).
+    auto put_hdr = [&](const RegOpHdr& h) {
+        amp_t w[2];
+        std::memcpy(w, &h, sizeof h);
+        out.words.push_back(w[0]);
+        out.words.push_back(w[1]);
+    };
+    uint32_t prev_reg = ~0u;
+    for (const Group& g : merged) {
+        const uint32_t rmask = g.reg;
         RPhase ph;
         std::memset(&ph, 0, sizeof ph);
-        const uint32_t rmask = g.first;
         ph.tmask = all & ~rmask;
         ph.tmask_global = pdep_host(ph.tmask, smask);
+        ph.transpose = (prev_reg != ~0u && prev_reg != rmask) ? 1u : 0u;
+        prev_reg = rmask;
         int rb[4], k = 0;
         for (int p = 0; p < kRL; ++p)
             if (rmask & (1u << p)) rb[k++] = p;
@@ -489,71 +667,87 @@ bool build_rtile(const std::vector<TileOp>& tops, uint32_t L, uint64_t smask, RT
                 if (rb[i] == static_cast<int>(pos)) return static_cast<uint32_t>(i);
             return 0u;
         };
-        auto rsplit = [&](uint32_t lmask, uint32_t& rctl, uint32_t& tctl) {
-            rctl = 0;
-            for (int i = 0; i < 4; ++i)
-                if (lmask & (1u << rb[i])) rctl |= 1u << i;
-            tctl = lmask & ph.tmask;
-        };
-        ph.first_op = static_cast<uint32_t>(out.ops.size());
-        for (uint32_t i : g.second) {
+        ph.op_off = static_cast<uint32_t>(out.words.size());  // absolute until segmented
+        for (uint32_t i : g.ops) {
             const TileOp& t = tops[i];
-            RegOp o;
-            std::memset(&o, 0, sizeof o);
-            o.gctrl = t.gctrl;
-            rsplit(t.lctrl, o.rctl, o.tctl);
+            RegOpHdr h;
+            std::memset(&h, 0, sizeof h);
+            uint32_t kind = 0, b0 = 0, b1 = 0, xr = 0, rctl = 0;
+            for (int q = 0; q < 4; ++q)
+                if (t.lctrl & (1u << rb[q])) rctl |= 1u << q;
+            h.tctl = t.lctrl & ph.tmask;
+            h.gctrl = pext_host(t.gctrl, outside);
             switch (t.kind) {
                 case TK_DENSE1:
-                    o.kind = is_x(t) ? RK_X : RK_DENSE1;
-                    o.b0 = ridx(t.t0);
-                    for (int e = 0; e < 4; ++e) o.m[e] = t.m[e];
+                    kind = is_x(t) ? RK_X : RK_DENSE1;
+                    b0 = ridx(t.t0);
                     break;
                 case TK_DENSE2:
-                    o.kind = RK_DENSE2;
-                    o.b0 = ridx(t.t0);
-                    o.b1 = ridx(t.t1);
-                    for (int e = 0; e < 16; ++e) o.m[e] = t.m[e];
+                    kind = RK_DENSE2;
+                    b0 = ridx(t.t0);
+                    b1 = ridx(t.t1);
                     break;
                 case TK_DIAG:
-                    o.kind = RK_DIAG;
-                    o.dl0 = t.dl0;
-                    o.dl1 = t.dl1;
-                    o.dg0 = t.dg0;
-                    o.dg1 = t.dg1;
-                    for (int e = 0; e < 4; ++e) o.m[e] = t.m[e];
+                    kind = RK_DIAG;
+                    h.la = t.dl0;
+                    h.lb = t.dl1;
+                    h.ga = pext_host(t.dg0, outside);
+                    h.gb = pext_host(t.dg1, outside);
                     break;
-                default: {
-                    o.kind = RK_PAULI;
-                    uint32_t xr = 0;
+                default:
+                    kind = RK_PAULI;
                     for (int bit = 0; bit < kRL; ++bit)
                         if (t.xl & (1u << bit)) xr |= 1u << ridx(bit);
-                    o.xr = xr;
-                    o.zl = t.zl;
-                    o.zg = t.zg;
-                    o.m[0] = t.m[0];
-                    o.m[1] = t.m[1];
+                    h.la = t.zl;
+                    h.ga = pext_host(t.zg, outside);
                     break;
-                }
             }
-            out.ops.push_back(o);
+            const uint32_t nm = matrix_words(kind);
+            h.code = kind | (b0 << 4) | (b1 << 6) | (xr << 8) | (rctl << 12) | (nm << 16);
+            if (out.words.size() + 2 + nm - ph.op_off > kRTileWords) {  // continue in a new phase
+                out.phases.push_back(ph);
+                ph.transpose = 0;
+                ph.op_off = static_cast<uint32_t>(out.words.size());
+                ph.n_ops = 0;
+            }
+            put_hdr(h);
+            for (uint32_t e = 0; e < nm; ++e) out.words.push_back(t.m[e]);
+            ++ph.n_ops;
         }
-        ph.n_ops = static_cast<uint32_t>(out.ops.size()) - ph.first_op;
         out.phases.push_back(ph);
+    }
+    // Segments: runs of consecutive phases whose words fit the staging area.
+    const uint32_t total = static_cast<uint32_t>(out.words.size());
+    auto end_of = [&](size_t i) { return i + 1 < out.phases.size() ? out.phases[i + 1].op_off : total; };
+    size_t p = 0;
+    while (p < out.phases.size()) {
+        const uint32_t start = out.phases[p].op_off;
+        size_t q = p + 1;
+        while (q < out.phases.size() && end_of(q) - start <= kRTileWords) ++q;
+        const uint32_t end = end_of(q - 1);
+        for (size_t i = p; i < q; ++i) {
+            out.phases[i].op_off -= start;
+            out.phases[i].seg_off = 0;
+            out.phases[i].seg_words = 0;
+        }
+        out.phases[p].seg_off = start;
+        out.phases[p].seg_words = end - start;
+        p = q;
     }
     return true;
 }
 
 void launch_rtile(amp_t* s, uint32_t nq, uint64_t smask, const RPhase* d_phases, uint32_t n_phases,
-                  const RegOp* d_ops, uint32_t batch) {
+                  const amp_t* d_words, bool restage, uint32_t batch) {
     const uint64_t tiles = uint64_t{1} << (nq - kRL);
     const uint64_t resident = uint64_t(num_cus()) * g_rtile_blocks_per_cu;
-    if (g_rtile_persistent && tiles > resident) {
-        k_rtile_persist<<<dim3(static_cast<unsigned>(resident), batch), kRT, 0, stream()>>>(
-            s, nq, smask, d_phases, n_phases, d_ops, static_cast<uint32_t>(tiles));
+    if (g_rtile_stream && tiles > resident) {
+        k_rtile_stream<<<dim3(static_cast<unsigned>(resident), batch), kRT, 0, stream()>>>(
+            s, nq, smask, d_phases, n_phases, d_words, static_cast<uint32_t>(tiles), restage ? 1u : 0u);
         return;
     }
     k_rtile<<<dim3(static_cast<unsigned>(tiles), batch), kRT, 0, stream()>>>(s, nq, smask, d_phases, n_phases,
-                                                                            d_ops);
+                                                                            d_words);
 }
 
 }  // namespace lqcs_dev

@@ -243,7 +243,7 @@ def test_errors_keep_reference_types(lq):
         _ = sv[4]
 
 
-@pytest.mark.parametrize("kernel", ["persistent", "register", "lds"])
+@pytest.mark.parametrize("kernel", ["register", "stream", "lds"])
 @pytest.mark.parametrize("plan,low", [("greedy", 4), ("learned", 3), ("learned", 2), ("greedy", 5)])
 def test_tile_kernels_and_planners_match_oracle(lq, kernel, plan, low):
     n = 17
@@ -255,5 +255,5 @@ def test_tile_kernels_and_planners_match_oracle(lq, kernel, plan, low):
     try:
         got = product_state(lq, c, True, plan=plan, tile_qubits=12, low_qubits=low)
     finally:
-        lq.set_tile_kernel("persistent")
+        lq.set_tile_kernel("register")
     assert np.max(np.abs(got - want)) <= TOL
