@@ -9,7 +9,13 @@ across steps; the timed region excludes host transfers.
 
 Prints one JSON line: gates/s (whole job), the roofline of the dominant
 kernel (tiled pass, HIP-event timed), and the reference's OpenMP path timed
-on this host on a bounded sample of the same workload.
+on this host on a bounded sample of the same workload. A second measurement
+covers the metric's other half: batched adjoint gradients of 256 Pauli-
+rotation ansatze (16-24 qubits, 400 rotations, 200-term Hamiltonians) against
+parameter-shift gradients on the same GPU (timed on a bounded sample).
+
+`traffic` (HBM bytes per tiled pass) comes from the rocprofv3 counter summary
+committed under profiles/ for the same kernel and workload, when one exists.
 """
 from __future__ import annotations
 
@@ -20,6 +26,7 @@ import sys
 import time
 
 PEAK_HBM_GBPS = 8000.0
+ROOT = os.path.dirname(os.path.abspath(__file__))
 
 
 def log(msg: str) -> None:
@@ -68,6 +75,80 @@ def cpu_baseline(n: int, layers: int, seed: int, budget_s: float) -> dict | None
     }
 
 
+def measured_traffic(kernel: str, n: int, layers: int, plan: str, tile: int, low: int) -> tuple[float | None, str | None]:
+    """HBM bytes per launch of `kernel` on this workload, from the counter
+    summary profiles/<kernel>_random_u3cx_<n>q.json: FETCH_SIZE and WRITE_SIZE
+    per dispatch (KiB), each scaled by its copy-kernel calibration factor."""
+    name = f"{kernel}_random_u3cx_{n}q.json"
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            pm = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    want = {"qubits": n, "layers": layers, "plan": plan, "tile_qubits": tile, "low_qubits": low}
+    if any(pm.get("workload", {}).get(k) != v for k, v in want.items()):
+        return None, None
+    kib = pm["fetch_kib_per_dispatch"] * pm["fetch_scale"] + pm["write_kib_per_dispatch"] * pm["write_scale"]
+    return kib * 1024.0, f"profiles/{name}"
+
+
+def adjoint_bench(lq, W, seed: int, n_circuits: int, ps_seconds: float) -> dict:
+    """Batched adjoint gradients of n_circuits Pauli-rotation ansatze of uneven
+    width (16-24 qubits, 400 rotations each, 200-term Hamiltonians) against
+    the parameter-shift gradient, both on this GPU. Parameter shift is timed
+    on whole circuits of the batch, in order, until ps_seconds have passed;
+    adjoint is timed on the whole batch and on that same sample."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    sizes = [int(x) for x in rng.integers(16, 25, size=n_circuits)]
+    pcs, thetas, hams = [], [], []
+    for i, n in enumerate(sizes):
+        rots, theta = W.random_pauli_ansatz(n, 400, seed + 1 + i)
+        pc = lq.ParametricCircuit(n)
+        for p, qs in rots:
+            pc.rp(p, qs)
+        pcs.append(pc)
+        thetas.append([float(t) for t in theta])
+        hams.append(W.random_hamiltonian(n, 200, seed + 100_000 + i))
+    lq.algorithms.adjoint_gradient_batched(hams[:2], pcs[:2], thetas[:2])  # warm-up
+    lq.synchronize()
+    t0 = time.perf_counter()
+    full = lq.algorithms.adjoint_gradient_batched(hams, pcs, thetas)
+    lq.synchronize()
+    t_adj = time.perf_counter() - t0
+    n_grads = sum(len(t) for t in thetas)
+    log(f"adjoint: {n_circuits} circuits, {n_grads} gradients in {t_adj:.3f} s")
+    ps, t_ps, k = [], 0.0, 0
+    while k < n_circuits and (k == 0 or t_ps < ps_seconds):
+        t0 = time.perf_counter()
+        ps.append(lq.algorithms.parameter_shift_gradient(hams[k], pcs[k], thetas[k]))
+        lq.synchronize()
+        t_ps += time.perf_counter() - t0
+        k += 1
+        log(f"parameter shift: circuit {k} ({sizes[k - 1]} qubits) done, {t_ps:.2f} s")
+    t0 = time.perf_counter()
+    same = lq.algorithms.adjoint_gradient_batched(hams[:k], pcs[:k], thetas[:k])
+    lq.synchronize()
+    t_adj_sample = time.perf_counter() - t0
+    err = max(float(np.max(np.abs(np.array(g) - np.array(p)))) for (_, g), p in zip(same, ps))
+    err_full = max(float(np.max(np.abs(np.array(g) - np.array(h)))) for (_, g), (_, h) in zip(same, full[:k]))
+    sample_grads = sum(len(t) for t in thetas[:k])
+    return {
+        "workload": (f"{n_circuits} Pauli-rotation ansatze, 16-24 qubits (uneven), 400 rotations of weight 2-8, "
+                     "200-term Hamiltonians, theta ~ U(-pi, pi)"),
+        "adjoint_grads_per_s": round(n_grads / t_adj, 1),
+        "adjoint_s": round(t_adj, 3),
+        "param_shift_grads_per_s": round(sample_grads / t_ps, 2),
+        "param_shift_sample": f"first {k} circuit(s) of the batch, widths {sizes[:k]}, {sample_grads} gradients",
+        "adjoint_grads_per_s_on_sample": round(sample_grads / t_adj_sample, 1),
+        "speedup_on_sample": round((sample_grads / t_adj_sample) / (sample_grads / t_ps), 1),
+        "max_abs_diff_vs_param_shift": err,
+        "max_abs_diff_batch_vs_sample_run": err_full,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +162,11 @@ def main() -> int:
     ap.add_argument("--low-qubits", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--adjoint-circuits", type=int, default=256)
+    ap.add_argument("--ps-seconds", type=float, default=20.0)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="end with one device-to-device copy of the register (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -129,17 +215,34 @@ def main() -> int:
         log(f"norm check failed: {norm}")
         return 1
 
+    if args.copy_calibration:
+        sv2 = lq.statevector(n)
+        sv2.copy_from(sv)
+        lq.synchronize()
+        log(f"copy calibration: one device-to-device copy of {32 << n >> 1} bytes")
+        del sv2
+
     pass_bytes = 32.0 * (1 << n)  # one read + one write of 2^n complex128
     if prof["tile_launches"]:
         avg_s = prof["tile_ms"] / 1e3 / prof["tile_launches"]
-        kernel = f"k_tile_pass<L={args.tile_qubits or 12}>"
+        kernel = "k_rtile"  # the register-tile kernel runs every tiled pass of this plan
     else:
         avg_s = prof["other_ms"] / 1e3 / max(1, prof["other_launches"])
         kernel = "gate-by-gate kernels"
     achieved = pass_bytes / avg_s / 1e9
+    tile_q = args.tile_qubits or 12
+    low_q = args.low_qubits or 4
+    traffic, traffic_src = measured_traffic(kernel, n, layers, args.plan, tile_q, low_q)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None, "kernel": kernel,
-                "bytes_per_launch": pass_bytes, "avg_launch_ms": round(avg_s * 1e3, 4)}
+                "frac": round(achieved / PEAK_HBM_GBPS, 4),
+                "traffic": round(traffic) if traffic is not None else None, "traffic_source": traffic_src,
+                "kernel": kernel, "bytes_per_launch": pass_bytes, "avg_launch_ms": round(avg_s * 1e3, 4),
+                "launches_per_step": prof["tile_launches"] // max(1, args.steps)}
+
+    adjoint = None
+    if not args.no_adjoint:
+        del sv
+        adjoint = adjoint_bench(lq, W, args.seed, args.adjoint_circuits, args.ps_seconds)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(n, layers, args.seed, args.cpu_seconds)
 
@@ -160,9 +263,11 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"random U3+CX layers, {n} qubits, {layers} layers, greedy 1q fusion",
                    "qubits": n, "layers": layers, "gates": gates, "plan": args.plan,
+                   "tile_qubits": tile_q, "low_qubits": low_q,
                    "passes_per_step": stats["passes"], "seed": args.seed},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
     return 0
