@@ -122,3 +122,15 @@ def qft_basis(n, x):
 def dft(n, x):
     k = np.arange(1 << n)
     return np.exp(2j * np.pi * ((x * k) % (1 << n)) / (1 << n)) / math.sqrt(1 << n)
+
+
+def noisy_ghz(n, layers=10, seed=7, p=1e-3, gamma=1e-3):
+    """The noisy-trajectory workload at width n: GHZ prep + `layers` random
+    U3/CZ layers, depolarizing(p) then amplitude_damping(gamma) after every
+    gate, measure_all. Returns (circuit, channels)."""
+    from litequantumcomputingsimulator_amd import workloads as W
+
+    c = O.Circuit(n, n)
+    W.noisy_ghz_layers(c, n, layers, seed)
+    c.measure_all()
+    return c, [O.depolarizing(p), O.amplitude_damping(gamma)]

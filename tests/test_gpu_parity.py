@@ -257,3 +257,69 @@ def test_tile_kernels_and_planners_match_oracle(lq, kernel, plan, low):
     finally:
         lq.set_tile_kernel("register")
     assert np.max(np.abs(got - want)) <= TOL
+
+
+def _noise_model(lq, p=1e-3, gamma=1e-3):
+    noise = lq.NoiseModel()
+    noise.add_all_qubit_channel(lq.KrausChannel.depolarizing(p))
+    noise.add_all_qubit_channel(lq.KrausChannel.amplitude_damping(gamma))
+    return noise
+
+
+def test_noisy_trajectories_12q_match_oracle_counts(lq):
+    """The noisy-trajectory workload at 12 qubits: batched device trajectories
+    reproduce the oracle's per-shot counts exactly (same draws)."""
+    from tests import cases
+
+    c, chans = cases.noisy_ghz(12)
+    want = O.run_counts(c, seed=11, shots=256, channels=chans)
+    got = lq.StatevectorSimulator(seed=11, noise=_noise_model(lq)).run(c.to_product(lq), 256).counts()
+    assert dict(got) == want
+
+
+@pytest.mark.parametrize("fixture,n,rate", [("dm_noisy_ghz12.npy", 12, 1e-3), ("dm_noisy_ghz10_strong.npy", 10, 0.05)])
+def test_noisy_trajectories_match_density_matrix(lq, fixture, n, rate):
+    """2048 device trajectories of the noisy workload against the exact
+    distribution from the reference's density-matrix simulator
+    (tests/golden/make_dm_fixture.py): every single-qubit marginal and the
+    Hamming-weight histogram within 5 sigma, and a chi-square over the
+    bitstrings (pooled, in order of probability, into groups expecting at
+    least 5 counts) within 5 sigma of its degrees of freedom. At rate 0.05 the
+    check fails for trajectories without noise or without the damping
+    channel (tried on the oracle); at 1e-3, 2048 shots cannot tell noise
+    from none, so the 12-qubit case is a consistency check."""
+    import os
+
+    from tests import cases
+
+    shots = 2048
+    p = np.load(os.path.join(os.path.dirname(__file__), "golden", fixture))
+    c, _ = cases.noisy_ghz(n, p=rate, gamma=rate)
+    counts = lq.StatevectorSimulator(seed=2024, noise=_noise_model(lq, rate, rate)).run(c.to_product(lq), shots).counts()
+    freq = np.zeros(1 << n)
+    for key, v in dict(counts).items():
+        freq[int(key, 2)] += v
+    assert freq.sum() == shots
+    idx = np.arange(1 << n)
+
+    def within(obs, prob):
+        sigma = np.sqrt(shots * prob * (1 - prob))
+        return np.all(np.abs(obs - shots * prob) <= 5 * sigma + 1)
+
+    for q in range(n):
+        bit = (idx >> q) & 1 == 1
+        assert within(freq[bit].sum(), p[bit].sum()), q
+    weight = np.array([bin(i).count("1") for i in idx])
+    for w in range(n + 1):
+        assert within(freq[weight == w].sum(), p[weight == w].sum()), w
+    order = np.argsort(p)
+    chi2, groups, e_acc, o_acc = 0.0, 0, 0.0, 0.0
+    for i in order:
+        e_acc += shots * p[i]
+        o_acc += freq[i]
+        if e_acc >= 5.0:
+            chi2 += (o_acc - e_acc) ** 2 / e_acc
+            groups += 1
+            e_acc = o_acc = 0.0
+    dof = groups - 1
+    assert chi2 <= dof + 5 * np.sqrt(2 * dof), (chi2, dof)
