@@ -5,6 +5,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "../circuit/quantum_circuit.hpp"
@@ -34,6 +35,10 @@ public:
     };
     std::vector<PassInfo> describe(const EngineOptions& engine = {}) const;
 
+    // Cost-model features of every pass of that plan (kCostFeatures each,
+    // row-major), for `batch` registers per launch.
+    std::vector<double> features(const EngineOptions& engine = {}, std::uint32_t batch = 1) const;
+
     std::size_t num_qubits() const { return nq_; }
     std::size_t num_ops() const;
 
@@ -53,5 +58,33 @@ struct LaunchProfile {
 };
 void profile_enable(bool on);
 LaunchProfile profile_read();
+// Per-launch milliseconds (and whether each was a tiled pass) since the last
+// read, in launch order.
+struct LaunchTimes {
+    std::vector<double> ms;
+    std::vector<std::uint8_t> tiled;
+};
+LaunchTimes profile_read_launches();
+
+// ---- learned planning cost model (engine extension) ----
+constexpr std::uint32_t kCostFeatures = 24;
+struct CostModel {
+    std::vector<double> params;   // packed MLP parameters (mlp_num_params(kCostFeatures))
+    std::vector<double> x_mean, x_scale;
+    double t_mean = 0.0, t_scale = 1.0;
+};
+std::uint64_t mlp_num_params(std::uint32_t n_features);
+// Installs (or, with empty params, removes) the model Plan::Learned consults.
+void install_cost_model(const CostModel& m);
+// Reads a model written by litequantumcomputingsimulator_amd.costmodel:
+// "lqcs-costmodel 1 <F> <n_params>" then x_mean, x_scale, t_mean, t_scale
+// and the parameters, whitespace separated.
+CostModel load_cost_model(const std::string& path);
+struct PlanChoice {
+    EngineOptions::Plan plan;
+    int low_qubits;
+    double predicted_ms;  // -1 when no model decided
+};
+PlanChoice last_plan_choice();
 
 }  // namespace lqcs

@@ -8,7 +8,10 @@
 namespace lqcs {
 
 struct EngineOptions {
-    enum class Plan : int { GateByGate = 0, Greedy = 1, Learned = 2, Auto = 3 };
+    // Greedy: first-fit tile sets in program order. Gain: tile sets grown by
+    // marginal gain. Learned: the installed cost model's pick among Greedy
+    // and Gain plans at 2..4 low qubits (Greedy when no model is installed).
+    enum class Plan : int { GateByGate = 0, Greedy = 1, Learned = 2, Auto = 3, Gain = 4 };
     Plan plan = Plan::Auto;
     int tile_qubits = 0;            // LDS tile width L (0 = engine default)
     int low_qubits = 0;             // contiguous low qubits per tile (0 = default)

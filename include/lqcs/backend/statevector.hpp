@@ -44,6 +44,8 @@ public:
 
     // Engine extension: reinitialise the register to |index>.
     void set_basis(std::size_t index);
+    // Engine extension: max_i |a_i - other_i|, computed on the device.
+    double max_abs_diff(const Statevector& other) const;
 
     // Engine access: the device register, made current first.
     lqcs_dev_state* device() const;

@@ -102,9 +102,11 @@ typedef struct lqcs_program {
 /* Execution plan choice for lqcs_state_apply. */
 enum {
     LQCS_PLAN_GATE_BY_GATE = 0,  /* one reference-seam kernel per op */
-    LQCS_PLAN_GREEDY = 1,        /* LDS-tiled passes, greedy grouping */
-    LQCS_PLAN_LEARNED = 2,       /* LDS-tiled passes, cost-model guided */
-    LQCS_PLAN_AUTO = 3           /* engine default */
+    LQCS_PLAN_GREEDY = 1,        /* tiled passes, first-fit grouping in program order */
+    LQCS_PLAN_LEARNED = 2,       /* the installed cost model's pick among GREEDY and GAIN
+                                    plans with 2..4 low qubits; GREEDY when none is installed */
+    LQCS_PLAN_AUTO = 3,          /* engine default (GREEDY) */
+    LQCS_PLAN_GAIN = 4           /* tiled passes, tile sets grown by marginal gain */
 };
 
 typedef struct lqcs_exec_options {
@@ -143,9 +145,51 @@ lqcs_status lqcs_profile_enable(int enable);
 lqcs_status lqcs_profile_read(uint64_t* tile_launches, double* tile_ms, uint64_t* other_launches,
                               double* other_ms);
 
+/* Launch times of the same recording, one entry per launch in order
+ * (tiled[i] = 1 for a tiled pass), then cleared; n_launches receives the
+ * count, arrays hold max_launches entries. */
+lqcs_status lqcs_profile_read_launches(double* ms, uint8_t* tiled, uint64_t max_launches,
+                                       uint64_t* n_launches);
+
+/* ---- learned cost model (no counterpart in the reference; it steers the
+ * planner in place of the reference's fixed greedy fusion). A multilayer
+ * perceptron F -> 64 -> 64 -> 1 (ReLU), FP64, trained and evaluated on the
+ * device. Parameters are packed as W1 [F][64] (input-major), b1 [64],
+ * W2 [64][64], b2 [64], w3 [64], b3. */
+#define LQCS_COST_FEATURES 24
+lqcs_status lqcs_mlp_num_params(uint32_t n_features, uint64_t* n_params);
+/* y[r] for each of n_rows rows of x (row-major, n_features columns). */
+lqcs_status lqcs_mlp_forward(uint32_t n_features, const double* params, const double* x,
+                             uint64_t n_rows, double* y);
+/* Mean squared error over the rows and its gradient (packed like params). */
+lqcs_status lqcs_mlp_loss_grad(uint32_t n_features, const double* params, const double* x,
+                               const double* t, uint64_t n_rows, double* loss, double* grad);
+/* Adam on shuffled minibatches (mt19937_64(seed) permutation per epoch),
+ * params updated in place; final_loss = loss over all rows afterwards. */
+lqcs_status lqcs_mlp_train(uint32_t n_features, double* params, const double* x, const double* t,
+                           uint64_t n_rows, uint32_t epochs, uint32_t batch_rows, double lr,
+                           uint64_t seed, double* final_loss);
+/* Installs the model LQCS_PLAN_LEARNED consults: predicted milliseconds of a
+ * plan step = exp(t_mean + t_scale * net((features - x_mean) / x_scale)).
+ * params NULL uninstalls it. */
+lqcs_status lqcs_costmodel_install(uint32_t n_features, const double* params, const double* x_mean,
+                                   const double* x_scale, double t_mean, double t_scale);
+/* Features of every step of the plan lqcs_state_apply would run for these
+ * options on a register of n_qubits (batch members per launch), in step
+ * order, LQCS_COST_FEATURES values per step; arrays hold max_steps rows. */
+lqcs_status lqcs_plan_features(uint32_t n_qubits, const lqcs_program* prog, const lqcs_exec_options* opts,
+                               uint32_t batch, double* features, uint64_t max_steps, uint64_t* n_steps);
+/* Plan kind and low-qubit count the last apply on this thread ran, and the
+ * cost model's predicted milliseconds for it (-1 when no model decided). */
+lqcs_status lqcs_last_plan_choice(int32_t* plan, int32_t* low_qubits, double* predicted_ms);
+
 /* ---- reductions and measurement ---------------------------------------- */
 /* Statevector::norm (statevector.cpp:27-31) squared, before the sqrt. */
 lqcs_status lqcs_state_norm_sq(const lqcs_dev_state* s, double* out);
+/* max_i |a_i - b_i| of two registers of one width, on the device (no
+ * counterpart in the reference: compares results where a host copy of the
+ * register is impractical; NaN when any difference is NaN). */
+lqcs_status lqcs_state_max_abs_diff(const lqcs_dev_state* a, const lqcs_dev_state* b, double* out);
 /* Statevector::probabilities (statevector.cpp:33-37) into host memory. */
 lqcs_status lqcs_state_probabilities(const lqcs_dev_state* s, double* out);
 /* Statevector::expectation_value (statevector.cpp:39-78) with the string

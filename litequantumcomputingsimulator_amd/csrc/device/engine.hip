@@ -54,8 +54,12 @@ bool g_use_rtile = true;
 
 // Launch timing state (lqcs_profile_*).
 struct Profile {
+    struct Launch {
+        bool tiled;
+        hipEvent_t a, b;
+    };
     bool on = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> tile, other;
+    std::vector<Launch> launches;  // in launch order
     std::vector<hipEvent_t> pool;
 };
 Profile& prof() {
@@ -74,6 +78,22 @@ hipEvent_t take_event() {
     return e;
 }
 thread_local uint64_t g_last_passes = 0, g_last_launches = 0;
+
+// Elapsed milliseconds of every recorded launch, in order; recycles events.
+std::vector<std::pair<bool, double>> drain_launches() {
+    Profile& p = prof();
+    std::vector<std::pair<bool, double>> out;
+    out.reserve(p.launches.size());
+    for (const auto& l : p.launches) {
+        float t = 0.0f;
+        LQCS_HIP(hipEventElapsedTime(&t, l.a, l.b));
+        out.emplace_back(l.tiled, double(t));
+        p.pool.push_back(l.a);
+        p.pool.push_back(l.b);
+    }
+    p.launches.clear();
+    return out;
+}
 
 std::mutex& api_mutex() {
     static std::mutex m;
@@ -500,7 +520,7 @@ void execute_plan(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan,
         }
         if (timed) {
             LQCS_HIP(hipEventRecord(e1, stream()));
-            (st.tiled ? prof().tile : prof().other).emplace_back(e0, e1);
+            prof().launches.push_back({st.tiled, e0, e1});
         }
     }
     g_last_passes = plan.steps.size();
@@ -533,17 +553,32 @@ void apply_ops(amp_t* s, uint32_t nq, const std::vector<DecodedOp>& ops, int pla
             run_single_op(s, nq, d, batch);
             if (timed) {
                 LQCS_HIP(hipEventRecord(e1, stream()));
-                prof().other.emplace_back(e0, e1);
+                prof().launches.push_back({false, e0, e1});
             }
         }
         g_last_passes = ops.size();
         g_last_launches = ops.size();
         return;
     }
+    const Plan plan = make_plan(plan_kind, ops, nq, L, low);
+    execute_plan(s, nq, L, plan, ops, batch);
+}
+
+bool rtile_enabled() { return g_use_rtile; }
+
+Plan make_plan(int kind, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t low) {
+    Plan plan;
+    if (kind == LQCS_PLAN_GATE_BY_GATE || L == 0) {
+        for (uint32_t i = 0; i < ops.size(); ++i) {
+            PlanStep st;
+            st.op = i;
+            plan.steps.push_back(st);
+        }
+        return plan;
+    }
     if (L < 6 || L > 12 || L > nq) throw Error(LQCS_ERR_INVALID_ARGUMENT, "tile width must be 6..12 and <= n");
     if (low + 2 > L) throw Error(LQCS_ERR_INVALID_ARGUMENT, "low qubits must leave two free tile slots");
-    const Plan plan = plan_greedy(ops, nq, L, low);
-    execute_plan(s, nq, L, plan, ops, batch);
+    return kind == LQCS_PLAN_GAIN ? plan_gain(ops, nq, L, low) : plan_greedy(ops, nq, L, low);
 }
 
 void set_last_stats(uint64_t passes, uint64_t launches) {
@@ -654,22 +689,30 @@ lqcs_status lqcs_profile_read(uint64_t* tile_launches, double* tile_ms, uint64_t
     return guarded([&] {
         ensure_ready();
         sync();
-        Profile& p = prof();
-        auto drain = [&](std::vector<std::pair<hipEvent_t, hipEvent_t>>& v, uint64_t* n, double* ms) {
-            double total = 0.0;
-            for (auto& [a, b] : v) {
-                float t = 0.0f;
-                LQCS_HIP(hipEventElapsedTime(&t, a, b));
-                total += t;
-                p.pool.push_back(a);
-                p.pool.push_back(b);
-            }
-            if (n) *n = v.size();
-            if (ms) *ms = total;
-            v.clear();
-        };
-        drain(p.tile, tile_launches, tile_ms);
-        drain(p.other, other_launches, other_ms);
+        uint64_t n[2] = {0, 0};
+        double ms[2] = {0.0, 0.0};
+        for (const auto& [tiled, t] : drain_launches()) {
+            ++n[tiled ? 1 : 0];
+            ms[tiled ? 1 : 0] += t;
+        }
+        if (tile_launches) *tile_launches = n[1];
+        if (tile_ms) *tile_ms = ms[1];
+        if (other_launches) *other_launches = n[0];
+        if (other_ms) *other_ms = ms[0];
+    });
+}
+
+lqcs_status lqcs_profile_read_launches(double* ms, uint8_t* tiled, uint64_t max_launches, uint64_t* n_launches) {
+    return guarded([&] {
+        need(n_launches, "n_launches");
+        ensure_ready();
+        sync();
+        const auto all = drain_launches();
+        *n_launches = all.size();
+        for (uint64_t i = 0; i < all.size() && i < max_launches; ++i) {
+            if (ms) ms[i] = all[i].second;
+            if (tiled) tiled[i] = all[i].first ? 1 : 0;
+        }
     });
 }
 
@@ -771,14 +814,9 @@ lqcs_status lqcs_state_apply(lqcs_dev_state* s, const lqcs_program* prog, const 
         std::vector<DecodedOp> ops;
         ops.reserve(prog->n_ops);
         for (uint64_t k = 0; k < prog->n_ops; ++k) ops.push_back(decode(*prog, k, s->nq));
-        const int plan = opts ? opts->plan : LQCS_PLAN_AUTO;
-        const uint32_t L = opts ? static_cast<uint32_t>(std::max(0, opts->tile_qubits)) : 0;
-        const uint32_t low = opts ? static_cast<uint32_t>(std::max(0, opts->low_qubits)) : 0;
-        if (plan == LQCS_PLAN_LEARNED) {
-            apply_learned(s->d, s->nq, ops, L, low, 1);
-        } else {
-            apply_ops(s->d, s->nq, ops, plan, L, low, 1);
-        }
+        uint32_t L = 0;
+        const Plan plan = choose_plan(opts, ops, s->nq, 1, L);
+        execute_plan(s->d, s->nq, L, plan, ops, 1);
     });
 }
 
@@ -791,22 +829,8 @@ lqcs_status lqcs_plan_describe(uint32_t n_qubits, const lqcs_program* prog, cons
         if (n_qubits == 0 || n_qubits > LQCS_DEVICE_MAX_QUBITS) throw Error(LQCS_ERR_INVALID_ARGUMENT, "bad width");
         std::vector<DecodedOp> ops;
         for (uint64_t k = 0; k < prog->n_ops; ++k) ops.push_back(decode(*prog, k, n_qubits));
-        const int kind = opts ? opts->plan : LQCS_PLAN_AUTO;
-        const TileChoice def = default_tile(n_qubits);
-        uint32_t L = opts && opts->tile_qubits > 0 ? static_cast<uint32_t>(opts->tile_qubits) : def.L;
-        uint32_t low = opts && opts->low_qubits > 0 ? static_cast<uint32_t>(opts->low_qubits) : def.low;
-        Plan plan;
-        if (kind == LQCS_PLAN_GATE_BY_GATE || L == 0) {
-            for (uint32_t i = 0; i < ops.size(); ++i) {
-                PlanStep st;
-                st.op = i;
-                plan.steps.push_back(st);
-            }
-        } else if (kind == LQCS_PLAN_LEARNED) {
-            plan = plan_gain(ops, n_qubits, L, low);
-        } else {
-            plan = plan_greedy(ops, n_qubits, L, low);
-        }
+        uint32_t L = 0;
+        const Plan plan = choose_plan(opts, ops, n_qubits, 1, L);
         *n_steps = plan.steps.size();
         for (uint64_t i = 0; i < plan.steps.size() && i < max_steps; ++i) {
             if (ops_per_step) ops_per_step[i] = plan.steps[i].tiled ? plan.steps[i].tile_ops.size() : 1;
@@ -820,6 +844,34 @@ lqcs_status lqcs_plan_describe(uint32_t n_qubits, const lqcs_program* prog, cons
             }
         }
     });
+}
+
+lqcs_status lqcs_plan_features(uint32_t n_qubits, const lqcs_program* prog, const lqcs_exec_options* opts,
+                               uint32_t batch, double* features, uint64_t max_steps, uint64_t* n_steps) {
+    return guarded([&] {
+        need(prog, "program");
+        need(n_steps, "n_steps");
+        if (n_qubits == 0 || n_qubits > LQCS_DEVICE_MAX_QUBITS) throw Error(LQCS_ERR_INVALID_ARGUMENT, "bad width");
+        std::vector<DecodedOp> ops;
+        for (uint64_t k = 0; k < prog->n_ops; ++k) ops.push_back(decode(*prog, k, n_qubits));
+        uint32_t L = 0;
+        const Plan plan = choose_plan(opts, ops, n_qubits, std::max<uint32_t>(batch, 1), L);
+        *n_steps = plan.steps.size();
+        if (!features) return;
+        for (uint64_t i = 0; i < plan.steps.size() && i < max_steps; ++i)
+            step_features(plan.steps[i], ops, n_qubits, L, std::max<uint32_t>(batch, 1),
+                          features + i * LQCS_COST_FEATURES);
+    });
+}
+
+lqcs_status lqcs_last_plan_choice(int32_t* plan, int32_t* low_qubits, double* predicted_ms) {
+    int p = 0, l = 0;
+    double ms = 0.0;
+    last_plan_choice(&p, &l, &ms);
+    if (plan) *plan = p;
+    if (low_qubits) *low_qubits = l;
+    if (predicted_ms) *predicted_ms = ms;
+    return LQCS_OK;
 }
 
 lqcs_status lqcs_last_plan_stats(uint64_t* passes, uint64_t* launches) {
@@ -836,6 +888,21 @@ lqcs_status lqcs_state_norm_sq(const lqcs_dev_state* s, double* out) {
         arena_reset();
         double* d = static_cast<double*>(arena(sizeof(double)));
         reduce_norm_sq(s->d, s->nq, 1, d);
+        LQCS_HIP(hipMemcpyAsync(out, d, sizeof(double), hipMemcpyDeviceToHost, stream()));
+        sync();
+    });
+}
+
+lqcs_status lqcs_state_max_abs_diff(const lqcs_dev_state* a, const lqcs_dev_state* b, double* out) {
+    return guarded([&] {
+        need(a, "a");
+        need(b, "b");
+        need(out, "out");
+        if (a->nq != b->nq) throw Error(LQCS_ERR_INVALID_ARGUMENT, "registers differ in width");
+        ensure_ready();
+        arena_reset();
+        double* d = static_cast<double*>(arena(sizeof(double)));
+        reduce_max_abs_diff(a->d, b->d, a->nq, d);
         LQCS_HIP(hipMemcpyAsync(out, d, sizeof(double), hipMemcpyDeviceToHost, stream()));
         sync();
     });

@@ -60,6 +60,21 @@ void apply_ops(amp_t* s, uint32_t nq, const std::vector<DecodedOp>& ops, int pla
 // Cost-model guided planning (planner.hip); falls back to greedy.
 void apply_learned(amp_t* s, uint32_t nq, const std::vector<DecodedOp>& ops, uint32_t L,
                    uint32_t low, uint32_t batch);
+// Plan for GATE_BY_GATE / GREEDY / GAIN at a fixed tile width and low count.
+Plan make_plan(int kind, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t low);
+// The plan lqcs_state_apply runs for these options (LEARNED: the cost
+// model's pick among greedy and gain plans at 2..4 low qubits, or greedy
+// without a model); L receives the tile width (0: gate by gate).
+Plan choose_plan(const lqcs_exec_options* opts, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t batch,
+                 uint32_t& L);
+// LQCS_COST_FEATURES values describing one planned step (planner.hip).
+void step_features(const PlanStep& st, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t batch,
+                   double* f);
+void last_plan_choice(int* plan, int* low, double* predicted_ms);
+bool rtile_enabled();
+// Installed cost model (mlp.hip).
+bool cost_model_installed();
+std::vector<double> cost_model_predict_ms(const std::vector<double>& raw_features, uint64_t rows);
 void set_last_stats(uint64_t passes, uint64_t launches);
 // Runs f under the engine lock, mapping exceptions to an lqcs_status and the
 // thread's last-error text.

@@ -9,6 +9,8 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <cmath>
+#include <cstring>
 
 namespace lqcs_dev {
 namespace {
@@ -96,17 +98,148 @@ Plan plan_gain(const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint3
     return plan;
 }
 
+// ---- learned plan choice ---------------------------------------------------------
+
+namespace {
+
+struct Choice {
+    int plan = LQCS_PLAN_GREEDY;
+    int low = 0;
+    double predicted_ms = -1.0;
+};
+thread_local Choice g_last_choice;
+
+uint32_t lowest_bits_qubits_at_least(uint64_t smask, uint32_t tmask, uint32_t lanes, uint32_t qmin) {
+    // Of the `lanes` lowest tile positions in tmask, how many map to qubits >= qmin.
+    uint32_t count = 0, seen = 0;
+    for (uint32_t p = 0; p < 12 && seen < lanes; ++p) {
+        if (!(tmask & (1u << p))) continue;
+        ++seen;
+        uint64_t m = smask;
+        for (uint32_t k = 0; k < p; ++k) m &= m - 1;
+        if (static_cast<uint32_t>(__builtin_ctzll(m)) >= qmin) ++count;
+    }
+    return count;
+}
+
+}  // namespace
+
+void step_features(const PlanStep& st, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t batch,
+                   double* f) {
+    std::fill(f, f + LQCS_COST_FEATURES, 0.0);
+    f[1] = nq;
+    f[2] = std::log2(double(std::max<uint32_t>(batch, 1)));
+    if (st.tiled) {
+        f[0] = 1.0;
+        f[15] = __builtin_popcountll(st.smask >> 20);
+        f[16] = __builtin_ctzll(~st.smask);
+        f[17] = double(st.tile_ops.size());
+        RTileProgram rp;
+        if (rtile_enabled() && build_rtile(st.tile_ops, L, st.smask, nq, rp)) {
+            f[3] = double(rp.phases.size());
+            for (const RPhase& ph : rp.phases) {
+                f[4] += ph.transpose;
+                f[5] += ph.seg_words ? 1.0 : 0.0;
+            }
+            for (size_t i = 0; i + 1 < rp.words.size();) {
+                RegOpHdr h;
+                std::memcpy(&h, &rp.words[i], sizeof h);
+                const uint32_t kind = h.code & 15u, rctl = (h.code >> 12) & 15u, nm = h.code >> 16;
+                const bool free = rctl == 0 && h.tctl == 0;
+                switch (kind) {
+                    case RK_DENSE1: f[free ? 6 : 7] += 1.0; break;
+                    case RK_X: f[free ? 8 : 9] += 1.0; break;
+                    case RK_DENSE2: f[10] += 1.0; break;
+                    case RK_DIAG: f[11] += 1.0; break;
+                    default: f[12] += 1.0; break;
+                }
+                if (h.gctrl) f[13] += 1.0;
+                i += 2 + nm;
+            }
+            f[14] = lowest_bits_qubits_at_least(st.smask, rp.phases.front().tmask, 6, 8);
+        }
+    } else {
+        const DecodedOp& d = ops[st.op];
+        f[18] = d.kind;
+        f[19] = double(d.targets.size());
+        f[20] = double(d.controls.size());
+        f[21] = d.diagonal ? 1.0 : 0.0;
+        uint64_t tm = 0;
+        for (uint32_t t : d.targets) tm |= uint64_t{1} << t;
+        if (d.kind == LQCS_OP_PAULI_ROTATION) tm = d.x | d.zy;
+        if (tm) {
+            f[22] = __builtin_ctzll(tm);
+            f[23] = 63 - __builtin_clzll(tm);
+        }
+    }
+}
+
+Plan choose_plan(const lqcs_exec_options* opts, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t batch,
+                 uint32_t& L) {
+    int kind = opts ? opts->plan : LQCS_PLAN_AUTO;
+    if (kind == LQCS_PLAN_AUTO) kind = LQCS_PLAN_GREEDY;
+    if (kind < LQCS_PLAN_GATE_BY_GATE || kind > LQCS_PLAN_GAIN)
+        throw Error(LQCS_ERR_INVALID_ARGUMENT, "unknown plan kind");
+    const TileChoice def = default_tile(nq);
+    L = opts && opts->tile_qubits > 0 ? static_cast<uint32_t>(opts->tile_qubits) : def.L;
+    const uint32_t low_req = opts && opts->low_qubits > 0 ? static_cast<uint32_t>(opts->low_qubits) : 0;
+    const uint32_t low = low_req ? low_req : def.low;
+    if (kind != LQCS_PLAN_LEARNED || L == 0 || !cost_model_installed()) {
+        // Without a cost model the learned plan falls back to greedy grouping.
+        const int k = kind == LQCS_PLAN_LEARNED ? LQCS_PLAN_GREEDY : kind;
+        g_last_choice = Choice{k, static_cast<int>(low), -1.0};
+        return make_plan(k, ops, nq, L, low);
+    }
+    ensure_ready();
+    std::vector<Plan> cands;
+    std::vector<Choice> meta;
+    for (int k : {LQCS_PLAN_GREEDY, LQCS_PLAN_GAIN}) {
+        for (uint32_t lw : {2u, 3u, 4u}) {
+            if (low_req && lw != low_req) continue;
+            if (lw + 2 > L) continue;
+            cands.push_back(make_plan(k, ops, nq, L, lw));
+            meta.push_back(Choice{k, static_cast<int>(lw), 0.0});
+        }
+    }
+    if (cands.empty()) {
+        g_last_choice = Choice{LQCS_PLAN_GREEDY, static_cast<int>(low), -1.0};
+        return make_plan(LQCS_PLAN_GREEDY, ops, nq, L, low);
+    }
+    const uint32_t F = LQCS_COST_FEATURES;
+    std::vector<double> rows;
+    std::vector<size_t> first(cands.size() + 1, 0);
+    for (size_t c = 0; c < cands.size(); ++c) {
+        first[c] = rows.size() / F;
+        for (const PlanStep& st : cands[c].steps) {
+            rows.resize(rows.size() + F);
+            step_features(st, ops, nq, L, batch, rows.data() + rows.size() - F);
+        }
+    }
+    first[cands.size()] = rows.size() / F;
+    const std::vector<double> ms = cost_model_predict_ms(rows, rows.size() / F);
+    size_t best = 0;
+    for (size_t c = 0; c < cands.size(); ++c) {
+        double t = 0.0;
+        for (size_t r = first[c]; r < first[c + 1]; ++r) t += ms[r];
+        meta[c].predicted_ms = t;
+        if (t < meta[best].predicted_ms || c == 0) best = c;
+    }
+    g_last_choice = meta[best];
+    return std::move(cands[best]);
+}
+
 void apply_learned(amp_t* s, uint32_t nq, const std::vector<DecodedOp>& ops, uint32_t L, uint32_t low,
                    uint32_t batch) {
-    const TileChoice def = default_tile(nq);
-    if (L == 0) L = def.L;
-    if (low == 0) low = def.low;
-    if (L == 0) {
-        apply_ops(s, nq, ops, LQCS_PLAN_GATE_BY_GATE, 0, 0, batch);
-        return;
-    }
-    const Plan plan = plan_gain(ops, nq, L, low);
-    execute_plan(s, nq, L, plan, ops, batch);
+    lqcs_exec_options o{LQCS_PLAN_LEARNED, static_cast<int32_t>(L), static_cast<int32_t>(low), 0};
+    uint32_t width = 0;
+    const Plan plan = choose_plan(&o, ops, nq, batch, width);
+    execute_plan(s, nq, width, plan, ops, batch);
+}
+
+void last_plan_choice(int* plan, int* low, double* predicted_ms) {
+    if (plan) *plan = g_last_choice.plan;
+    if (low) *low = g_last_choice.low;
+    if (predicted_ms) *predicted_ms = g_last_choice.predicted_ms;
 }
 
 }  // namespace lqcs_dev

@@ -51,6 +51,40 @@ __global__ void __launch_bounds__(kBlock) k_norm_partial(const amp_t* __restrict
     if (threadIdx.x == 0) part[uint64_t(blockIdx.y) * gridDim.x + blockIdx.x] = acc[0];
 }
 
+// ---- max |a_i - b_i| over two registers (NaN propagates) ----------------------------
+__device__ __forceinline__ double nan_max(double m, double d) { return (d > m || d != d) ? d : m; }
+
+__device__ __forceinline__ double block_max(double m) {
+    __shared__ double sh[kBlock];
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) sh[threadIdx.x] = nan_max(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+__global__ void __launch_bounds__(kBlock) k_maxdiff_partial(const amp_t* __restrict__ a, const amp_t* __restrict__ b,
+                                                            uint64_t n, double* __restrict__ part) {
+    double m = 0.0;
+    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock) {
+        const amp_t x = a[i], y = b[i];
+        const double dx = x.x - y.x, dy = x.y - y.y;
+        m = nan_max(m, sqrt(dx * dx + dy * dy));
+    }
+    m = block_max(m);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+__global__ void __launch_bounds__(kBlock) k_max_finish(const double* __restrict__ part, uint32_t count,
+                                                       double* __restrict__ out) {
+    double m = 0.0;
+    for (uint32_t j = threadIdx.x; j < count; j += kBlock) m = nan_max(m, part[j]);
+    m = block_max(m);
+    if (threadIdx.x == 0) out[0] = m;
+}
+
 // Sums `count` partials of each of gridDim.x rows (V values interleaved).
 template <int V>
 __global__ void __launch_bounds__(kBlock) k_finish(const double* __restrict__ part,
@@ -291,6 +325,13 @@ void reduce_norm_sq(const amp_t* s, uint32_t nq, uint32_t batch, double* d_out) 
     double* part = static_cast<double*>(arena(sizeof(double) * g * batch));
     k_norm_partial<<<dim3(g, batch), kBlock, 0, stream()>>>(s, nq, part);
     k_finish<1><<<batch, kBlock, 0, stream()>>>(part, g, d_out);
+}
+
+void reduce_max_abs_diff(const amp_t* a, const amp_t* b, uint32_t nq, double* d_out) {
+    const unsigned g = red_blocks(nq);
+    double* part = static_cast<double*>(arena(sizeof(double) * g));
+    k_maxdiff_partial<<<g, kBlock, 0, stream()>>>(a, b, uint64_t{1} << nq, part);
+    k_max_finish<<<1, kBlock, 0, stream()>>>(part, g, d_out);
 }
 
 void reduce_pauli_terms(const amp_t* s, uint32_t nq, const uint64_t* d_x, const uint64_t* d_zy,

@@ -7,6 +7,8 @@
 namespace lqcs_dev {
 
 void reduce_norm_sq(const amp_t* s, uint32_t nq, uint32_t batch, double* d_out);
+// max_i |a_i - b_i| of two registers of width nq (NaN if any difference is NaN).
+void reduce_max_abs_diff(const amp_t* a, const amp_t* b, uint32_t nq, double* d_out);
 // d_out[2*t .. 2*t+1] = (re, im) of sum_i sign_t(i) a_i conj(a_{i ^ x_t}).
 void reduce_pauli_terms(const amp_t* s, uint32_t nq, const uint64_t* d_x, const uint64_t* d_zy,
                         uint32_t n_terms, double* d_out);

@@ -3,6 +3,7 @@
 // follows src/backend/statevector_simulator.cpp and statevector.cpp of the
 // reference; every amplitude operation runs on the GPU through lqcs_device.h.
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <optional>
 #include <stdexcept>
@@ -113,6 +114,12 @@ double Statevector::norm() const {
     double s = 0.0;
     check(lqcs_state_norm_sq(device(), &s));
     return std::sqrt(s);
+}
+
+double Statevector::max_abs_diff(const Statevector& other) const {
+    double d = 0.0;
+    check(lqcs_state_max_abs_diff(device(), other.device(), &d));
+    return d;
 }
 
 std::vector<double> Statevector::probabilities() const {
@@ -312,7 +319,78 @@ std::vector<CompiledCircuit::PassInfo> CompiledCircuit::describe(const EngineOpt
     return out;
 }
 
+std::vector<double> CompiledCircuit::features(const EngineOptions& engine, std::uint32_t batch) const {
+    const lqcs_program prog = prog_->program();
+    const lqcs_exec_options o = exec_options(engine);
+    uint64_t n = 0;
+    check(lqcs_plan_features(static_cast<uint32_t>(nq_), &prog, &o, batch, nullptr, 0, &n));
+    std::vector<double> f(n * kCostFeatures);
+    check(lqcs_plan_features(static_cast<uint32_t>(nq_), &prog, &o, batch, f.data(), n, &n));
+    return f;
+}
+
 void synchronize() { check(lqcs_dev_synchronize()); }
+
+LaunchTimes profile_read_launches() {
+    uint64_t n = 0;
+    LaunchTimes t;
+    // one read drains the recording: size generously, then trim
+    t.ms.resize(1 << 20);
+    t.tiled.resize(1 << 20);
+    check(lqcs_profile_read_launches(t.ms.data(), t.tiled.data(), t.ms.size(), &n));
+    if (n > t.ms.size()) throw std::length_error("profile_read_launches: more than 2^20 launches recorded");
+    t.ms.resize(n);
+    t.tiled.resize(n);
+    return t;
+}
+
+std::uint64_t mlp_num_params(std::uint32_t n_features) {
+    uint64_t n = 0;
+    check(lqcs_mlp_num_params(n_features, &n));
+    return n;
+}
+
+void install_cost_model(const CostModel& m) {
+    if (m.params.empty()) {
+        check(lqcs_costmodel_install(kCostFeatures, nullptr, nullptr, nullptr, 0.0, 1.0));
+        return;
+    }
+    if (m.params.size() != mlp_num_params(kCostFeatures) || m.x_mean.size() != kCostFeatures ||
+        m.x_scale.size() != kCostFeatures)
+        throw std::invalid_argument("install_cost_model: sizes do not match the feature count");
+    check(lqcs_costmodel_install(kCostFeatures, m.params.data(), m.x_mean.data(), m.x_scale.data(), m.t_mean,
+                                 m.t_scale));
+}
+
+CostModel load_cost_model(const std::string& path) {
+    std::ifstream in(path);
+    if (!in) throw std::invalid_argument("load_cost_model: cannot open " + path);
+    std::string tag;
+    int version = 0;
+    std::uint32_t F = 0;
+    std::uint64_t np = 0;
+    in >> tag >> version >> F >> np;
+    if (!in || tag != "lqcs-costmodel" || version != 1 || F != kCostFeatures || np != mlp_num_params(F))
+        throw std::invalid_argument("load_cost_model: not a version-1 model with " + std::to_string(kCostFeatures) +
+                                    " features: " + path);
+    CostModel m;
+    m.x_mean.resize(F);
+    m.x_scale.resize(F);
+    m.params.resize(np);
+    for (double& v : m.x_mean) in >> v;
+    for (double& v : m.x_scale) in >> v;
+    in >> m.t_mean >> m.t_scale;
+    for (double& v : m.params) in >> v;
+    if (!in) throw std::invalid_argument("load_cost_model: truncated file " + path);
+    return m;
+}
+
+PlanChoice last_plan_choice() {
+    int32_t plan = 0, low = 0;
+    double ms = 0.0;
+    check(lqcs_last_plan_choice(&plan, &low, &ms));
+    return {static_cast<EngineOptions::Plan>(plan), low, ms};
+}
 
 void profile_enable(bool on) { check(lqcs_profile_enable(on ? 1 : 0)); }
 

@@ -8,6 +8,7 @@
 #include <pybind11/stl.h>
 
 #include <algorithm>
+#include <cstring>
 #include <optional>
 #include <vector>
 
@@ -38,7 +39,8 @@ EngineOptions::Plan plan_of(const std::string& s) {
     if (s == "greedy") return EngineOptions::Plan::Greedy;
     if (s == "learned") return EngineOptions::Plan::Learned;
     if (s == "auto") return EngineOptions::Plan::Auto;
-    throw std::invalid_argument("plan must be gate_by_gate, greedy, learned or auto");
+    if (s == "gain") return EngineOptions::Plan::Gain;
+    throw std::invalid_argument("plan must be gate_by_gate, greedy, gain, learned or auto");
 }
 
 }  // namespace
@@ -157,6 +159,7 @@ PYBIND11_MODULE(_pylqcs, m) {
         .def(py::init<std::size_t>(), "num_qubits"_a)
         .def("num_qubits", &Statevector::num_qubits)
         .def("norm", &Statevector::norm)
+        .def("max_abs_diff", &Statevector::max_abs_diff, "other"_a, "max |a_i - b_i|, on the device")
         .def("__len__", &Statevector::size)
         .def("__getitem__",
              [](const Statevector& sv, std::size_t i) {
@@ -363,7 +366,26 @@ PYBIND11_MODULE(_pylqcs, m) {
                      out.append(py::dict("ops"_a = p.ops, "tile_mask"_a = p.tile_mask, "phases"_a = p.phases));
                  return out;
              },
-             "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0);
+             "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0)
+        .def("features",
+             [](const CompiledCircuit& c, const std::string& plan, int tile_qubits, int low_qubits,
+                std::uint32_t batch) {
+                 EngineOptions e;
+                 e.plan = plan_of(plan);
+                 e.tile_qubits = tile_qubits;
+                 e.low_qubits = low_qubits;
+                 std::vector<double> f;
+                 {
+                     py::gil_scoped_release nogil;
+                     f = c.features(e, batch);
+                 }
+                 const py::ssize_t rows = static_cast<py::ssize_t>(f.size() / kCostFeatures);
+                 py::array_t<double> out({rows, static_cast<py::ssize_t>(kCostFeatures)});
+                 std::memcpy(out.mutable_data(), f.data(), f.size() * sizeof(double));
+                 return out;
+             },
+             "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0, "batch"_a = 1,
+             "cost-model features of every pass of the plan (rows x COST_FEATURES)");
     m.def("statevector", [](std::size_t n) { return Statevector(n, LQCS_DEVICE_MAX_QUBITS); }, "num_qubits"_a,
           "register of up to MAX_DEVICE_QUBITS qubits (beyond the 30-qubit reference cap)");
     m.def("synchronize", [] {
@@ -380,6 +402,95 @@ PYBIND11_MODULE(_pylqcs, m) {
         const LaunchProfile p = profile_read();
         return py::dict("tile_launches"_a = p.tile_launches, "tile_ms"_a = p.tile_ms,
                         "other_launches"_a = p.other_launches, "other_ms"_a = p.other_ms);
+    });
+
+    m.def("profile_read_launches", [] {
+        const LaunchTimes t = profile_read_launches();
+        py::array_t<double> ms(static_cast<py::ssize_t>(t.ms.size()));
+        py::array_t<std::uint8_t> tiled(static_cast<py::ssize_t>(t.tiled.size()));
+        std::memcpy(ms.mutable_data(), t.ms.data(), t.ms.size() * sizeof(double));
+        std::memcpy(tiled.mutable_data(), t.tiled.data(), t.tiled.size());
+        return py::make_tuple(ms, tiled);
+    }, "per-launch milliseconds and tiled flags since the last read");
+
+    // ---- learned planning cost model ----
+    m.attr("COST_FEATURES") = kCostFeatures;
+    m.def("mlp_num_params", &mlp_num_params, "n_features"_a);
+    m.def(
+        "mlp_forward",
+        [](py::array_t<double, py::array::c_style | py::array::forcecast> params,
+           py::array_t<double, py::array::c_style | py::array::forcecast> x) {
+            if (x.ndim() != 2) throw std::invalid_argument("x must be 2-D");
+            const auto rows = static_cast<std::uint64_t>(x.shape(0));
+            const auto F = static_cast<std::uint32_t>(x.shape(1));
+            if (static_cast<std::uint64_t>(params.size()) != mlp_num_params(F))
+                throw std::invalid_argument("params size does not match the feature count");
+            py::array_t<double> y(static_cast<py::ssize_t>(rows));
+            {
+                py::gil_scoped_release nogil;
+                if (lqcs_mlp_forward(F, params.data(), x.data(), rows, y.mutable_data()) != LQCS_OK)
+                    throw std::runtime_error(lqcs_dev_last_error());
+            }
+            return y;
+        },
+        "params"_a, "x"_a, "network output per row, evaluated on the device");
+    m.def(
+        "mlp_loss_grad",
+        [](py::array_t<double, py::array::c_style | py::array::forcecast> params,
+           py::array_t<double, py::array::c_style | py::array::forcecast> x,
+           py::array_t<double, py::array::c_style | py::array::forcecast> t) {
+            if (x.ndim() != 2 || t.size() != x.shape(0)) throw std::invalid_argument("x must be rows x F, t rows");
+            const auto rows = static_cast<std::uint64_t>(x.shape(0));
+            const auto F = static_cast<std::uint32_t>(x.shape(1));
+            if (static_cast<std::uint64_t>(params.size()) != mlp_num_params(F))
+                throw std::invalid_argument("params size does not match the feature count");
+            py::array_t<double> g(params.size());
+            double loss = 0.0;
+            {
+                py::gil_scoped_release nogil;
+                if (lqcs_mlp_loss_grad(F, params.data(), x.data(), t.data(), rows, &loss, g.mutable_data()) != LQCS_OK)
+                    throw std::runtime_error(lqcs_dev_last_error());
+            }
+            return py::make_tuple(loss, g);
+        },
+        "params"_a, "x"_a, "t"_a, "(mean squared error, gradient), on the device");
+    m.def(
+        "mlp_train",
+        [](py::array_t<double, py::array::c_style | py::array::forcecast> params,
+           py::array_t<double, py::array::c_style | py::array::forcecast> x,
+           py::array_t<double, py::array::c_style | py::array::forcecast> t, std::uint32_t epochs,
+           std::uint32_t batch_rows, double lr, std::uint64_t seed) {
+            if (x.ndim() != 2 || t.size() != x.shape(0)) throw std::invalid_argument("x must be rows x F, t rows");
+            const auto rows = static_cast<std::uint64_t>(x.shape(0));
+            const auto F = static_cast<std::uint32_t>(x.shape(1));
+            if (static_cast<std::uint64_t>(params.size()) != mlp_num_params(F))
+                throw std::invalid_argument("params size does not match the feature count");
+            py::array_t<double> p(params.size());
+            std::memcpy(p.mutable_data(), params.data(), params.size() * sizeof(double));
+            double loss = 0.0;
+            {
+                py::gil_scoped_release nogil;
+                if (lqcs_mlp_train(F, p.mutable_data(), x.data(), t.data(), rows, epochs, batch_rows, lr, seed,
+                                   &loss) != LQCS_OK)
+                    throw std::runtime_error(lqcs_dev_last_error());
+            }
+            return py::make_tuple(p, loss);
+        },
+        "params"_a, "x"_a, "t"_a, "epochs"_a, "batch_rows"_a = 256, "lr"_a = 1e-3, "seed"_a = 0,
+        "Adam on the device; returns (trained params, final loss)");
+    m.def(
+        "costmodel_install",
+        [](std::vector<double> params, std::vector<double> x_mean, std::vector<double> x_scale, double t_mean,
+           double t_scale) { install_cost_model(CostModel{params, x_mean, x_scale, t_mean, t_scale}); },
+        "params"_a, "x_mean"_a, "x_scale"_a, "t_mean"_a, "t_scale"_a);
+    m.def("costmodel_uninstall", [] { install_cost_model(CostModel{}); });
+    m.def("costmodel_load", [](const std::string& path) { install_cost_model(load_cost_model(path)); }, "path"_a);
+    m.def("last_plan_choice", [] {
+        const PlanChoice c = last_plan_choice();
+        static const char* names[] = {"gate_by_gate", "greedy", "learned", "auto", "gain"};
+        const int k = static_cast<int>(c.plan);
+        return py::dict("plan"_a = std::string(k >= 0 && k <= 4 ? names[k] : "?"), "low_qubits"_a = c.low_qubits,
+                        "predicted_ms"_a = c.predicted_ms);
     });
 
     m.def("device_info", [] {

@@ -25,14 +25,30 @@ synchronize = _native.synchronize
 profile_enable = _native.profile_enable
 profile_read = _native.profile_read
 set_tile_kernel = _native.set_tile_kernel
+profile_read_launches = _native.profile_read_launches
+COST_FEATURES = _native.COST_FEATURES
+mlp_num_params = _native.mlp_num_params
+mlp_forward = _native.mlp_forward
+mlp_loss_grad = _native.mlp_loss_grad
+mlp_train = _native.mlp_train
+costmodel_install = _native.costmodel_install
+costmodel_uninstall = _native.costmodel_uninstall
+costmodel_load = _native.costmodel_load
+last_plan_choice = _native.last_plan_choice
 
 __all__ = [
     "GateType", "Gate", "QuantumCircuit", "Statevector", "Result", "KrausChannel",
     "NoiseModel", "StatevectorSimulator", "ParametricCircuit", "algorithms",
     "device_info", "last_plan_stats", "MAX_DEVICE_QUBITS", "CompiledCircuit",
     "statevector", "synchronize", "profile_enable", "profile_read", "set_tile_kernel", "ExpectationFunction",
-    "expectation",
+    "expectation", "profile_read_launches", "COST_FEATURES", "mlp_num_params", "mlp_forward", "mlp_loss_grad",
+    "mlp_train", "costmodel_install", "costmodel_uninstall", "costmodel_load", "last_plan_choice",
 ]
+
+# The packaged cost model (if trained) steers plan="learned".
+from . import costmodel as _costmodel  # noqa: E402
+
+_costmodel.install_default(_native)
 
 
 def _torch():
