@@ -26,20 +26,20 @@ hipStream_t stream();
 int num_cus();
 
 // ---- complex arithmetic on double2 ------------------------------------------
-__device__ __forceinline__ amp_t cmul(amp_t a, amp_t b) {
+__host__ __device__ __forceinline__ amp_t cmul(amp_t a, amp_t b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ __forceinline__ amp_t cadd(amp_t a, amp_t b) {
+__host__ __device__ __forceinline__ amp_t cadd(amp_t a, amp_t b) {
     return make_double2(a.x + b.x, a.y + b.y);
 }
-__device__ __forceinline__ amp_t cscale(amp_t a, double s) { return make_double2(a.x * s, a.y * s); }
+__host__ __device__ __forceinline__ amp_t cscale(amp_t a, double s) { return make_double2(a.x * s, a.y * s); }
 __device__ __forceinline__ amp_t cconj(amp_t a) { return make_double2(a.x, -a.y); }
 // m0*a0 + m1*a1
 __device__ __forceinline__ amp_t cmac2(amp_t m0, amp_t a0, amp_t m1, amp_t a1) {
     return make_double2(m0.x * a0.x - m0.y * a0.y + m1.x * a1.x - m1.y * a1.y,
                         m0.x * a0.y + m0.y * a0.x + m1.x * a1.y + m1.y * a1.x);
 }
-__device__ __forceinline__ double cnorm(amp_t a) { return a.x * a.x + a.y * a.y; }
+__host__ __device__ __forceinline__ double cnorm(amp_t a) { return a.x * a.x + a.y * a.y; }
 
 __host__ __device__ __forceinline__ uint64_t insert_zero_bit(uint64_t v, unsigned pos) {
     const uint64_t low = (uint64_t{1} << pos) - 1;

@@ -1,17 +1,37 @@
 // Batched noisy trajectories (the per-shot path of run_per_shot,
 // src/backend/statevector_simulator.cpp:122-190).
 //
-// B shots evolve side by side as B registers in one allocation; every gate is
-// one launch over the whole batch (blockIdx.y = shot). A Kraus channel on a
-// qubit costs one read pass for the qubit's 2x2 statistics (S00, S11, S01),
-// from which every branch probability ||K psi||^2 follows, and one pass that
-// picks the branch with the shot's own uniform draw and applies K/sqrt(p).
-// Draw order is the reference's: per unitary instruction, channel-major then
-// qubit order; one draw per Measure/Reset; shot s owns draws [s*D, (s+1)*D).
+// B shots evolve side by side as B registers in one allocation (blockIdx.y =
+// shot). Draw order is the reference's: per unitary instruction, channel-major
+// then qubit order; one draw per Measure/Reset; shot s owns draws
+// [s*D, (s+1)*D).
+//
+// The reference picks a Kraus branch per (gate, channel, qubit) with
+// p_i = ||K_i psi||^2 of the normalised state and applies K_i / sqrt(p_i).
+// Most of those choices are known before the state is:
+//   * a channel whose K_i^dag K_i are all multiples c_i of the identity
+//     (depolarizing, bit/phase flip) has p_i = c_i whatever psi is, so the
+//     host decides the branch from the draw alone;
+//   * for any other channel (amplitude damping), p_0 >= lambda_min(K_0^dag K_0),
+//     so a draw below that bound picks branch 0 whatever psi is.
+// Every decision compares ratios, so the registers are kept unnormalised
+// between reductions: the default branch is applied as K_0 without 1/sqrt(p).
+// A shot leaves this default path only at an "event": another branch of a
+// state-independent channel (applied as the unitary K_i / sqrt(c_i)), or a
+// draw the bound does not settle (one 2x2-statistics read of that shot's
+// register, then the reference's rule on the normalised probabilities).
+//
+// Between events the whole batch shares one op sequence (the circuit's gates
+// and the default branches), which runs as tiled passes over the batch: a
+// trajectory costs a few HBM passes per tile set instead of two per noise
+// site. Branch choices equal the reference's except for draws within
+// rounding of a cumulative boundary.
 #include "common.hpp"
 #include "engine.hpp"
 #include "reduce.hpp"
 
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -24,16 +44,19 @@ struct KrausSet {
     uint32_t n;
 };
 
-__global__ void __launch_bounds__(kBlock) k_kraus_apply(amp_t* __restrict__ s, uint32_t nq,
-                                                        uint32_t q, const double* __restrict__ stats,
-                                                        KrausSet ks,
-                                                        const double* __restrict__ draws,
-                                                        uint64_t draw_stride, uint64_t draw_offset) {
-    const uint32_t r = blockIdx.y;
-    const double* st = stats + 4 * r;
+// Event members at one noise site: decide among all branches from the 2x2
+// statistics of the member's (unnormalised) register, then apply K/sqrt(p),
+// which also renormalises it. Block row j handles member members[j].
+__global__ void __launch_bounds__(kBlock) k_kraus_members(amp_t* __restrict__ s, uint32_t nq, uint32_t q,
+                                                          const double* __restrict__ stats, KrausSet ks,
+                                                          const double* __restrict__ draws, uint64_t draw_stride,
+                                                          uint64_t draw_offset, const uint32_t* __restrict__ members) {
+    const uint32_t r = members[blockIdx.y];
+    const double* st = stats + 4 * blockIdx.y;
+    const double norm2 = st[0] + st[1];
     const double u = draws[uint64_t(r) * draw_stride + draw_offset];
-    // Branch choice as sample_kraus_branch: skip p <= 0, first with u < cum,
-    // falling back to the last non-zero branch.
+    // sample_kraus_branch on the normalised state: skip p <= 0, first with
+    // u < cum, falling back to the last non-zero branch.
     int chosen = -1;
     double chosen_p = 0.0, cum = 0.0;
     for (uint32_t i = 0; i < ks.n; ++i) {
@@ -42,7 +65,7 @@ __global__ void __launch_bounds__(kBlock) k_kraus_apply(amp_t* __restrict__ s, u
         const double c11 = cnorm(K[1]) + cnorm(K[3]);
         const double xr = K[0].x * K[1].x + K[0].y * K[1].y + K[2].x * K[3].x + K[2].y * K[3].y;
         const double xi = K[0].x * K[1].y - K[0].y * K[1].x + K[2].x * K[3].y - K[2].y * K[3].x;
-        const double p = c00 * st[0] + c11 * st[1] + 2.0 * (xr * st[2] - xi * st[3]);
+        const double p = (c00 * st[0] + c11 * st[1] + 2.0 * (xr * st[2] - xi * st[3])) / norm2;
         if (p <= 0.0) continue;
         chosen = static_cast<int>(i);
         chosen_p = p;
@@ -50,13 +73,12 @@ __global__ void __launch_bounds__(kBlock) k_kraus_apply(amp_t* __restrict__ s, u
         if (u < cum) break;
     }
     if (chosen < 0) return;
-    const double inv = 1.0 / sqrt(chosen_p);
+    const double inv = 1.0 / sqrt(chosen_p * norm2);
     const amp_t m0 = cscale(ks.k[chosen][0], inv), m1 = cscale(ks.k[chosen][1], inv);
     const amp_t m2 = cscale(ks.k[chosen][2], inv), m3 = cscale(ks.k[chosen][3], inv);
     amp_t* b = s + (uint64_t(r) << nq);
     const uint64_t half = uint64_t{1} << (nq - 1);
-    for (uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x; g < half;
-         g += uint64_t(gridDim.x) * kBlock) {
+    for (uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x; g < half; g += uint64_t(gridDim.x) * kBlock) {
         const uint64_t i0 = insert_zero_bit(g, q), i1 = i0 | (uint64_t{1} << q);
         const amp_t a0 = b[i0], a1 = b[i1];
         b[i0] = cmac2(m0, a0, m1, a1);
@@ -64,9 +86,154 @@ __global__ void __launch_bounds__(kBlock) k_kraus_apply(amp_t* __restrict__ s, u
     }
 }
 
+// A 2x2 on qubit q of every batch member except those flagged in skip.
+__global__ void __launch_bounds__(kBlock) k_single_skip(amp_t* __restrict__ s, uint32_t nq, uint32_t q, Mat2 m,
+                                                        const uint8_t* __restrict__ skip) {
+    if (skip[blockIdx.y]) return;
+    amp_t* b = s + (uint64_t(blockIdx.y) << nq);
+    const uint64_t half = uint64_t{1} << (nq - 1);
+    for (uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x; g < half; g += uint64_t(gridDim.x) * kBlock) {
+        const uint64_t i0 = insert_zero_bit(g, q), i1 = i0 | (uint64_t{1} << q);
+        const amp_t a0 = b[i0], a1 = b[i1];
+        b[i0] = cmac2(m.m[0], a0, m.m[1], a1);
+        b[i1] = cmac2(m.m[2], a0, m.m[3], a1);
+    }
+}
+
+// Measure / reset of qubit q on every member (collapse_qubit / reset_qubit,
+// gate_kernels.cpp:227-260) from unnormalised statistics: p1 = S11/(S00+S11),
+// outcome u < p1, the kept branch scaled by 1/sqrt(S_kept) (renormalised).
+__global__ void __launch_bounds__(kBlock) k_collapse_stats(amp_t* __restrict__ s, uint32_t nq, uint32_t q,
+                                                           const double* __restrict__ stats,
+                                                           const double* __restrict__ draws, uint64_t draw_stride,
+                                                           uint64_t draw_offset, int reset, uint8_t* __restrict__ creg,
+                                                           uint32_t creg_stride, int32_t clbit) {
+    const uint32_t r = blockIdx.y;
+    amp_t* b = s + (uint64_t(r) << nq);
+    const double s00 = stats[4 * r], s11 = stats[4 * r + 1];
+    const double p1 = s11 / (s00 + s11);
+    const double u = draws[uint64_t(r) * draw_stride + draw_offset];
+    const bool one = u < p1;
+    const double scale = 1.0 / sqrt(one ? s11 : s00);
+    const uint64_t half = uint64_t{1} << (nq - 1);
+    const uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (g == 0 && clbit >= 0) creg[uint64_t(r) * creg_stride + clbit] = one ? 1 : 0;
+    for (uint64_t k = g; k < half; k += uint64_t(gridDim.x) * kBlock) {
+        const uint64_t i0 = insert_zero_bit(k, q), i1 = i0 | (uint64_t{1} << q);
+        const amp_t a0 = b[i0], a1 = b[i1];
+        const amp_t z = make_double2(0.0, 0.0);
+        if (one) {
+            if (reset) {
+                b[i0] = cscale(a1, scale);
+                b[i1] = z;
+            } else {
+                b[i0] = z;
+                b[i1] = cscale(a1, scale);
+            }
+        } else {
+            b[i0] = cscale(a0, scale);
+            b[i1] = z;
+        }
+    }
+}
+
 unsigned pass_blocks(uint32_t nq) {
     const uint64_t want = ((uint64_t{1} << (nq - 1)) + kBlock * 4 - 1) / (kBlock * 4);
-    return static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(want, 1u << 20)));
+    return static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(want, 1u << 16)));
+}
+
+// How a channel's branch is chosen.
+struct ChannelPlan {
+    bool mixture = false;            // every K_i^dag K_i = c_i * I
+    std::vector<double> c;           // mixture: the c_i
+    int def = -1;                    // branch taken without an event
+    double bound = 0.0;              // general: draws below this pick `def`
+    bool def_identity = false;       // default branch is the identity (nothing to apply)
+    Mat2 def_op{};                   // default branch as applied (unitary for mixtures, K_0 otherwise)
+    std::vector<Mat2> unit;          // mixture: K_i / sqrt(c_i)
+};
+
+ChannelPlan plan_channel(const KrausSet& ks) {
+    ChannelPlan cp;
+    cp.mixture = true;
+    cp.c.resize(ks.n);
+    double scale = 0.0;
+    for (uint32_t i = 0; i < ks.n; ++i) {
+        const amp_t* K = ks.k[i];
+        const double g00 = cnorm(K[0]) + cnorm(K[2]), g11 = cnorm(K[1]) + cnorm(K[3]);
+        const double gr = K[0].x * K[1].x + K[0].y * K[1].y + K[2].x * K[3].x + K[2].y * K[3].y;
+        const double gi = K[0].x * K[1].y - K[0].y * K[1].x + K[2].x * K[3].y - K[2].y * K[3].x;
+        scale = std::max(scale, std::max(g00, g11));
+        if (std::abs(g00 - g11) > 1e-14 * std::max(1.0, g00) || std::hypot(gr, gi) > 1e-14 * std::max(1.0, g00))
+            cp.mixture = false;
+        cp.c[i] = 0.5 * (g00 + g11);
+    }
+    if (cp.mixture) {
+        for (uint32_t i = 0; i < ks.n; ++i) {
+            Mat2 u{};
+            const double inv = cp.c[i] > 0.0 ? 1.0 / std::sqrt(cp.c[i]) : 0.0;
+            for (int e = 0; e < 4; ++e) u.m[e] = cscale(ks.k[i][e], inv);
+            cp.unit.push_back(u);
+            if (cp.def < 0 && cp.c[i] > 0.0) cp.def = static_cast<int>(i);
+        }
+        if (cp.def >= 0) {
+            cp.def_op = cp.unit[cp.def];
+            const amp_t* u = cp.def_op.m;
+            cp.def_identity = std::abs(u[0].x - 1.0) < 1e-15 && std::abs(u[0].y) < 1e-15 &&
+                              std::abs(u[3].x - 1.0) < 1e-15 && std::abs(u[3].y) < 1e-15 && cnorm(u[1]) == 0.0 &&
+                              cnorm(u[2]) == 0.0;
+        }
+        return cp;
+    }
+    // General channel: branch 0 (the first non-zero operator) is certain for
+    // draws below lambda_min(K_0^dag K_0), with a margin for rounding.
+    for (uint32_t i = 0; i < ks.n && cp.def < 0; ++i) {
+        const amp_t* K = ks.k[i];
+        if (cnorm(K[0]) + cnorm(K[1]) + cnorm(K[2]) + cnorm(K[3]) > 0.0) cp.def = static_cast<int>(i);
+    }
+    if (cp.def < 0) return cp;
+    const amp_t* K = ks.k[cp.def];
+    const double a = cnorm(K[0]) + cnorm(K[2]), d = cnorm(K[1]) + cnorm(K[3]);
+    const double br = K[0].x * K[1].x + K[0].y * K[1].y + K[2].x * K[3].x + K[2].y * K[3].y;
+    const double bi = K[0].x * K[1].y - K[0].y * K[1].x + K[2].x * K[3].y - K[2].y * K[3].x;
+    const double lmin = 0.5 * (a + d) - std::sqrt(0.25 * (a - d) * (a - d) + br * br + bi * bi);
+    cp.bound = std::max(0.0, lmin * (1.0 - 1e-12) - 1e-15);
+    for (int e = 0; e < 4; ++e) cp.def_op.m[e] = K[e];
+    return cp;
+}
+
+// Branch of a mixture channel for draw u (host restatement of the kernel's rule).
+int mixture_branch(const ChannelPlan& cp, double u) {
+    int chosen = -1;
+    double cum = 0.0;
+    for (size_t i = 0; i < cp.c.size(); ++i) {
+        if (cp.c[i] <= 0.0) continue;
+        chosen = static_cast<int>(i);
+        cum += cp.c[i];
+        if (u < cum) break;
+    }
+    return chosen;
+}
+
+DecodedOp single_op(uint32_t q, const Mat2& m) {
+    DecodedOp d;
+    d.targets = {q};
+    d.touched = uint64_t{1} << q;
+    const bool diag = cnorm(m.m[1]) == 0.0 && cnorm(m.m[2]) == 0.0;
+    d.diagonal = diag;
+    if (diag) {
+        d.kind = LQCS_OP_DIAGONAL_SINGLE;
+        d.m = {m.m[0], m.m[3]};
+    } else {
+        d.kind = LQCS_OP_SINGLE;
+        d.m = {m.m[0], m.m[1], m.m[2], m.m[3]};
+    }
+    return d;
+}
+
+template <typename T>
+T* dev_array(size_t n) {
+    return static_cast<T*>(device_alloc(std::max<size_t>(n, 1) * sizeof(T)));
 }
 
 }  // namespace
@@ -91,6 +258,7 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
         std::vector<DecodedOp> ops;
         for (uint64_t k = 0; k < prog->n_ops; ++k) ops.push_back(decode(*prog, k, nq));
         std::vector<KrausSet> sets(n_channels);
+        std::vector<ChannelPlan> cplan(n_channels);
         for (uint32_t c = 0; c < n_channels; ++c) {
             if (channels[c].n_ops == 0 || channels[c].n_ops > kMaxKraus)
                 throw Error(LQCS_ERR_INVALID_ARGUMENT, "channel needs 1..8 Kraus operators");
@@ -99,16 +267,24 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
             for (uint32_t i = 0; i < channels[c].n_ops; ++i)
                 for (int e = 0; e < 4; ++e)
                     sets[c].k[i][e] = make_double2(channels[c].ops[8 * i + 2 * e], channels[c].ops[8 * i + 2 * e + 1]);
+            cplan[c] = plan_channel(sets[c]);
         }
         // Draw offsets per step and a consistency check against D.
         std::vector<uint64_t> off(n_steps);
-        uint64_t d = 0;
+        uint64_t d = 0, sites = 0;
+        double events_per_shot = 0.0;
         for (uint64_t i = 0; i < n_steps; ++i) {
             off[i] = d;
             const lqcs_traj_step& st = steps[i];
             if (st.kind == LQCS_TOP_UNITARY) {
                 if (st.op_index >= ops.size()) throw Error(LQCS_ERR_INVALID_ARGUMENT, "step op index out of range");
                 d += uint64_t(n_channels) * st.n_noise_qubits;
+                sites += uint64_t(n_channels) * st.n_noise_qubits;
+                for (uint32_t c = 0; c < n_channels; ++c) {
+                    const ChannelPlan& cp = cplan[c];
+                    const double p_event = cp.mixture ? 1.0 - (cp.def >= 0 ? cp.c[cp.def] : 0.0) : 1.0 - cp.bound;
+                    events_per_shot += st.n_noise_qubits * std::max(0.0, p_event);
+                }
                 for (uint32_t j = 0; j < st.n_noise_qubits; ++j)
                     if (noise_qubits[st.noise_qubit_offset + j] >= nq)
                         throw Error(LQCS_ERR_OUT_OF_RANGE, "noise qubit out of range");
@@ -123,7 +299,9 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
         }
         if (d != D) throw Error(LQCS_ERR_INVALID_ARGUMENT, "draws_per_shot does not match the steps");
 
-        // Batch size: at most max_batch, at most ~1/2 of free HBM.
+        // Batch size: at most max_batch and ~1/2 of free HBM; and small enough
+        // that the batch's events leave runs of ~25 noise sites between them
+        // (every event splits the shared op sequence for the whole batch).
         size_t free_b = 0, total_b = 0;
         LQCS_HIP(hipMemGetInfo(&free_b, &total_b));
         const uint64_t reg_bytes = (uint64_t{1} << nq) * sizeof(amp_t);
@@ -131,20 +309,40 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
         B = std::min<uint64_t>(B, max_batch ? max_batch : 65535);
         B = std::min<uint64_t>(B, n_shots);
         B = std::min<uint64_t>(B, 65535);
+        if (!max_batch && events_per_shot > 0.0 && nq >= 14) {
+            const double target = std::max(1.0, double(sites) / (25.0 * events_per_shot));
+            B = std::min<uint64_t>(B, static_cast<uint64_t>(target));
+        }
         amp_t* states = nullptr;
         LQCS_HIP(hipMalloc(&states, B * reg_bytes));
-        struct Free {
-            amp_t* p;
-            ~Free() { if (p) (void)hipFree(p); }
-        } guard{states};
         const uint32_t creg_w = std::max<uint32_t>(1, n_clbits);
-        uint8_t* d_creg = static_cast<uint8_t*>(arena(B * creg_w));
-        double* d_draws = static_cast<double*>(arena(B * std::max<uint64_t>(D, 1) * sizeof(double)));
-        double* d_stats = static_cast<double*>(arena(B * 4 * sizeof(double)));
+        uint8_t* d_creg = dev_array<uint8_t>(B * creg_w);
+        double* d_draws = dev_array<double>(B * std::max<uint64_t>(D, 1));
+        double* d_stats = dev_array<double>(B * 4);
+        uint32_t* d_members = dev_array<uint32_t>(B);
+        uint8_t* d_skip = dev_array<uint8_t>(B);
+        struct Free {
+            std::vector<void*> p;
+            ~Free() {
+                for (void* x : p) device_free(x);
+            }
+        } guard{{states, d_creg, d_draws, d_stats, d_members, d_skip}};
         std::vector<uint8_t> host_creg(B * creg_w);
+        const TileChoice tc = default_tile(nq);
+
+        std::vector<DecodedOp> pending;
+        uint32_t nb = 0;
+        auto flush = [&] {
+            if (pending.empty()) return;
+            const Plan plan = make_plan(tc.L ? LQCS_PLAN_GREEDY : LQCS_PLAN_GATE_BY_GATE, pending, nq, tc.L, tc.low);
+            execute_plan(states, nq, tc.L, plan, pending, nb);
+            pending.clear();
+        };
 
         for (uint64_t s0 = 0; s0 < n_shots; s0 += B) {
-            const uint32_t nb = static_cast<uint32_t>(std::min<uint64_t>(B, n_shots - s0));
+            nb = static_cast<uint32_t>(std::min<uint64_t>(B, n_shots - s0));
+            sync();
+            arena_reset();
             launch_fill_basis(states, nq, 0, nb);
             LQCS_HIP(hipMemsetAsync(d_creg, 0, uint64_t(nb) * creg_w, stream()));
             if (D)
@@ -152,23 +350,76 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
                                         hipMemcpyHostToDevice, stream()));
             for (uint64_t i = 0; i < n_steps; ++i) {
                 const lqcs_traj_step& st = steps[i];
-                if (st.kind == LQCS_TOP_UNITARY) {
-                    run_single_op(states, nq, ops[st.op_index], nb);
-                    uint64_t dd = off[i];
-                    for (uint32_t c = 0; c < n_channels; ++c)
-                        for (uint32_t j = 0; j < st.n_noise_qubits; ++j, ++dd) {
-                            const uint32_t q = noise_qubits[st.noise_qubit_offset + j];
-                            reduce_qstats(states, nq, q, nb, d_stats);
-                            k_kraus_apply<<<dim3(pass_blocks(nq), nb), kBlock, 0, stream()>>>(
-                                states, nq, q, d_stats, sets[c], d_draws, D, dd);
-                        }
-                } else {
-                    reduce_p1(states, nq, st.qubit, nb, d_stats);
+                if (st.kind != LQCS_TOP_UNITARY) {
+                    flush();
+                    reduce_qstats(states, nq, st.qubit, nb, d_stats);
                     const int reset = st.kind == LQCS_TOP_RESET;
-                    launch_collapse(states, nq, st.qubit, d_stats, d_draws, D, off[i], reset, d_creg, creg_w,
-                                    reset ? -1 : static_cast<int32_t>(st.clbit), nb);
+                    k_collapse_stats<<<dim3(pass_blocks(nq), nb), kBlock, 0, stream()>>>(
+                        states, nq, st.qubit, d_stats, d_draws, D, off[i], reset, d_creg, creg_w,
+                        reset ? -1 : static_cast<int32_t>(st.clbit));
+                    LQCS_HIP(hipGetLastError());
+                    continue;
+                }
+                pending.push_back(ops[st.op_index]);
+                uint64_t dd = off[i];
+                for (uint32_t c = 0; c < n_channels; ++c) {
+                    const ChannelPlan& cp = cplan[c];
+                    for (uint32_t j = 0; j < st.n_noise_qubits; ++j, ++dd) {
+                        const uint32_t q = noise_qubits[st.noise_qubit_offset + j];
+                        if (cp.def < 0) continue;  // all operators zero: nothing happens
+                        // Events of this site: (member, branch) for mixtures, or
+                        // (member, -1) when the bound does not settle the draw.
+                        std::vector<std::pair<uint32_t, int>> ev;
+                        for (uint32_t b = 0; b < nb; ++b) {
+                            const double u = draws[(s0 + b) * D + dd];
+                            if (cp.mixture) {
+                                const int k = mixture_branch(cp, u);
+                                if (k != cp.def) ev.emplace_back(b, k);
+                            } else if (!(u < cp.bound)) {
+                                ev.emplace_back(b, -1);
+                            }
+                        }
+                        if (ev.empty()) {
+                            if (!cp.def_identity) pending.push_back(single_op(q, cp.def_op));
+                            continue;
+                        }
+                        flush();
+                        // Undecided members: statistics of their own registers first.
+                        std::vector<uint32_t> undecided;
+                        for (const auto& e : ev)
+                            if (e.second < 0) undecided.push_back(e.first);
+                        for (size_t k = 0; k < undecided.size(); ++k)
+                            reduce_qstats(states + (uint64_t(undecided[k]) << nq), nq, q, 1, d_stats + 4 * k);
+                        // Default branch on everyone else.
+                        if (!cp.def_identity) {
+                            std::vector<uint8_t> skip(nb, 0);
+                            for (const auto& e : ev) skip[e.first] = 1;
+                            LQCS_HIP(hipMemcpyAsync(d_skip, skip.data(), nb, hipMemcpyHostToDevice, stream()));
+                            k_single_skip<<<dim3(pass_blocks(nq), nb), kBlock, 0, stream()>>>(states, nq, q,
+                                                                                             cp.def_op, d_skip);
+                            LQCS_HIP(hipGetLastError());
+                        }
+                        // Other branches of mixtures: their unitary.
+                        for (const auto& e : ev) {
+                            if (e.second < 0) continue;
+                            const DecodedOp op = single_op(q, cp.unit[e.second]);
+                            run_single_op(states + (uint64_t(e.first) << nq), nq, op, 1);
+                        }
+                        // Undecided: the reference's rule on their statistics.
+                        if (!undecided.empty()) {
+                            LQCS_HIP(hipMemcpyAsync(d_members, undecided.data(), undecided.size() * sizeof(uint32_t),
+                                                    hipMemcpyHostToDevice, stream()));
+                            k_kraus_members<<<dim3(pass_blocks(nq), static_cast<unsigned>(undecided.size())), kBlock,
+                                              0, stream()>>>(states, nq, q, d_stats, sets[c], d_draws, D, dd,
+                                                             d_members);
+                            LQCS_HIP(hipGetLastError());
+                        }
+                        // The host vectors above are read by queued copies.
+                        sync();
+                    }
                 }
             }
+            flush();
             LQCS_HIP(hipMemcpyAsync(host_creg.data(), d_creg, uint64_t(nb) * creg_w, hipMemcpyDeviceToHost,
                                     stream()));
             sync();
