@@ -12,7 +12,10 @@ kernel (tiled pass, HIP-event timed), and the reference's OpenMP path timed
 on this host on a bounded sample of the same workload. A second measurement
 covers the metric's other half: batched adjoint gradients of 256 Pauli-
 rotation ansatze (16-24 qubits, 400 rotations, 200-term Hamiltonians) against
-parameter-shift gradients on the same GPU (timed on a bounded sample).
+parameter-shift gradients on the same GPU (timed on a bounded sample). Two
+more objects report the noisy-trajectory workload (24 qubits, 2048 shots,
+trajectories/s) and the learned planner on the headline workload (ms per step
+of the cost model's plan next to greedy's).
 
 `traffic` (HBM bytes per tiled pass) comes from the rocprofv3 counter summary
 committed under profiles/ for the same kernel and workload, when one exists.
@@ -149,6 +152,31 @@ def adjoint_bench(lq, W, seed: int, n_circuits: int, ps_seconds: float) -> dict:
     }
 
 
+def trajectories_bench(lq, W, seed: int, n: int, shots: int) -> dict:
+    """The noisy-trajectory workload: GHZ prep + 10 random U3/CZ layers at n
+    qubits, depolarizing(1e-3) then amplitude_damping(1e-3) after every gate,
+    measure_all, `shots` trajectories in batched launches."""
+    qc = lq.QuantumCircuit(n, n)
+    W.noisy_ghz_layers(qc, n, 10, seed)
+    qc.measure_all()
+    noise = lq.NoiseModel()
+    noise.add_all_qubit_channel(lq.KrausChannel.depolarizing(1e-3))
+    noise.add_all_qubit_channel(lq.KrausChannel.amplitude_damping(1e-3))
+    sim = lq.StatevectorSimulator(seed=seed, noise=noise)
+    sim.run(qc, 8)  # warm-up
+    lq.synchronize()
+    t0 = time.perf_counter()
+    counts = sim.run(qc, shots).counts()
+    lq.synchronize()
+    dt = time.perf_counter() - t0
+    gates = qc.num_instructions() - n
+    log(f"trajectories: {shots} shots of {n} qubits in {dt:.3f} s")
+    return {"workload": (f"GHZ prep + 10 random U3/CZ layers, {n} qubits, {gates} gates, depolarizing(1e-3) + "
+                         f"amplitude_damping(1e-3) after every gate, measure_all"),
+            "shots": shots, "seconds": round(dt, 3), "trajectories_per_s": round(shots / dt, 2),
+            "trajectory_gates_per_s": round(shots * gates / dt, 1), "distinct_outcomes": len(dict(counts))}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +193,11 @@ def main() -> int:
     ap.add_argument("--adjoint-circuits", type=int, default=256)
     ap.add_argument("--ps-seconds", type=float, default=20.0)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--traj-qubits", type=int, default=24)
+    ap.add_argument("--traj-shots", type=int, default=2048)
+    ap.add_argument("--no-trajectories", action="store_true")
+    ap.add_argument("--no-planner", action="store_true",
+                    help="skip timing the learned plan on the same workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with one device-to-device copy of the register (counter calibration)")
     args = ap.parse_args()
@@ -239,10 +272,29 @@ def main() -> int:
                 "kernel": kernel, "bytes_per_launch": pass_bytes, "avg_launch_ms": round(avg_s * 1e3, 4),
                 "launches_per_step": prof["tile_launches"] // max(1, args.steps)}
 
+    planner = None
+    if not args.no_planner:
+        comp.apply(sv, plan="learned")  # warm-up (and the cost model's pick)
+        choice = lq.last_plan_choice()
+        lq.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(2):
+            sv.set_basis(0)
+            comp.apply(sv, plan="learned")
+        lq.synchronize()
+        learned_ms = (time.perf_counter() - t0) / 2 * 1e3
+        planner = {"greedy_ms_per_step": round(elapsed / args.steps * 1e3, 3),
+                   "learned_ms_per_step": round(learned_ms, 3), "learned_pick": choice,
+                   "learned_vs_greedy": round(learned_ms / (elapsed / args.steps * 1e3), 4)}
+        log(f"planner: learned {learned_ms:.1f} ms/step ({choice})")
+
     adjoint = None
+    trajectories = None
+    del sv
     if not args.no_adjoint:
-        del sv
         adjoint = adjoint_bench(lq, W, args.seed, args.adjoint_circuits, args.ps_seconds)
+    if not args.no_trajectories:
+        trajectories = trajectories_bench(lq, W, args.seed, args.traj_qubits, args.traj_shots)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(n, layers, args.seed, args.cpu_seconds)
 
@@ -268,6 +320,8 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "adjoint": adjoint,
+        "trajectories": trajectories,
+        "planner": planner,
     }
     print(json.dumps(line), flush=True)
     return 0
