@@ -133,8 +133,9 @@ lqcs_status lqcs_plan_describe(uint32_t n_qubits, const lqcs_program* prog, cons
 
 /* Kernel used for tiled passes: 1 = register-resident tile, one workgroup
  * per tile (default); 2 = register tile, streaming (each workgroup walks
- * tiles and prefetches the next one); 0 = LDS tile. A switch for
- * comparisons; results agree to rounding. */
+ * tiles and prefetches the next one into registers); 3 = register tile,
+ * landing (one workgroup per CU, the next tile prefetched into LDS by DMA);
+ * 0 = LDS tile. A switch for comparisons; results agree to rounding. */
 lqcs_status lqcs_set_tile_kernel(int kernel);
 
 /* Launch timing (no counterpart in the reference): while enabled, every

@@ -47,6 +47,7 @@ thread_local std::string g_last_error;
 }  // namespace
 
 extern bool g_rtile_stream;  // rtile.hip
+extern bool g_rtile_land;    // rtile.hip
 
 namespace {
 // Register tiles on (default) or off (LDS tiles only), for comparisons.
@@ -669,11 +670,12 @@ lqcs_status lqcs_dev_synchronize(void) {
 
 lqcs_status lqcs_set_tile_kernel(int kernel) {
     return guarded([&] {
-        if (kernel < 0 || kernel > 2)
+        if (kernel < 0 || kernel > 3)
             throw Error(LQCS_ERR_INVALID_ARGUMENT,
-                        "tile kernel must be 0 (LDS), 1 (register) or 2 (register, streaming)");
+                        "tile kernel must be 0 (LDS), 1 (register), 2 (register, streaming) or 3 (register, landing)");
         g_use_rtile = kernel >= 1;
         g_rtile_stream = kernel == 2;
+        g_rtile_land = kernel == 3;
     });
 }
 

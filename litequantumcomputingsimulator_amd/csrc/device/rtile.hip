@@ -6,7 +6,8 @@
 // layout. A gate whose non-diagonal support lies in the register bits runs
 // entirely inside each thread: no LDS traffic, no barrier. The pass is split
 // into phases; between phases the tile is transposed through LDS into the
-// next phase's layout (one LDS write + read of the tile, two barriers).
+// next phase's layout, half a tile at a time (32 KiB of LDS), so that with
+// 8 KiB of staged op words three workgroups fit on a CU.
 //
 // Global load and store use the first and last phases' layouts. Their
 // register bits avoid the tile's contiguous low qubits, so that the lanes of
@@ -27,6 +28,8 @@ namespace lqcs_dev {
 
 // Streaming register tiles (next-tile prefetch) and their workgroups per CU.
 bool g_rtile_stream = false;
+// Landing-buffer register tiles (LDS DMA prefetch of the next tile).
+bool g_rtile_land = false;
 uint32_t g_rtile_blocks_per_cu = 2;
 
 namespace {
@@ -54,14 +57,6 @@ __device__ __forceinline__ uint32_t pdep32(uint32_t v, uint32_t mask) {
         mask ^= low;
     }
     return r;
-}
-
-// LDS slot of tile index j: low 3 bits XOR-folded with higher bits so that
-// lanes differing only in high bits still spread over the LDS banks. The map
-// is linear over XOR, so slot(jt | jr) = slot(jt) ^ slot(jr) for disjoint jt,
-// jr; the host precomputes slot(jr) per register.
-__host__ __device__ __forceinline__ uint32_t swz(uint32_t j) {
-    return j ^ (((j >> 3) ^ (j >> 6) ^ (j >> 9)) & 7u);
 }
 
 using Regs = amp_t[kRR];
@@ -360,27 +355,39 @@ __device__ __forceinline__ void stage_words(amp_t* staged, const amp_t* words, u
 __device__ __forceinline__ void run_phases(Regs& v, Regs& w, amp_t* lds, amp_t* staged,
                                            const RPhase* __restrict__ phases, uint32_t n_phases,
                                            const amp_t* __restrict__ words, uint32_t tile, bool& lds_used) {
-    uint32_t sjt_prev = 0;
+    uint32_t jt_prev = 0;
     for (uint32_t p = 0; p < n_phases; ++p) {
         const RPhase& ph = phases[p];
         const uint32_t jt = pdep32(threadIdx.x, ph.tmask);
-        const uint32_t sjt = swz(jt);
         if (p > 0 && ph.transpose) {
-            const RPhase& pp = phases[p - 1];
-            if (lds_used) __syncthreads();  // every thread has read the previous layout
+            // Half a tile per round: the amplitudes whose bit h is `half`.
+            // Bit h is a thread bit of both layouts, so a thread writes all
+            // or none of its registers in a round, and reads likewise; reads
+            // land in w so that nothing still to be written is overwritten.
+            const uint32_t h = ph.h;
+            const uint32_t hp = (jt_prev >> h) & 1u, hc = (jt >> h) & 1u;
+            const uint32_t sp = rtile_slot(jt_prev, h), sc = rtile_slot(jt, h);
+            for (uint32_t half = 0; half < 2; ++half) {
+                if (lds_used) __syncthreads();  // every thread has read the previous round
+                if (hp == half) {
 #pragma unroll
-            for (int r = 0; r < kRR; ++r) lds[sjt_prev ^ pp.sjr[r]] = v[r];
-            __syncthreads();
+                    for (int r = 0; r < kRR; ++r) lds[sp ^ ph.cs_prev[r]] = v[r];
+                }
+                __syncthreads();
+                if (hc == half) {
 #pragma unroll
-            for (int r = 0; r < kRR; ++r) v[r] = lds[sjt ^ ph.sjr[r]];
-            lds_used = true;
+                    for (int r = 0; r < kRR; ++r) w[r] = lds[sc ^ ph.cs_cur[r]];
+                }
+                lds_used = true;
+            }
+            copy_regs(w, v);
         }
+        jt_prev = jt;
         if (p > 0 && ph.seg_words) {
             __syncthreads();  // every thread is done with the staged op words
             stage_words(staged, words, ph.seg_off, ph.seg_words);
             __syncthreads();
         }
-        sjt_prev = sjt;
         uint32_t off = ph.op_off;
         uint32_t i = 0;
         for (; i + 1 < ph.n_ops; i += 2) {
@@ -398,7 +405,7 @@ __device__ __forceinline__ void run_phases(Regs& v, Regs& w, amp_t* lds, amp_t* 
 __global__ void __launch_bounds__(kRT) k_rtile(amp_t* __restrict__ s, uint32_t nq, uint64_t smask,
                                               const RPhase* __restrict__ phases, uint32_t n_phases,
                                               const amp_t* __restrict__ words) {
-    __shared__ amp_t lds[1 << kRL];
+    __shared__ amp_t lds[1 << (kRL - 1)];
     __shared__ amp_t staged[kRTileWords];
     amp_t* b = s + (uint64_t(blockIdx.y) << nq);
     const uint32_t tile = blockIdx.x;
@@ -443,7 +450,7 @@ __global__ void __launch_bounds__(kRT) k_rtile_stream(amp_t* __restrict__ s, uin
                                                      const RPhase* __restrict__ phases, uint32_t n_phases,
                                                      const amp_t* __restrict__ words, uint32_t tiles,
                                                      uint32_t restage) {
-    __shared__ amp_t lds[1 << kRL];
+    __shared__ amp_t lds[1 << (kRL - 1)];
     __shared__ amp_t staged[kRTileWords];
     amp_t* b = s + (uint64_t(blockIdx.y) << nq);
     const uint64_t outside = ((uint64_t{1} << nq) - 1) & ~smask;
@@ -488,6 +495,86 @@ __global__ void __launch_bounds__(kRT) k_rtile_stream(amp_t* __restrict__ s, uin
             __syncthreads();
         }
         copy_regs(nx, v);
+        tile = next;
+    }
+}
+
+// One 16-byte-per-lane DMA from global memory into LDS: lane i's 16 bytes
+// land at LDS byte address lds_addr + 16 i (lds_addr wave-uniform). Written
+// in asm so that the compiler's own wait counting does not treat every
+// later barrier as needing the transfer complete; the kernel below waits for
+// it explicitly. No vector register is written, so nothing is in flight in
+// a register.
+__device__ __forceinline__ void dma16(const void* g, uint32_t lds_addr) {
+    asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(lds_addr) : "memory", "m0");
+}
+
+// s_waitcnt vmcnt(16): all but the 16 most recent vector-memory operations
+// done (vmcnt low bits [3:0] = 0, high bits [15:14] = 1; expcnt and lgkmcnt
+// at their maxima).
+__device__ __forceinline__ void wait_vmem_but_16() { __builtin_amdgcn_s_waitcnt(0x4F70); }
+
+// Landing form: one workgroup per CU walks tiles blockIdx.x, + gridDim.x, ...
+// The next tile streams from HBM straight into an LDS landing buffer (64 KiB,
+// tile index order) by DMA while the current tile's phases run in
+// registers; the tile is then read from LDS in phase 0's layout, so loads
+// need no registers and no particular layout.
+__global__ void __launch_bounds__(kRT) k_rtile_land(amp_t* __restrict__ s, uint32_t nq, uint64_t smask,
+                                                   const RPhase* __restrict__ phases, uint32_t n_phases,
+                                                   const amp_t* __restrict__ words, uint32_t tiles,
+                                                   uint32_t restage) {
+    __shared__ amp_t land[1 << kRL];
+    __shared__ amp_t lds[1 << (kRL - 1)];
+    __shared__ amp_t staged[kRTileWords];
+    amp_t* b = s + (uint64_t(blockIdx.y) << nq);
+    const uint64_t outside = ((uint64_t{1} << nq) - 1) & ~smask;
+    // Tile index j = 256 k + thread: the thread's part over the 8 lowest
+    // tile qubits, k's part over the 4 highest.
+    uint64_t lo_mask = 0, hi_mask = smask;
+    for (int i = 0; i < 8; ++i) {
+        const uint64_t bit = hi_mask & (~hi_mask + 1);
+        lo_mask |= bit;
+        hi_mask ^= bit;
+    }
+    const uint64_t lane_off = pdep64(threadIdx.x, lo_mask);
+    const uint32_t wave = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6)));
+    const uint32_t land_base = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(land)) + wave * 64 * 16;
+    auto issue = [&](uint32_t t) {
+        const amp_t* g = b + (pdep64(t, outside) | lane_off);
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k) dma16(g + pdep64(k, hi_mask), land_base + k * 256 * 16);
+    };
+    const RPhase& p0 = phases[0];
+    const RPhase& pl = phases[n_phases - 1];
+    const uint32_t jt0 = pdep32(threadIdx.x, p0.tmask);
+    const uint64_t gtl = pdep64(threadIdx.x, pl.tmask_global);
+    uint32_t tile = blockIdx.x;
+    stage_words(staged, words, p0.seg_off, p0.seg_words);
+    issue(tile);
+    wait_vmem();
+    __syncthreads();
+    bool lds_used = false;
+    Regs v, w;
+    for (;;) {
+#pragma unroll
+        for (int r = 0; r < kRR; ++r) v[r] = land[jt0 | p0.jr[r]];
+        __syncthreads();  // every thread has read the landing buffer
+        const uint32_t next = tile + gridDim.x;
+        if (next < tiles) issue(next);
+        run_phases(v, w, lds, staged, phases, n_phases, words, tile, lds_used);
+        {
+            const uint64_t gt = pdep64(tile, outside) | gtl;
+#pragma unroll
+            for (int r = 0; r < kRR; ++r) b[gt | pl.gofs[r]] = v[r];
+        }
+        if (next >= tiles) break;
+        if (restage) {
+            __syncthreads();  // every thread is done with the last segment
+            stage_words(staged, words, p0.seg_off, p0.seg_words);
+        }
+        wait_vmem_but_16();  // the next tile's DMA (older than the 16 stores) has landed
+        if (restage) wait_vmem();
+        __syncthreads();
         tile = next;
     }
 }
@@ -659,8 +746,28 @@ bool build_rtile(const std::vector<TileOp>& tops, uint32_t L, uint64_t smask, ui
             for (int i = 0; i < 4; ++i)
                 if (r & (1 << i)) j |= 1u << rb[i];
             ph.jr[r] = j;
-            ph.sjr[r] = swz(j);
             ph.gofs[r] = pdep_host(j, smask);
+        }
+        if (ph.transpose) {
+            // Split bit: a thread bit of both layouts, preferring one that is
+            // the top thread-id bit in both (then whole waves take turns).
+            const RPhase& pp = out.phases.back();
+            const uint32_t both = pp.tmask & ph.tmask;
+            auto rank = [](uint32_t mask, uint32_t bit) { return popc(mask & ((1u << bit) - 1u)); };
+            int best = -1, best_score = -1;
+            for (uint32_t b = 0; b < static_cast<uint32_t>(kRL); ++b) {
+                if (!(both & (1u << b))) continue;
+                const int score = rank(pp.tmask, b) + rank(ph.tmask, b);
+                if (score > best_score) {
+                    best_score = score;
+                    best = static_cast<int>(b);
+                }
+            }
+            ph.h = static_cast<uint32_t>(best);
+            for (int r = 0; r < kRR; ++r) {
+                ph.cs_prev[r] = rtile_slot(pp.jr[r], ph.h);
+                ph.cs_cur[r] = rtile_slot(ph.jr[r], ph.h);
+            }
         }
         auto ridx = [&](uint32_t pos) {
             for (int i = 0; i < 4; ++i)
@@ -740,6 +847,11 @@ bool build_rtile(const std::vector<TileOp>& tops, uint32_t L, uint64_t smask, ui
 void launch_rtile(amp_t* s, uint32_t nq, uint64_t smask, const RPhase* d_phases, uint32_t n_phases,
                   const amp_t* d_words, bool restage, uint32_t batch) {
     const uint64_t tiles = uint64_t{1} << (nq - kRL);
+    if (g_rtile_land && tiles > uint64_t(num_cus())) {
+        k_rtile_land<<<dim3(static_cast<unsigned>(num_cus()), batch), kRT, 0, stream()>>>(
+            s, nq, smask, d_phases, n_phases, d_words, static_cast<uint32_t>(tiles), restage ? 1u : 0u);
+        return;
+    }
     const uint64_t resident = uint64_t(num_cus()) * g_rtile_blocks_per_cu;
     if (g_rtile_stream && tiles > resident) {
         k_rtile_stream<<<dim3(static_cast<unsigned>(resident), batch), kRT, 0, stream()>>>(

@@ -45,7 +45,7 @@ struct RegOpHdr {
 };
 static_assert(sizeof(RegOpHdr) == 2 * sizeof(amp_t), "op header is two staged words");
 
-constexpr uint32_t kRTileWords = 1024;  // op words staged per workgroup (16 KiB of LDS)
+constexpr uint32_t kRTileWords = 512;  // op words staged per workgroup (8 KiB of LDS)
 
 struct RPhase {
     uint32_t n_ops;
@@ -56,9 +56,24 @@ struct RPhase {
     uint32_t transpose;      // 1: the tile is transposed into this layout first
     uint64_t tmask_global;   // their global qubit positions
     uint32_t jr[16];         // tile index of register r (thread bits zero)
-    uint32_t sjr[16];        // its LDS slot contribution (XOR-linear swizzle)
     uint64_t gofs[16];       // global offset of register r
+    // The transpose into this layout moves the tile through LDS in two
+    // halves, split by tile bit h (a thread bit of both layouts): the LDS
+    // slot of tile index j is slot(j without bit h), XOR-linear, so slot(jt |
+    // jr) = slot(jt) ^ slot(jr); these are the register parts.
+    uint32_t h;
+    uint32_t pad2;
+    uint32_t cs_prev[16];    // previous layout's register r
+    uint32_t cs_cur[16];     // this layout's register r
 };
+
+// LDS slot of tile index j in a half-tile transpose split by bit h: j with
+// bit h removed (11 bits), low 3 bits XOR-folded with higher ones so that
+// lanes differing only in high bits spread over the banks.
+__host__ __device__ __forceinline__ uint32_t rtile_slot(uint32_t j, uint32_t h) {
+    const uint32_t c = (j & ((1u << h) - 1u)) | ((j >> (h + 1)) << h);
+    return c ^ (((c >> 3) ^ (c >> 6) ^ (c >> 9)) & 7u);
+}
 
 struct RTileProgram {
     std::vector<RPhase> phases;

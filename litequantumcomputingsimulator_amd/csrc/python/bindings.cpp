@@ -394,8 +394,8 @@ PYBIND11_MODULE(_pylqcs, m) {
     });
     m.def("profile_enable", &profile_enable, "on"_a);
     m.def("set_tile_kernel", [](const std::string& k) {
-        const int code = k == "stream" ? 2 : k == "register" ? 1 : k == "lds" ? 0 : -1;
-        if (code < 0) throw std::invalid_argument("tile kernel must be 'register', 'stream' or 'lds'");
+        const int code = k == "land" ? 3 : k == "stream" ? 2 : k == "register" ? 1 : k == "lds" ? 0 : -1;
+        if (code < 0) throw std::invalid_argument("tile kernel must be 'register', 'stream', 'land' or 'lds'");
         if (lqcs_set_tile_kernel(code) != LQCS_OK) throw std::runtime_error(lqcs_dev_last_error());
     }, "kernel"_a);
     m.def("profile_read", [] {

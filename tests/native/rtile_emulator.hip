@@ -215,6 +215,34 @@ int main(int argc, char** argv) {
                 const RPhase& ph = prog.phases[p];
                 if (__builtin_popcount(ph.tmask) != 8 || (ph.tmask & ~0xfffu)) { why = "tmask"; return -1; }
                 if (p > 0 && !ph.transpose && ph.tmask != prog.phases[p - 1].tmask) { why = "layout change without transpose"; return -1; }
+                if (p > 0 && ph.transpose) {
+                    // The two half-tile rounds through LDS, slot by slot: every
+                    // slot written once per round, and every read finds the
+                    // amplitude of the tile index it stands for.
+                    const RPhase& pp = prog.phases[p - 1];
+                    const uint32_t h = ph.h;
+                    if (h >= 12 || !(pp.tmask & ph.tmask & (1u << h))) { why = "split bit is not a thread bit of both layouts"; return -1; }
+                    for (uint32_t half = 0; half < 2; ++half) {
+                        std::vector<int64_t> owner(kN / 2, -1);
+                        for (uint32_t t = 0; t < 256; ++t) {
+                            const uint32_t jp = static_cast<uint32_t>(pdep(t, pp.tmask));
+                            if (((jp >> h) & 1u) != half) continue;
+                            for (int r = 0; r < 16; ++r) {
+                                const uint32_t slot = lqcs_dev::rtile_slot(jp, h) ^ ph.cs_prev[r];
+                                if (slot >= kN / 2 || owner[slot] != -1) { why = "LDS slot collision"; return -1; }
+                                owner[slot] = jp | pp.jr[r];
+                            }
+                        }
+                        for (uint32_t t = 0; t < 256; ++t) {
+                            const uint32_t jc = static_cast<uint32_t>(pdep(t, ph.tmask));
+                            if (((jc >> h) & 1u) != half) continue;
+                            for (int r = 0; r < 16; ++r) {
+                                const uint32_t slot = lqcs_dev::rtile_slot(jc, h) ^ ph.cs_cur[r];
+                                if (slot >= kN / 2 || owner[slot] != int64_t(jc | ph.jr[r])) { why = "LDS read of the wrong amplitude"; return -1; }
+                            }
+                        }
+                    }
+                }
                 if (p == 0 || ph.seg_words) {
                     if (ph.seg_words > lqcs_dev::kRTileWords || ph.seg_off + ph.seg_words > prog.words.size()) { why = "segment bounds"; return -1; }
                     for (uint32_t i = 0; i < ph.seg_words; ++i) staged[i] = prog.words[ph.seg_off + i];

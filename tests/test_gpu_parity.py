@@ -243,7 +243,7 @@ def test_errors_keep_reference_types(lq):
         _ = sv[4]
 
 
-@pytest.mark.parametrize("kernel", ["register", "stream", "lds"])
+@pytest.mark.parametrize("kernel", ["register", "stream", "land", "lds"])
 @pytest.mark.parametrize("plan,low", [("greedy", 4), ("learned", 3), ("learned", 2), ("greedy", 5)])
 def test_tile_kernels_and_planners_match_oracle(lq, kernel, plan, low):
     n = 17
@@ -323,3 +323,37 @@ def test_noisy_trajectories_match_density_matrix(lq, fixture, n, rate):
             e_acc = o_acc = 0.0
     dof = groups - 1
     assert chi2 <= dof + 5 * np.sqrt(2 * dof), (chi2, dof)
+
+
+@pytest.mark.parametrize("kernel", ["stream", "land"])
+def test_walking_tile_kernels_match_register_kernel(lq, kernel):
+    """The tile-walking kernels (next tile prefetched into registers, or into
+    an LDS landing buffer by DMA) at a width with more tiles than workgroups
+    (22 qubits: 1024 tiles), on a circuit whose passes need several op-word
+    segments: same state as the one-tile-per-workgroup kernel, which the
+    oracle checks above."""
+    n = 22
+    qc = lq.QuantumCircuit(n)
+    rng = np.random.default_rng(21)
+    for i in range(200):  # one pass of 400 ops on qubits 0..7: op words span several segments
+        qc.u(*(float(a) for a in rng.uniform(0, 6.28, 3)), i % 8)
+        qc.cx(i % 8, (i + 3) % 8)
+    W.random_u3_cx_cp(qc, n, 40, seed=21)
+    for q in range(0, n - 2, 3):
+        qc.ccx(q, q + 1, q + 2)
+    comp = lq.CompiledCircuit(qc, True)
+    want = lq.statevector(n)
+    comp.apply(want, plan="greedy", tile_qubits=12, low_qubits=4)
+    lq.set_tile_kernel(kernel)
+    try:
+        got = lq.statevector(n)
+        comp.apply(got, plan="greedy", tile_qubits=12, low_qubits=4)
+        got2 = lq.statevector(n)
+        comp.apply(got2, plan="gain", tile_qubits=12, low_qubits=2)
+    finally:
+        lq.set_tile_kernel("register")
+    ref2 = lq.statevector(n)
+    comp.apply(ref2, plan="gain", tile_qubits=12, low_qubits=2)
+    assert got.max_abs_diff(want) <= 1e-12
+    assert got2.max_abs_diff(ref2) <= 1e-12
+    assert abs(got.norm() - 1.0) <= 1e-10
