@@ -12,6 +12,8 @@
 #include "engine.hpp"
 #include "statevector.hpp"
 
+struct lqcs_graph;
+
 namespace lqcs {
 
 namespace detail {
@@ -42,9 +44,47 @@ public:
     std::size_t num_qubits() const { return nq_; }
     std::size_t num_ops() const;
 
+    // Autotuner: times every candidate tile shape on `scratch` (a register of
+    // the circuit's width, overwritten) -- greedy and gain grouping at 2, 3
+    // and 4 low qubits on the register tile, and the other tile kernels at
+    // the default shape -- keeping the fastest; apply() with Plan::Auto then
+    // uses it. Each candidate: one untimed apply, then the minimum of `reps`
+    // timed ones (kernel time from HIP events).
+    struct TuneTrial {
+        EngineOptions options;
+        double ms;
+    };
+    struct TuneResult {
+        EngineOptions best;
+        double best_ms = 0.0;
+        std::vector<TuneTrial> trials;
+    };
+    TuneResult autotune(Statevector& scratch, int reps = 2);
+    // The tuned options (Plan::Auto when not tuned).
+    EngineOptions tuned() const;
+
 private:
+    friend class CircuitGraph;
+    EngineOptions resolve(const EngineOptions& e) const;
     std::shared_ptr<detail::ProgramBuilder> prog_;
     std::size_t nq_;
+    std::shared_ptr<EngineOptions> tuned_;
+};
+
+// A compiled circuit's launches on one register, recorded once into a HIP
+// graph and replayed: launch() applies U to the register again (queued on
+// the engine stream; synchronize() waits). The register must outlive it.
+class CircuitGraph {
+public:
+    CircuitGraph(const CompiledCircuit& circuit, Statevector& sv, const EngineOptions& engine = {});
+    ~CircuitGraph();
+    CircuitGraph(const CircuitGraph&) = delete;
+    CircuitGraph& operator=(const CircuitGraph&) = delete;
+    void launch();
+
+private:
+    lqcs_graph* graph_ = nullptr;
+    Statevector* sv_;
 };
 
 // Waits for all engine work.

@@ -109,16 +109,38 @@ enum {
     LQCS_PLAN_GAIN = 4           /* tiled passes, tile sets grown by marginal gain */
 };
 
+/* Kernel for tiled passes, per call (lqcs_exec_options.tile_kernel). */
+enum {
+    LQCS_TILE_DEFAULT = 0,   /* the one lqcs_set_tile_kernel selected */
+    LQCS_TILE_LDS = 1,
+    LQCS_TILE_REGISTER = 2,
+    LQCS_TILE_STREAM = 3,    /* register tile, next tile prefetched into registers */
+    LQCS_TILE_LAND = 4       /* register tile, next tile prefetched into LDS by DMA */
+};
+
 typedef struct lqcs_exec_options {
     int32_t plan;         /* LQCS_PLAN_* */
     int32_t tile_qubits;  /* tile width L (0 = engine default) */
     int32_t low_qubits;   /* contiguous low qubits kept in every tile (0 = default) */
-    int32_t reserved;
+    int32_t tile_kernel;  /* LQCS_TILE_* */
 } lqcs_exec_options;
 
 /* Applies the whole op stream in order, in place. opts may be NULL. */
 lqcs_status lqcs_state_apply(lqcs_dev_state* s, const lqcs_program* prog,
                              const lqcs_exec_options* opts);
+/* ---- captured launch sequences (no counterpart in the reference): the
+ * launches lqcs_state_apply would issue for (s, prog, opts), recorded once
+ * into a HIP graph with the tile programs in device memory owned by the
+ * graph, then replayed on s by lqcs_graph_launch (asynchronous; ordered with
+ * the engine's other work; lqcs_dev_synchronize waits). Stand-alone
+ * controlled-unitary and controlled-permutation steps cannot be captured
+ * (their tables are staged per call): INVALID_ARGUMENT. */
+typedef struct lqcs_graph lqcs_graph;
+lqcs_status lqcs_graph_create(lqcs_dev_state* s, const lqcs_program* prog, const lqcs_exec_options* opts,
+                              lqcs_graph** out);
+lqcs_status lqcs_graph_launch(lqcs_graph* g);
+lqcs_status lqcs_graph_destroy(lqcs_graph* g);
+
 /* Plan summary of the last lqcs_state_apply on this thread: HBM passes and
  * kernel launches it issued. */
 lqcs_status lqcs_last_plan_stats(uint64_t* passes, uint64_t* launches);

@@ -469,48 +469,41 @@ Plan plan_greedy(const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uin
     return plan;
 }
 
-void execute_plan(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan,
-                  const std::vector<DecodedOp>& ops, uint32_t batch) {
+StagedPlan stage_plan(const Plan& plan, uint32_t nq, uint32_t L) {
     // Register-tile programs where possible, LDS tile programs otherwise.
-    std::vector<TileOp> flat;
-    std::vector<RPhase> rph;
-    std::vector<amp_t> rwords;
-    struct Where {
-        int kind;  // 0 stand-alone, 1 LDS tile, 2 register tile
-        size_t first, count, words;
-        bool restage;
-    };
-    std::vector<Where> where;
+    StagedPlan sp;
     for (const auto& st : plan.steps) {
         if (!st.tiled) {
-            where.push_back({0, 0, 0, 0, false});
+            sp.where.push_back({0, 0, 0, 0, false});
             continue;
         }
         RTileProgram rp;
         if (g_use_rtile && build_rtile(st.tile_ops, L, st.smask, nq, rp)) {
             bool restage = false;
             for (size_t p = 1; p < rp.phases.size(); ++p) restage |= rp.phases[p].seg_words != 0;
-            where.push_back({2, rph.size(), rp.phases.size(), rwords.size(), restage});
-            rph.insert(rph.end(), rp.phases.begin(), rp.phases.end());
-            rwords.insert(rwords.end(), rp.words.begin(), rp.words.end());
+            sp.where.push_back({2, sp.rph.size(), rp.phases.size(), sp.rwords.size(), restage});
+            sp.rph.insert(sp.rph.end(), rp.phases.begin(), rp.phases.end());
+            sp.rwords.insert(sp.rwords.end(), rp.words.begin(), rp.words.end());
         } else {
-            where.push_back({1, flat.size(), st.tile_ops.size(), 0, false});
-            flat.insert(flat.end(), st.tile_ops.begin(), st.tile_ops.end());
+            sp.where.push_back({1, sp.flat.size(), st.tile_ops.size(), 0, false});
+            sp.flat.insert(sp.flat.end(), st.tile_ops.begin(), st.tile_ops.end());
         }
     }
-    const TileOp* d_ops = flat.empty() ? nullptr : stage(flat.data(), flat.size());
-    const RPhase* d_ph = rph.empty() ? nullptr : stage(rph.data(), rph.size());
-    const amp_t* d_rwords = rwords.empty() ? nullptr : stage(rwords.data(), rwords.size());
+    return sp;
+}
+
+void launch_staged(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan, const std::vector<DecodedOp>& ops,
+                   uint32_t batch, const StagedPlan& sp, const TileOp* d_ops, const RPhase* d_ph,
+                   const amp_t* d_rwords, bool timed) {
     for (size_t i = 0; i < plan.steps.size(); ++i) {
         const PlanStep& st = plan.steps[i];
-        const bool timed = prof().on;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timed) {
             e0 = take_event();
             e1 = take_event();
             LQCS_HIP(hipEventRecord(e0, stream()));
         }
-        const Where& w = where[i];
+        const StagedPlan::Where& w = sp.where[i];
         if (w.kind == 2) {
             launch_rtile(s, nq, st.smask, d_ph + w.first, static_cast<uint32_t>(w.count),
                          d_rwords ? d_rwords + w.words : nullptr, w.restage, batch);
@@ -524,6 +517,15 @@ void execute_plan(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan,
             prof().launches.push_back({st.tiled, e0, e1});
         }
     }
+}
+
+void execute_plan(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan,
+                  const std::vector<DecodedOp>& ops, uint32_t batch) {
+    const StagedPlan sp = stage_plan(plan, nq, L);
+    const TileOp* d_ops = sp.flat.empty() ? nullptr : stage(sp.flat.data(), sp.flat.size());
+    const RPhase* d_ph = sp.rph.empty() ? nullptr : stage(sp.rph.data(), sp.rph.size());
+    const amp_t* d_rwords = sp.rwords.empty() ? nullptr : stage(sp.rwords.data(), sp.rwords.size());
+    launch_staged(s, nq, L, plan, ops, batch, sp, d_ops, d_ph, d_rwords, prof().on);
     g_last_passes = plan.steps.size();
     g_last_launches = plan.steps.size();
 }
@@ -566,6 +568,29 @@ void apply_ops(amp_t* s, uint32_t nq, const std::vector<DecodedOp>& ops, int pla
 }
 
 bool rtile_enabled() { return g_use_rtile; }
+
+// The tile kernel of one call (lqcs_exec_options.tile_kernel), restored on exit.
+class KernelScope {
+public:
+    explicit KernelScope(const lqcs_exec_options* o) : use_(g_use_rtile), stream_(g_rtile_stream), land_(g_rtile_land) {
+        const int k = o ? o->tile_kernel : LQCS_TILE_DEFAULT;
+        if (k == LQCS_TILE_DEFAULT) return;
+        if (k < LQCS_TILE_LDS || k > LQCS_TILE_LAND) throw Error(LQCS_ERR_INVALID_ARGUMENT, "unknown tile kernel");
+        g_use_rtile = k != LQCS_TILE_LDS;
+        g_rtile_stream = k == LQCS_TILE_STREAM;
+        g_rtile_land = k == LQCS_TILE_LAND;
+    }
+    ~KernelScope() {
+        g_use_rtile = use_;
+        g_rtile_stream = stream_;
+        g_rtile_land = land_;
+    }
+    KernelScope(const KernelScope&) = delete;
+    KernelScope& operator=(const KernelScope&) = delete;
+
+private:
+    bool use_, stream_, land_;
+};
 
 Plan make_plan(int kind, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t low) {
     Plan plan;
@@ -816,9 +841,92 @@ lqcs_status lqcs_state_apply(lqcs_dev_state* s, const lqcs_program* prog, const 
         std::vector<DecodedOp> ops;
         ops.reserve(prog->n_ops);
         for (uint64_t k = 0; k < prog->n_ops; ++k) ops.push_back(decode(*prog, k, s->nq));
+        const KernelScope ks(opts);
         uint32_t L = 0;
         const Plan plan = choose_plan(opts, ops, s->nq, 1, L);
         execute_plan(s->d, s->nq, L, plan, ops, 1);
+    });
+}
+
+struct lqcs_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void* buf = nullptr;  // the tile programs the captured kernels read
+};
+
+lqcs_status lqcs_graph_create(lqcs_dev_state* s, const lqcs_program* prog, const lqcs_exec_options* opts,
+                              lqcs_graph** out) {
+    return guarded([&] {
+        need(s, "state");
+        need(prog, "program");
+        need(out, "out");
+        *out = nullptr;
+        if (prog->n_ops && !prog->ops) throw Error(LQCS_ERR_INVALID_ARGUMENT, "ops is null");
+        ensure_ready();
+        std::vector<DecodedOp> ops;
+        for (uint64_t k = 0; k < prog->n_ops; ++k) ops.push_back(decode(*prog, k, s->nq));
+        const KernelScope ks(opts);
+        uint32_t L = 0;
+        const Plan plan = choose_plan(opts, ops, s->nq, 1, L);
+        for (const PlanStep& st : plan.steps)
+            if (!st.tiled && (ops[st.op].kind == LQCS_OP_CONTROLLED_UNITARY ||
+                              ops[st.op].kind == LQCS_OP_CONTROLLED_PERMUTATION))
+                throw Error(LQCS_ERR_INVALID_ARGUMENT,
+                            "graph capture: a stand-alone controlled-unitary or permutation step stages tables per call");
+        const StagedPlan sp = stage_plan(plan, s->nq, L);
+        auto pad = [](size_t b) { return (b + 255) & ~size_t{255}; };
+        const size_t b_ops = pad(sp.flat.size() * sizeof(TileOp)), b_ph = pad(sp.rph.size() * sizeof(RPhase));
+        const size_t b_w = pad(sp.rwords.size() * sizeof(amp_t));
+        auto* g = new lqcs_graph;
+        try {
+            g->buf = device_alloc(std::max<size_t>(b_ops + b_ph + b_w, 256));
+            char* base = static_cast<char*>(g->buf);
+            auto* d_ops = reinterpret_cast<TileOp*>(base);
+            auto* d_ph = reinterpret_cast<RPhase*>(base + b_ops);
+            auto* d_w = reinterpret_cast<amp_t*>(base + b_ops + b_ph);
+            sync();
+            if (!sp.flat.empty()) LQCS_HIP(hipMemcpy(d_ops, sp.flat.data(), sp.flat.size() * sizeof(TileOp), hipMemcpyHostToDevice));
+            if (!sp.rph.empty()) LQCS_HIP(hipMemcpy(d_ph, sp.rph.data(), sp.rph.size() * sizeof(RPhase), hipMemcpyHostToDevice));
+            if (!sp.rwords.empty()) LQCS_HIP(hipMemcpy(d_w, sp.rwords.data(), sp.rwords.size() * sizeof(amp_t), hipMemcpyHostToDevice));
+            LQCS_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+            try {
+                launch_staged(s->d, s->nq, L, plan, ops, 1, sp, sp.flat.empty() ? nullptr : d_ops,
+                              sp.rph.empty() ? nullptr : d_ph, sp.rwords.empty() ? nullptr : d_w, false);
+            } catch (...) {
+                hipGraph_t dead = nullptr;
+                (void)hipStreamEndCapture(stream(), &dead);
+                if (dead) (void)hipGraphDestroy(dead);
+                throw;
+            }
+            LQCS_HIP(hipStreamEndCapture(stream(), &g->graph));
+            LQCS_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+        } catch (...) {
+            if (g->exec) (void)hipGraphExecDestroy(g->exec);
+            if (g->graph) (void)hipGraphDestroy(g->graph);
+            if (g->buf) device_free(g->buf);
+            delete g;
+            throw;
+        }
+        *out = g;
+    });
+}
+
+lqcs_status lqcs_graph_launch(lqcs_graph* g) {
+    return guarded([&] {
+        need(g, "graph");
+        ensure_ready();
+        LQCS_HIP(hipGraphLaunch(g->exec, stream()));
+    });
+}
+
+lqcs_status lqcs_graph_destroy(lqcs_graph* g) {
+    return guarded([&] {
+        if (!g) return;
+        sync();
+        if (g->exec) LQCS_HIP(hipGraphExecDestroy(g->exec));
+        if (g->graph) LQCS_HIP(hipGraphDestroy(g->graph));
+        if (g->buf) device_free(g->buf);
+        delete g;
     });
 }
 
@@ -831,6 +939,7 @@ lqcs_status lqcs_plan_describe(uint32_t n_qubits, const lqcs_program* prog, cons
         if (n_qubits == 0 || n_qubits > LQCS_DEVICE_MAX_QUBITS) throw Error(LQCS_ERR_INVALID_ARGUMENT, "bad width");
         std::vector<DecodedOp> ops;
         for (uint64_t k = 0; k < prog->n_ops; ++k) ops.push_back(decode(*prog, k, n_qubits));
+        const KernelScope ks(opts);
         uint32_t L = 0;
         const Plan plan = choose_plan(opts, ops, n_qubits, 1, L);
         *n_steps = plan.steps.size();
@@ -856,6 +965,7 @@ lqcs_status lqcs_plan_features(uint32_t n_qubits, const lqcs_program* prog, cons
         if (n_qubits == 0 || n_qubits > LQCS_DEVICE_MAX_QUBITS) throw Error(LQCS_ERR_INVALID_ARGUMENT, "bad width");
         std::vector<DecodedOp> ops;
         for (uint64_t k = 0; k < prog->n_ops; ++k) ops.push_back(decode(*prog, k, n_qubits));
+        const KernelScope ks(opts);
         uint32_t L = 0;
         const Plan plan = choose_plan(opts, ops, n_qubits, std::max<uint32_t>(batch, 1), L);
         *n_steps = plan.steps.size();

@@ -55,6 +55,25 @@ Plan plan_gain(const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint3
 Plan plan_greedy(const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t low);
 void execute_plan(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan,
                   const std::vector<DecodedOp>& ops, uint32_t batch);
+// A plan's tile programs, ready to copy to the device: LDS-tile ops, and
+// register-tile phases and op words, with each step's place in them.
+struct StagedPlan {
+    struct Where {
+        int kind;  // 0 stand-alone, 1 LDS tile, 2 register tile
+        size_t first, count, words;
+        bool restage;
+    };
+    std::vector<TileOp> flat;
+    std::vector<RPhase> rph;
+    std::vector<amp_t> rwords;
+    std::vector<Where> where;
+};
+StagedPlan stage_plan(const Plan& plan, uint32_t nq, uint32_t L);
+// Launches every step of a staged plan whose programs are at d_ops / d_ph /
+// d_rwords on the device; `timed` brackets each launch with profile events.
+void launch_staged(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan, const std::vector<DecodedOp>& ops,
+                   uint32_t batch, const StagedPlan& sp, const TileOp* d_ops, const RPhase* d_ph,
+                   const amp_t* d_rwords, bool timed);
 void apply_ops(amp_t* s, uint32_t nq, const std::vector<DecodedOp>& ops, int plan_kind,
                uint32_t L, uint32_t low, uint32_t batch);
 // Cost-model guided planning (planner.hip); falls back to greedy.

@@ -143,6 +143,7 @@ lqcs_exec_options exec_options(const EngineOptions& e) {
     o.plan = static_cast<int32_t>(e.plan);
     o.tile_qubits = e.tile_qubits;
     o.low_qubits = e.low_qubits;
+    o.tile_kernel = e.tile_kernel;
     return o;
 }
 
@@ -299,10 +300,72 @@ CompiledCircuit::~CompiledCircuit() = default;
 
 std::size_t CompiledCircuit::num_ops() const { return prog_->size(); }
 
+EngineOptions CompiledCircuit::resolve(const EngineOptions& e) const {
+    if (e.plan != EngineOptions::Plan::Auto || !tuned_) return e;
+    EngineOptions t = *tuned_;
+    t.max_qubits = e.max_qubits;
+    t.max_batch = e.max_batch;
+    return t;
+}
+
+EngineOptions CompiledCircuit::tuned() const { return tuned_ ? *tuned_ : EngineOptions{}; }
+
+CompiledCircuit::TuneResult CompiledCircuit::autotune(Statevector& scratch, int reps) {
+    if (scratch.num_qubits() != nq_) throw std::invalid_argument("autotune: register width differs from the circuit");
+    reps = std::max(1, reps);
+    std::vector<EngineOptions> cands;
+    auto add = [&](EngineOptions::Plan plan, int low, int kernel) {
+        EngineOptions e;
+        e.plan = plan;
+        e.low_qubits = low;
+        e.tile_kernel = kernel;
+        cands.push_back(e);
+    };
+    for (auto plan : {EngineOptions::Plan::Greedy, EngineOptions::Plan::Gain})
+        for (int low : {2, 3, 4}) add(plan, low, 2);
+    for (int kernel : {1, 3, 4}) add(EngineOptions::Plan::Greedy, 0, kernel);
+    TuneResult r;
+    for (const EngineOptions& e : cands) {
+        apply(scratch, e);  // untimed
+        double best = -1.0;
+        for (int k = 0; k < reps; ++k) {
+            synchronize();
+            profile_enable(true);
+            (void)profile_read();
+            apply(scratch, e);
+            const LaunchProfile p = profile_read();
+            profile_enable(false);
+            const double ms = p.tile_ms + p.other_ms;
+            if (best < 0.0 || ms < best) best = ms;
+        }
+        r.trials.push_back({e, best});
+        if (r.trials.size() == 1 || best < r.best_ms) {
+            r.best_ms = best;
+            r.best = e;
+        }
+    }
+    tuned_ = std::make_shared<EngineOptions>(r.best);
+    return r;
+}
+
+CircuitGraph::CircuitGraph(const CompiledCircuit& circuit, Statevector& sv, const EngineOptions& engine) : sv_(&sv) {
+    if (sv.num_qubits() != circuit.nq_) throw std::invalid_argument("CircuitGraph: register width differs from the circuit");
+    const lqcs_program prog = circuit.prog_->program();
+    const lqcs_exec_options o = exec_options(circuit.resolve(engine));
+    check(lqcs_graph_create(sv.device(), &prog, &o, &graph_));
+}
+
+CircuitGraph::~CircuitGraph() { (void)lqcs_graph_destroy(graph_); }
+
+void CircuitGraph::launch() {
+    check(lqcs_graph_launch(graph_));
+    sv_->mark_device_newer();
+}
+
 void CompiledCircuit::apply(Statevector& sv, const EngineOptions& engine) const {
     if (sv.num_qubits() != nq_) throw std::invalid_argument("apply: register width differs from the circuit");
     const lqcs_program prog = prog_->program();
-    const lqcs_exec_options o = exec_options(engine);
+    const lqcs_exec_options o = exec_options(resolve(engine));
     check(lqcs_state_apply(sv.device(), &prog, &o));
     sv.mark_device_newer();
 }

@@ -43,6 +43,24 @@ EngineOptions::Plan plan_of(const std::string& s) {
     throw std::invalid_argument("plan must be gate_by_gate, greedy, gain, learned or auto");
 }
 
+int tile_kernel_of(const std::string& s) {
+    if (s == "default") return 0;
+    if (s == "lds") return 1;
+    if (s == "register") return 2;
+    if (s == "stream") return 3;
+    if (s == "land") return 4;
+    throw std::invalid_argument("tile_kernel must be default, lds, register, stream or land");
+}
+
+py::dict options_dict(const EngineOptions& e) {
+    static const char* plans[] = {"gate_by_gate", "greedy", "learned", "auto", "gain"};
+    static const char* kernels[] = {"default", "lds", "register", "stream", "land"};
+    const int p = static_cast<int>(e.plan);
+    return py::dict("plan"_a = std::string(p >= 0 && p <= 4 ? plans[p] : "?"), "tile_qubits"_a = e.tile_qubits,
+                    "low_qubits"_a = e.low_qubits,
+                    "tile_kernel"_a = std::string(e.tile_kernel >= 0 && e.tile_kernel <= 4 ? kernels[e.tile_kernel] : "?"));
+}
+
 }  // namespace
 
 PYBIND11_MODULE(_pylqcs, m) {
@@ -343,16 +361,31 @@ PYBIND11_MODULE(_pylqcs, m) {
         .def(py::init<const QuantumCircuit&, bool>(), "circuit"_a, "fuse_gates"_a = true)
         .def("apply",
              [](const CompiledCircuit& c, Statevector& sv, const std::string& plan, int tile_qubits,
-                int low_qubits) {
+                int low_qubits, const std::string& tile_kernel) {
                  EngineOptions e;
                  e.plan = plan_of(plan);
                  e.tile_qubits = tile_qubits;
                  e.low_qubits = low_qubits;
+                 e.tile_kernel = tile_kernel_of(tile_kernel);
                  e.max_qubits = LQCS_DEVICE_MAX_QUBITS;
                  py::gil_scoped_release nogil;
                  c.apply(sv, e);
              },
-             "statevector"_a, "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0)
+             "statevector"_a, "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0, "tile_kernel"_a = "default")
+        .def("autotune",
+             [](CompiledCircuit& c, Statevector& scratch, int reps) {
+                 CompiledCircuit::TuneResult r;
+                 {
+                     py::gil_scoped_release nogil;
+                     r = c.autotune(scratch, reps);
+                 }
+                 py::list trials;
+                 for (const auto& t : r.trials) trials.append(py::make_tuple(options_dict(t.options), t.ms));
+                 return py::dict("best"_a = options_dict(r.best), "best_ms"_a = r.best_ms, "trials"_a = trials);
+             },
+             "scratch"_a, "reps"_a = 2,
+             "time every candidate tile shape on scratch (overwritten); apply(plan='auto') then uses the fastest")
+        .def("tuned", [](const CompiledCircuit& c) { return options_dict(c.tuned()); })
         .def("num_qubits", &CompiledCircuit::num_qubits)
         .def("num_ops", &CompiledCircuit::num_ops)
         .def("describe",
@@ -386,6 +419,22 @@ PYBIND11_MODULE(_pylqcs, m) {
              },
              "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0, "batch"_a = 1,
              "cost-model features of every pass of the plan (rows x COST_FEATURES)");
+    py::class_<CircuitGraph>(m, "CircuitGraph")
+        .def(py::init([](const CompiledCircuit& c, Statevector& sv, const std::string& plan, int tile_qubits,
+                         int low_qubits, const std::string& tile_kernel) {
+                 EngineOptions e;
+                 e.plan = plan_of(plan);
+                 e.tile_qubits = tile_qubits;
+                 e.low_qubits = low_qubits;
+                 e.tile_kernel = tile_kernel_of(tile_kernel);
+                 return std::make_unique<CircuitGraph>(c, sv, e);
+             }),
+             "circuit"_a, "statevector"_a, "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0,
+             "tile_kernel"_a = "default", py::keep_alive<1, 3>())
+        .def("launch", [](CircuitGraph& g) {
+            py::gil_scoped_release nogil;
+            g.launch();
+        }, "replay the captured launches (sv <- U sv), queued on the engine stream");
     m.def("statevector", [](std::size_t n) { return Statevector(n, LQCS_DEVICE_MAX_QUBITS); }, "num_qubits"_a,
           "register of up to MAX_DEVICE_QUBITS qubits (beyond the 30-qubit reference cap)");
     m.def("synchronize", [] {

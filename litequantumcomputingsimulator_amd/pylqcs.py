@@ -20,6 +20,7 @@ device_info = _native.device_info
 last_plan_stats = _native.last_plan_stats
 MAX_DEVICE_QUBITS = _native.MAX_DEVICE_QUBITS
 CompiledCircuit = _native.CompiledCircuit
+CircuitGraph = _native.CircuitGraph
 statevector = _native.statevector
 synchronize = _native.synchronize
 profile_enable = _native.profile_enable
@@ -43,6 +44,7 @@ __all__ = [
     "statevector", "synchronize", "profile_enable", "profile_read", "set_tile_kernel", "ExpectationFunction",
     "expectation", "profile_read_launches", "COST_FEATURES", "mlp_num_params", "mlp_forward", "mlp_loss_grad",
     "mlp_train", "costmodel_install", "costmodel_uninstall", "costmodel_load", "last_plan_choice",
+    "CircuitGraph",
 ]
 
 # The packaged cost model (if trained) steers plan="learned".

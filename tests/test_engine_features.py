@@ -1,0 +1,75 @@
+"""Engine extensions around the hot path: the autotuner, captured launch
+sequences (HIP graphs) and the per-call tile-kernel choice. Results must be
+the same state as the plain greedy plan (device comparison, 1e-12)."""
+import numpy as np
+import pytest
+
+from litequantumcomputingsimulator_amd import workloads as W
+
+
+@pytest.fixture(scope="module")
+def lq():
+    from litequantumcomputingsimulator_amd import pylqcs
+
+    return pylqcs
+
+
+def _circuit(lq, n, depth=10, seed=3):
+    qc = lq.QuantumCircuit(n)
+    W.random_u3_cx_cp(qc, n, depth, seed)
+    return qc
+
+
+def test_tile_kernel_names_are_checked(lq):
+    qc = _circuit(lq, 8, 2)
+    comp = lq.CompiledCircuit(qc, True)
+    with pytest.raises(ValueError):
+        comp.describe(plan="warp")
+    with pytest.raises(ValueError):
+        lq.set_tile_kernel("warp")
+
+
+@pytest.mark.gpu
+def test_autotuner_picks_the_fastest_candidate_and_keeps_the_state(lq):
+    n = 24
+    comp = lq.CompiledCircuit(_circuit(lq, n, 12, 5), True)
+    scratch = lq.statevector(n)
+    r = comp.autotune(scratch, reps=2)
+    assert len(r["trials"]) == 9
+    assert r["best_ms"] == min(ms for _, ms in r["trials"])
+    assert comp.tuned() == r["best"]
+    want, got = lq.statevector(n), lq.statevector(n)
+    comp.apply(want, plan="greedy", tile_qubits=12, low_qubits=4)
+    comp.apply(got)  # plan="auto": the tuned options
+    assert got.max_abs_diff(want) <= 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel", ["register", "lds"])
+def test_captured_graph_replays_match_eager_applies(lq, kernel):
+    """A circuit's launches captured once and replayed three times, with the
+    engine's own eager work and synchronisation in between; every replay
+    agrees with the eager result."""
+    n = 20
+    comp = lq.CompiledCircuit(_circuit(lq, n, 8, 7), True)
+    a, b = lq.statevector(n), lq.statevector(n)
+    g = lq.CircuitGraph(comp, a, plan="greedy", tile_qubits=12, low_qubits=4, tile_kernel=kernel)
+    for k in range(3):
+        g.launch()
+        comp.apply(b, plan="greedy", tile_qubits=12, low_qubits=4, tile_kernel=kernel)
+        lq.synchronize()
+        assert abs(b.norm() - 1.0) <= 1e-10  # eager reduction between replays
+        assert a.max_abs_diff(b) <= 1e-12, k
+    del g
+
+
+@pytest.mark.gpu
+def test_per_call_tile_kernels_agree(lq):
+    n = 22
+    comp = lq.CompiledCircuit(_circuit(lq, n, 10, 9), True)
+    ref = lq.statevector(n)
+    comp.apply(ref, plan="greedy", tile_qubits=12, low_qubits=4, tile_kernel="register")
+    for kernel in ("lds", "stream", "land"):
+        sv = lq.statevector(n)
+        comp.apply(sv, plan="greedy", tile_qubits=12, low_qubits=4, tile_kernel=kernel)
+        assert sv.max_abs_diff(ref) <= 1e-12, kernel
