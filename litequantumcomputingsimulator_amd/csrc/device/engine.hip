@@ -474,18 +474,16 @@ StagedPlan stage_plan(const Plan& plan, uint32_t nq, uint32_t L) {
     StagedPlan sp;
     for (const auto& st : plan.steps) {
         if (!st.tiled) {
-            sp.where.push_back({0, 0, 0, 0, false});
+            sp.where.push_back({0, 0, 0, 0});
             continue;
         }
         RTileProgram rp;
         if (g_use_rtile && build_rtile(st.tile_ops, L, st.smask, nq, rp)) {
-            bool restage = false;
-            for (size_t p = 1; p < rp.phases.size(); ++p) restage |= rp.phases[p].seg_words != 0;
-            sp.where.push_back({2, sp.rph.size(), rp.phases.size(), sp.rwords.size(), restage});
+            sp.where.push_back({2, sp.rph.size(), rp.phases.size(), sp.rwords.size()});
             sp.rph.insert(sp.rph.end(), rp.phases.begin(), rp.phases.end());
             sp.rwords.insert(sp.rwords.end(), rp.words.begin(), rp.words.end());
         } else {
-            sp.where.push_back({1, sp.flat.size(), st.tile_ops.size(), 0, false});
+            sp.where.push_back({1, sp.flat.size(), st.tile_ops.size(), 0});
             sp.flat.insert(sp.flat.end(), st.tile_ops.begin(), st.tile_ops.end());
         }
     }
@@ -506,7 +504,7 @@ void launch_staged(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan, const st
         const StagedPlan::Where& w = sp.where[i];
         if (w.kind == 2) {
             launch_rtile(s, nq, st.smask, d_ph + w.first, static_cast<uint32_t>(w.count),
-                         d_rwords ? d_rwords + w.words : nullptr, w.restage, batch);
+                         d_rwords ? d_rwords + w.words : nullptr, batch);
         } else if (w.kind == 1) {
             launch_tile_pass(s, nq, L, st.smask, d_ops + w.first, static_cast<uint32_t>(w.count), batch);
         } else {

@@ -61,7 +61,6 @@ struct StagedPlan {
     struct Where {
         int kind;  // 0 stand-alone, 1 LDS tile, 2 register tile
         size_t first, count, words;
-        bool restage;
     };
     std::vector<TileOp> flat;
     std::vector<RPhase> rph;

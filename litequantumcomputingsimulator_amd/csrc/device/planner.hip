@@ -137,10 +137,8 @@ void step_features(const PlanStep& st, const std::vector<DecodedOp>& ops, uint32
         RTileProgram rp;
         if (rtile_enabled() && build_rtile(st.tile_ops, L, st.smask, nq, rp)) {
             f[3] = double(rp.phases.size());
-            for (const RPhase& ph : rp.phases) {
-                f[4] += ph.transpose;
-                f[5] += ph.seg_words ? 1.0 : 0.0;
-            }
+            for (const RPhase& ph : rp.phases) f[4] += ph.transpose;
+            f[5] = std::ceil(double(rp.words.size()) / 512.0);  // op words, in 8 KiB units
             for (size_t i = 0; i + 1 < rp.words.size();) {
                 RegOpHdr h;
                 std::memcpy(&h, &rp.words[i], sizeof h);

@@ -6,8 +6,8 @@
 // layout. A gate whose non-diagonal support lies in the register bits runs
 // entirely inside each thread: no LDS traffic, no barrier. The pass is split
 // into phases; between phases the tile is transposed through LDS into the
-// next phase's layout, half a tile at a time (32 KiB of LDS), so that with
-// 8 KiB of staged op words three workgroups fit on a CU.
+// next phase's layout, half a tile at a time (32 KiB of LDS), so that three
+// workgroups fit on a CU.
 //
 // Global load and store use the first and last phases' layouts. Their
 // register bits avoid the tile's contiguous low qubits, so that the lanes of
@@ -15,9 +15,9 @@
 // high qubits touches a separate cache line (and page) per lane and runs the
 // pass several times slower.
 //
-// Op records (a 32-byte header plus matrix entries) are staged per segment
-// into a second LDS array and read from there with uniform addresses, so an
-// op costs one LDS round trip instead of a chain of scalar-cache misses.
+// Op records (a 32-byte header plus matrix entries) are read from global
+// memory with scalar loads: every lane uses the same address, so they land
+// in scalar registers, where the FMAs take the matrix entries as operands.
 #include "common.hpp"
 #include "engine.hpp"
 #include "tile.hpp"
@@ -30,6 +30,7 @@ namespace lqcs_dev {
 bool g_rtile_stream = false;
 // Landing-buffer register tiles (LDS DMA prefetch of the next tile).
 bool g_rtile_land = false;
+
 uint32_t g_rtile_blocks_per_cu = 2;
 
 namespace {
@@ -37,7 +38,6 @@ namespace {
 constexpr int kRL = 12;   // tile bits
 constexpr int kRT = 256;  // threads
 constexpr int kRR = 16;   // registers per thread
-constexpr uint32_t kStage = kRTileWords / kRT;  // op words each thread stages
 
 __device__ __forceinline__ uint64_t pdep64(uint64_t v, uint64_t mask) {
     uint64_t r = 0;
@@ -61,7 +61,7 @@ __device__ __forceinline__ uint32_t pdep32(uint32_t v, uint32_t mask) {
 
 using Regs = amp_t[kRR];
 
-// An op's uniform fields, decoded from its staged header.
+// An op's uniform fields, decoded from its header.
 struct Hdr {
     uint32_t kind, b0, b1, xr, rctl, tctl, la, lb, gctrl, ga, gb;
 };
@@ -76,8 +76,8 @@ __device__ __forceinline__ double rfl(double v) {
     return __longlong_as_double(static_cast<long long>(lo | (hi << 32)));
 }
 
-// A matrix entry read from the staged op words (same address in every lane)
-// and moved to scalar registers, where the FMAs take it as an operand.
+// A matrix entry of the op words (same address in every lane): a scalar
+// load, kept in scalar registers (readfirstlane folds away on them).
 __device__ __forceinline__ amp_t uni(const amp_t* p) {
     const amp_t v = *p;
     return make_double2(rfl(v.x), rfl(v.y));
@@ -315,8 +315,8 @@ __device__ __forceinline__ void apply_reg_op(const Regs& a, Regs& o, const Hdr& 
     }
 }
 
-// Applies the staged op at word `off` and returns the next op's word.
-__device__ __forceinline__ uint32_t apply_staged(const Regs& a, Regs& o, const amp_t* words, uint32_t off,
+// Applies the op at word `off` of the pass's words; returns the next op's word.
+__device__ __forceinline__ uint32_t apply_op(const Regs& a, Regs& o, const amp_t* words, uint32_t off,
                                                  uint32_t jt, const RPhase& ph, uint32_t tile) {
     const uint4* hp = reinterpret_cast<const uint4*>(words + off);
     const uint4 h0 = hp[0], h1 = hp[1];
@@ -341,20 +341,11 @@ __device__ __forceinline__ uint32_t apply_staged(const Regs& a, Regs& o, const a
 // maxima, i.e. not waited for).
 __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0xF70); }
 
-// Stages op words [off, off + n) of the pass into LDS (all threads).
-__device__ __forceinline__ void stage_words(amp_t* staged, const amp_t* words, uint32_t off, uint32_t n) {
-#pragma unroll
-    for (uint32_t k = 0; k < kStage; ++k) {
-        const uint32_t i = threadIdx.x + k * kRT;
-        if (i < n) staged[i] = words[off + i];
-    }
-}
-
 // Runs every phase of the pass on the tile held in v (phase 0's layout on
-// entry, the last phase's on exit). Segment 0 is already staged.
-__device__ __forceinline__ void run_phases(Regs& v, Regs& w, amp_t* lds, amp_t* staged,
-                                           const RPhase* __restrict__ phases, uint32_t n_phases,
-                                           const amp_t* __restrict__ words, uint32_t tile, bool& lds_used) {
+// entry, the last phase's on exit).
+__device__ __forceinline__ void run_phases(Regs& v, Regs& w, amp_t* lds, const RPhase* __restrict__ phases,
+                                           uint32_t n_phases, const amp_t* __restrict__ words, uint32_t tile,
+                                           bool& lds_used) {
     uint32_t jt_prev = 0;
     for (uint32_t p = 0; p < n_phases; ++p) {
         const RPhase& ph = phases[p];
@@ -383,19 +374,14 @@ __device__ __forceinline__ void run_phases(Regs& v, Regs& w, amp_t* lds, amp_t* 
             copy_regs(w, v);
         }
         jt_prev = jt;
-        if (p > 0 && ph.seg_words) {
-            __syncthreads();  // every thread is done with the staged op words
-            stage_words(staged, words, ph.seg_off, ph.seg_words);
-            __syncthreads();
-        }
         uint32_t off = ph.op_off;
         uint32_t i = 0;
         for (; i + 1 < ph.n_ops; i += 2) {
-            off = apply_staged(v, w, staged, off, jt, ph, tile);
-            off = apply_staged(w, v, staged, off, jt, ph, tile);
+            off = apply_op(v, w, words, off, jt, ph, tile);
+            off = apply_op(w, v, words, off, jt, ph, tile);
         }
         if (i < ph.n_ops) {
-            apply_staged(v, w, staged, off, jt, ph, tile);
+            apply_op(v, w, words, off, jt, ph, tile);
             copy_regs(w, v);
         }
     }
@@ -406,33 +392,18 @@ __global__ void __launch_bounds__(kRT) k_rtile(amp_t* __restrict__ s, uint32_t n
                                               const RPhase* __restrict__ phases, uint32_t n_phases,
                                               const amp_t* __restrict__ words) {
     __shared__ amp_t lds[1 << (kRL - 1)];
-    __shared__ amp_t staged[kRTileWords];
     amp_t* b = s + (uint64_t(blockIdx.y) << nq);
     const uint32_t tile = blockIdx.x;
     const uint64_t base = pdep64(tile, ((uint64_t{1} << nq) - 1) & ~smask);
     Regs v, w;
     {
-        // Op words of the first segment, then the tile; the words go to LDS
-        // while the tile loads are still in flight.
         const RPhase& p0 = phases[0];
-        amp_t stg[kStage];
-#pragma unroll
-        for (uint32_t k = 0; k < kStage; ++k) {
-            const uint32_t i = threadIdx.x + k * kRT;
-            stg[k] = i < p0.seg_words ? words[p0.seg_off + i] : make_double2(0.0, 0.0);
-        }
         const uint64_t gt = base | pdep64(threadIdx.x, p0.tmask_global);
 #pragma unroll
         for (int r = 0; r < kRR; ++r) v[r] = b[gt | p0.gofs[r]];
-#pragma unroll
-        for (uint32_t k = 0; k < kStage; ++k) {
-            const uint32_t i = threadIdx.x + k * kRT;
-            if (i < p0.seg_words) staged[i] = stg[k];
-        }
-        __syncthreads();
     }
     bool lds_used = false;
-    run_phases(v, w, lds, staged, phases, n_phases, words, tile, lds_used);
+    run_phases(v, w, lds, phases, n_phases, words, tile, lds_used);
     {
         const RPhase& pl = phases[n_phases - 1];
         const uint64_t gt = base | pdep64(threadIdx.x, pl.tmask_global);
@@ -444,14 +415,10 @@ __global__ void __launch_bounds__(kRT) k_rtile(amp_t* __restrict__ s, uint32_t n
 // Streaming form: each workgroup walks tiles blockIdx.x, + gridDim.x, ...
 // and issues the global loads of its next tile before running the phases
 // of the current one, so HBM reads overlap the in-register gate work.
-// `restage`: the pass has more than one op-word segment, so segment 0 is
-// staged again for every tile.
 __global__ void __launch_bounds__(kRT) k_rtile_stream(amp_t* __restrict__ s, uint32_t nq, uint64_t smask,
                                                      const RPhase* __restrict__ phases, uint32_t n_phases,
-                                                     const amp_t* __restrict__ words, uint32_t tiles,
-                                                     uint32_t restage) {
+                                                     const amp_t* __restrict__ words, uint32_t tiles) {
     __shared__ amp_t lds[1 << (kRL - 1)];
-    __shared__ amp_t staged[kRTileWords];
     amp_t* b = s + (uint64_t(blockIdx.y) << nq);
     const uint64_t outside = ((uint64_t{1} << nq) - 1) & ~smask;
     const RPhase& p0 = phases[0];
@@ -460,13 +427,11 @@ __global__ void __launch_bounds__(kRT) k_rtile_stream(amp_t* __restrict__ s, uin
     const uint64_t gtl = pdep64(threadIdx.x, pl.tmask_global);
     uint32_t tile = blockIdx.x;
     Regs v, w, nx;
-    stage_words(staged, words, p0.seg_off, p0.seg_words);
     {
         const uint64_t gt = pdep64(tile, outside) | gt0;
 #pragma unroll
         for (int r = 0; r < kRR; ++r) v[r] = b[gt | p0.gofs[r]];
     }
-    __syncthreads();
     // Explicit waits for the vector-memory counter (vmcnt 0) keep the
     // compiler's own waits out of the phases: without them it cannot tell
     // the next tile's loads from the current tile's and waits for both
@@ -481,7 +446,7 @@ __global__ void __launch_bounds__(kRT) k_rtile_stream(amp_t* __restrict__ s, uin
 #pragma unroll
             for (int r = 0; r < kRR; ++r) nx[r] = b[gt | p0.gofs[r]];
         }
-        run_phases(v, w, lds, staged, phases, n_phases, words, tile, lds_used);
+        run_phases(v, w, lds, phases, n_phases, words, tile, lds_used);
         wait_vmem();  // the next tile has landed
         {
             const uint64_t gt = base | gtl;
@@ -489,11 +454,6 @@ __global__ void __launch_bounds__(kRT) k_rtile_stream(amp_t* __restrict__ s, uin
             for (int r = 0; r < kRR; ++r) b[gt | pl.gofs[r]] = v[r];
         }
         if (next >= tiles) break;
-        if (restage) {
-            __syncthreads();  // every thread is done with the last segment
-            stage_words(staged, words, p0.seg_off, p0.seg_words);
-            __syncthreads();
-        }
         copy_regs(nx, v);
         tile = next;
     }
@@ -521,11 +481,9 @@ __device__ __forceinline__ void wait_vmem_but_16() { __builtin_amdgcn_s_waitcnt(
 // need no registers and no particular layout.
 __global__ void __launch_bounds__(kRT) k_rtile_land(amp_t* __restrict__ s, uint32_t nq, uint64_t smask,
                                                    const RPhase* __restrict__ phases, uint32_t n_phases,
-                                                   const amp_t* __restrict__ words, uint32_t tiles,
-                                                   uint32_t restage) {
+                                                   const amp_t* __restrict__ words, uint32_t tiles) {
     __shared__ amp_t land[1 << kRL];
     __shared__ amp_t lds[1 << (kRL - 1)];
-    __shared__ amp_t staged[kRTileWords];
     amp_t* b = s + (uint64_t(blockIdx.y) << nq);
     const uint64_t outside = ((uint64_t{1} << nq) - 1) & ~smask;
     // Tile index j = 256 k + thread: the thread's part over the 8 lowest
@@ -549,7 +507,6 @@ __global__ void __launch_bounds__(kRT) k_rtile_land(amp_t* __restrict__ s, uint3
     const uint32_t jt0 = pdep32(threadIdx.x, p0.tmask);
     const uint64_t gtl = pdep64(threadIdx.x, pl.tmask_global);
     uint32_t tile = blockIdx.x;
-    stage_words(staged, words, p0.seg_off, p0.seg_words);
     issue(tile);
     wait_vmem();
     __syncthreads();
@@ -561,19 +518,14 @@ __global__ void __launch_bounds__(kRT) k_rtile_land(amp_t* __restrict__ s, uint3
         __syncthreads();  // every thread has read the landing buffer
         const uint32_t next = tile + gridDim.x;
         if (next < tiles) issue(next);
-        run_phases(v, w, lds, staged, phases, n_phases, words, tile, lds_used);
+        run_phases(v, w, lds, phases, n_phases, words, tile, lds_used);
         {
             const uint64_t gt = pdep64(tile, outside) | gtl;
 #pragma unroll
             for (int r = 0; r < kRR; ++r) b[gt | pl.gofs[r]] = v[r];
         }
         if (next >= tiles) break;
-        if (restage) {
-            __syncthreads();  // every thread is done with the last segment
-            stage_words(staged, words, p0.seg_off, p0.seg_words);
-        }
         wait_vmem_but_16();  // the next tile's DMA (older than the 16 stores) has landed
-        if (restage) wait_vmem();
         __syncthreads();
         tile = next;
     }
@@ -721,8 +673,6 @@ bool build_rtile(const std::vector<TileOp>& tops, uint32_t L, uint64_t smask, ui
     }
 
     // Encode: per phase its layout, per op a header and its matrix words.
-    // A phase whose words exceed the staging area continues in further
-    // phases of the same layout (no transpose).
     auto put_hdr = [&](const RegOpHdr& h) {
         amp_t w[2];
         std::memcpy(w, &h, sizeof h);
@@ -774,7 +724,7 @@ bool build_rtile(const std::vector<TileOp>& tops, uint32_t L, uint64_t smask, ui
                 if (rb[i] == static_cast<int>(pos)) return static_cast<uint32_t>(i);
             return 0u;
         };
-        ph.op_off = static_cast<uint32_t>(out.words.size());  // absolute until segmented
+        ph.op_off = static_cast<uint32_t>(out.words.size());
         for (uint32_t i : g.ops) {
             const TileOp& t = tops[i];
             RegOpHdr h;
@@ -811,51 +761,27 @@ bool build_rtile(const std::vector<TileOp>& tops, uint32_t L, uint64_t smask, ui
             }
             const uint32_t nm = matrix_words(kind);
             h.code = kind | (b0 << 4) | (b1 << 6) | (xr << 8) | (rctl << 12) | (nm << 16);
-            if (out.words.size() + 2 + nm - ph.op_off > kRTileWords) {  // continue in a new phase
-                out.phases.push_back(ph);
-                ph.transpose = 0;
-                ph.op_off = static_cast<uint32_t>(out.words.size());
-                ph.n_ops = 0;
-            }
             put_hdr(h);
             for (uint32_t e = 0; e < nm; ++e) out.words.push_back(t.m[e]);
             ++ph.n_ops;
         }
         out.phases.push_back(ph);
     }
-    // Segments: runs of consecutive phases whose words fit the staging area.
-    const uint32_t total = static_cast<uint32_t>(out.words.size());
-    auto end_of = [&](size_t i) { return i + 1 < out.phases.size() ? out.phases[i + 1].op_off : total; };
-    size_t p = 0;
-    while (p < out.phases.size()) {
-        const uint32_t start = out.phases[p].op_off;
-        size_t q = p + 1;
-        while (q < out.phases.size() && end_of(q) - start <= kRTileWords) ++q;
-        const uint32_t end = end_of(q - 1);
-        for (size_t i = p; i < q; ++i) {
-            out.phases[i].op_off -= start;
-            out.phases[i].seg_off = 0;
-            out.phases[i].seg_words = 0;
-        }
-        out.phases[p].seg_off = start;
-        out.phases[p].seg_words = end - start;
-        p = q;
-    }
     return true;
 }
 
 void launch_rtile(amp_t* s, uint32_t nq, uint64_t smask, const RPhase* d_phases, uint32_t n_phases,
-                  const amp_t* d_words, bool restage, uint32_t batch) {
+                  const amp_t* d_words, uint32_t batch) {
     const uint64_t tiles = uint64_t{1} << (nq - kRL);
     if (g_rtile_land && tiles > uint64_t(num_cus())) {
         k_rtile_land<<<dim3(static_cast<unsigned>(num_cus()), batch), kRT, 0, stream()>>>(
-            s, nq, smask, d_phases, n_phases, d_words, static_cast<uint32_t>(tiles), restage ? 1u : 0u);
+            s, nq, smask, d_phases, n_phases, d_words, static_cast<uint32_t>(tiles));
         return;
     }
     const uint64_t resident = uint64_t(num_cus()) * g_rtile_blocks_per_cu;
     if (g_rtile_stream && tiles > resident) {
         k_rtile_stream<<<dim3(static_cast<unsigned>(resident), batch), kRT, 0, stream()>>>(
-            s, nq, smask, d_phases, n_phases, d_words, static_cast<uint32_t>(tiles), restage ? 1u : 0u);
+            s, nq, smask, d_phases, n_phases, d_words, static_cast<uint32_t>(tiles));
         return;
     }
     k_rtile<<<dim3(static_cast<unsigned>(tiles), batch), kRT, 0, stream()>>>(s, nq, smask, d_phases, n_phases,

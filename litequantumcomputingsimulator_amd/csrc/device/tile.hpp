@@ -31,10 +31,11 @@ struct TileOp {
 // Register-resident tile (rtile.hip): per-phase layout and in-register ops.
 enum RegKind : uint32_t { RK_DENSE1 = 0, RK_DENSE2 = 1, RK_X = 2, RK_DIAG = 3, RK_PAULI = 4 };
 
-// Header of one in-register op as staged in LDS: two 16-byte words, then
+// Header of one in-register op in a pass's op words: two 16-byte words, then
 // the op's matrix entries (4 for DENSE1 and DIAG, 16 for DENSE2, 2 for
 // PAULI, none for X). Masks named "tile number" are over the qubits outside
-// the tile, compressed to the tile's index.
+// the tile, compressed to the tile's index. The kernel reads them with
+// scalar loads (the same address in every lane).
 struct RegOpHdr {
     uint32_t code;    // kind | b0 << 4 | b1 << 6 | xr << 8 | rctl << 12 | matrix words << 16
     uint32_t tctl;    // controls among the thread's tile bits (tile-local mask)
@@ -43,15 +44,11 @@ struct RegOpHdr {
     uint32_t ga, gb;  // DIAG: tile-number diagonal bits; PAULI: ga = tile-number Z bits
     uint32_t pad;
 };
-static_assert(sizeof(RegOpHdr) == 2 * sizeof(amp_t), "op header is two staged words");
-
-constexpr uint32_t kRTileWords = 512;  // op words staged per workgroup (8 KiB of LDS)
+static_assert(sizeof(RegOpHdr) == 2 * sizeof(amp_t), "op header is two 16-byte words");
 
 struct RPhase {
     uint32_t n_ops;
-    uint32_t op_off;         // first op word, relative to the staged segment
-    uint32_t seg_off;        // words [seg_off, seg_off + seg_words) of the pass are
-    uint32_t seg_words;      // staged at this phase's start (0: keep the staged ones)
+    uint32_t op_off;         // first op word within the pass's words
     uint32_t tmask;          // tile bits indexed by the thread id (ascending)
     uint32_t transpose;      // 1: the tile is transposed into this layout first
     uint64_t tmask_global;   // their global qubit positions
@@ -83,9 +80,8 @@ struct RTileProgram {
 // Splits a tiled step into register phases; false when some op needs more
 // than 4 register bits or the tile width is not the register tile's.
 bool build_rtile(const std::vector<TileOp>& tops, uint32_t L, uint64_t smask, uint32_t nq, RTileProgram& out);
-// `restage`: some phase after the first stages another op-word segment.
 void launch_rtile(amp_t* s, uint32_t nq, uint64_t smask, const RPhase* d_phases, uint32_t n_phases,
-                  const amp_t* d_words, bool restage, uint32_t batch = 1);
+                  const amp_t* d_words, uint32_t batch = 1);
 
 void launch_tile_pass(amp_t* s, uint32_t nq, uint32_t L, uint64_t smask, const TileOp* d_ops,
                       uint32_t n_ops, uint32_t batch = 1);

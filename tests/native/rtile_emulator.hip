@@ -1,10 +1,11 @@
 // CPU check of the register-tile program builder (build_rtile).
 //
 // Random tile-op sequences are turned into register programs; the program
-// is then executed on the host exactly as k_rtile reads it (staged op words,
-// segment reloads, per-phase layouts, header decoding) and compared with the
-// tile ops applied directly in tile-index space. Also checks the global
-// load/store offsets of the first and last layouts and the segment bounds.
+// is then executed on the host exactly as k_rtile reads it (op words,
+// per-phase layouts, the LDS slots of both half-tile transpose rounds,
+// header decoding) and compared with the tile ops applied directly in
+// tile-index space. Also checks the global load/store offsets of the first
+// and last layouts.
 // Exit status 0 and a line "ok <cases>" when every case agrees.
 #include "tile.hpp"
 
@@ -94,7 +95,7 @@ int main(int argc, char** argv) {
     const int cases = argc > 1 ? std::atoi(argv[1]) : 400;
     std::mt19937_64 rng(20240611);
     std::uniform_real_distribution<double> U(-1, 1);
-    int total_phases = 0, total_segments = 0, total_splits = 0;
+    int total_phases = 0, total_transposes = 0;
     for (int c = 0; c < cases; ++c) {
         const uint32_t nq = 12 + rng() % 9;  // 12..20
         // Tile set: `low` contiguous low qubits plus random others.
@@ -110,7 +111,7 @@ int main(int argc, char** argv) {
             do b = uint64_t{1} << (rng() % nq); while (!(b & outside));
             return b;
         };
-        // Ops: long sequences now and then, so segments and phase splits occur.
+        // Ops: long sequences now and then (long phases, many op words).
         const int n_ops = (c % 10 == 9) ? 300 + static_cast<int>(rng() % 200) : 1 + static_cast<int>(rng() % 60);
         // Every tenth case keeps its dense ops on four tile bits, so one
         // layout collects more op words than one staging segment holds.
@@ -189,13 +190,13 @@ int main(int argc, char** argv) {
         std::vector<cd> init(kN);
         for (uint32_t j = 0; j < kN; ++j) init[j] = cd(U(rng), U(rng));
         const char* why = nullptr;
-        int n_ph = 0, n_seg = 0, n_split = 0;
+        int n_ph = 0, n_tr = 0;
         auto check = [&](const std::vector<TileOp>& ops) -> double {
             RTileProgram prog;
             if (!lqcs_dev::build_rtile(ops, kL, smask, nq, prog)) { why = "build_rtile refused"; return -1; }
             if (prog.phases.empty()) { why = "no phases"; return -1; }
             n_ph = static_cast<int>(prog.phases.size());
-            n_seg = n_split = 0;
+            n_tr = 0;
             std::vector<cd> want = init, a = init;
             for (const TileOp& t : ops) apply_direct(want, t, base);
             for (const RPhase* ph : {&prog.phases.front(), &prog.phases.back()}) {
@@ -208,14 +209,13 @@ int main(int argc, char** argv) {
                 const uint32_t avoid = (1u << std::min<uint32_t>(run, 6)) - 1;
                 if ((~ph->tmask & 0xfffu) & avoid) { why = "edge layout keeps low qubits in registers"; return -1; }
             }
-            std::vector<amp_t> staged(lqcs_dev::kRTileWords);
-            uint32_t seg_words = 0;
             size_t seen_ops = 0;
             for (size_t p = 0; p < prog.phases.size(); ++p) {
                 const RPhase& ph = prog.phases[p];
                 if (__builtin_popcount(ph.tmask) != 8 || (ph.tmask & ~0xfffu)) { why = "tmask"; return -1; }
                 if (p > 0 && !ph.transpose && ph.tmask != prog.phases[p - 1].tmask) { why = "layout change without transpose"; return -1; }
                 if (p > 0 && ph.transpose) {
+                    ++n_tr;
                     // The two half-tile rounds through LDS, slot by slot: every
                     // slot written once per round, and every read finds the
                     // amplitude of the tile index it stands for.
@@ -243,22 +243,16 @@ int main(int argc, char** argv) {
                         }
                     }
                 }
-                if (p == 0 || ph.seg_words) {
-                    if (ph.seg_words > lqcs_dev::kRTileWords || ph.seg_off + ph.seg_words > prog.words.size()) { why = "segment bounds"; return -1; }
-                    for (uint32_t i = 0; i < ph.seg_words; ++i) staged[i] = prog.words[ph.seg_off + i];
-                    seg_words = ph.seg_words;
-                    ++n_seg;
-                }
-                if (p > 0 && !ph.transpose) ++n_split;
                 uint32_t off = ph.op_off;
+                const size_t total = prog.words.size();
                 for (uint32_t k = 0; k < ph.n_ops; ++k) {
-                    if (off + 2 > seg_words) { why = "header outside segment"; return -1; }
+                    if (off + 2 > total) { why = "header outside the op words"; return -1; }
                     RegOpHdr h;
-                    std::memcpy(&h, &staged[off], sizeof h);
+                    std::memcpy(&h, &prog.words[off], sizeof h);
                     const uint32_t kind = h.code & 15u, b0 = (h.code >> 4) & 3u, b1 = (h.code >> 6) & 3u;
                     const uint32_t xr = (h.code >> 8) & 15u, rctl = (h.code >> 12) & 15u, nm = h.code >> 16;
-                    if (off + 2 + nm > seg_words) { why = "matrix outside segment"; return -1; }
-                    const amp_t* m = &staged[off + 2];
+                    if (off + 2 + nm > total) { why = "matrix outside the op words"; return -1; }
+                    const amp_t* m = &prog.words[off + 2];
                     std::vector<cd> o = a;
                     if ((tile & h.gctrl) == h.gctrl) {
                         for (uint32_t t = 0; t < 256; ++t) {
@@ -326,11 +320,10 @@ int main(int argc, char** argv) {
         };
         const double err = check(ops);
         total_phases += n_ph;
-        total_segments += n_seg;
-        total_splits += n_split;
+        total_transposes += n_tr;
         if (!(err >= 0 && err <= 1e-11)) {
-            std::printf("FAIL case %d: %s error %g (nq=%u smask=%llx ops=%zu phases=%d segments=%d splits=%d)\n", c,
-                        why ? why : "values", err, nq, static_cast<unsigned long long>(smask), ops.size(), n_ph, n_seg, n_split);
+            std::printf("FAIL case %d: %s error %g (nq=%u smask=%llx ops=%zu phases=%d transposes=%d)\n", c,
+                        why ? why : "values", err, nq, static_cast<unsigned long long>(smask), ops.size(), n_ph, n_tr);
             size_t lo = 1, hi = ops.size();  // smallest failing prefix
             while (lo < hi) {
                 const size_t mid = (lo + hi) / 2;
@@ -338,11 +331,11 @@ int main(int argc, char** argv) {
                 if (e >= 0 && e <= 1e-11) lo = mid + 1; else hi = mid;
             }
             const double e = check(std::vector<TileOp>(ops.begin(), ops.begin() + lo));
-            std::printf("  smallest failing prefix %zu ops: %s error %g phases=%d segments=%d splits=%d\n", lo,
-                        why ? why : "values", e, n_ph, n_seg, n_split);
+            std::printf("  smallest failing prefix %zu ops: %s error %g phases=%d transposes=%d\n", lo,
+                        why ? why : "values", e, n_ph, n_tr);
             return 1;
         }
     }
-    std::printf("ok %d phases=%d segments=%d splits=%d\n", cases, total_phases, total_segments, total_splits);
+    std::printf("ok %d phases=%d transposes=%d\n", cases, total_phases, total_transposes);
     return 0;
 }
