@@ -14,8 +14,9 @@ covers the metric's other half: batched adjoint gradients of 256 Pauli-
 rotation ansatze (16-24 qubits, 400 rotations, 200-term Hamiltonians) against
 parameter-shift gradients on the same GPU (timed on a bounded sample). Two
 more objects report the noisy-trajectory workload (24 qubits, 2048 shots,
-trajectories/s) and the learned planner on the headline workload (ms per step
-of the cost model's plan next to greedy's).
+trajectories/s) and the learned planner (ms per step of the cost model's plan
+next to greedy's, on the headline workload and on random U3/CX/CP layers of
+depth 40 at 32 qubits, with the two final states compared).
 
 `traffic` (HBM bytes per tiled pass) comes from the rocprofv3 counter summary
 committed under profiles/ for the same kernel and workload, when one exists.
@@ -177,6 +178,37 @@ def trajectories_bench(lq, W, seed: int, n: int, shots: int) -> dict:
             "trajectory_gates_per_s": round(shots * gates / dt, 1), "distinct_outcomes": len(dict(counts))}
 
 
+def planner_32q(lq, W, seed: int, n: int = 32, depth: int = 40) -> dict:
+    """The learned-planner workload: random U3/CX/CP layers of depth 40 at 32
+    qubits (64 GiB per register); one apply with greedy grouping and one with
+    the learned plan on two registers, timed, and the two states compared on
+    the device."""
+    qc = lq.QuantumCircuit(n)
+    W.random_u3_cx_cp(qc, n, depth, seed)
+    comp = lq.CompiledCircuit(qc, True)
+    a = lq.statevector(n)
+    b = lq.statevector(n)
+    comp.apply(a, plan="greedy")  # untimed: stages both plans on the device
+    comp.apply(b, plan="learned")
+    a.set_basis(0)
+    b.set_basis(0)
+    lq.synchronize()
+    t0 = time.perf_counter()
+    comp.apply(a, plan="greedy")
+    lq.synchronize()
+    t_greedy = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    comp.apply(b, plan="learned")
+    lq.synchronize()
+    t_learned = time.perf_counter() - t0
+    choice = lq.last_plan_choice()
+    diff = a.max_abs_diff(b)
+    log(f"planner at {n} qubits: greedy {t_greedy:.3f} s, learned {t_learned:.3f} s ({choice}), max diff {diff:.2e}")
+    return {"workload": f"random U3/CX/CP layers, {n} qubits, depth {depth}, {qc.num_instructions()} gates",
+            "greedy_s": round(t_greedy, 3), "learned_s": round(t_learned, 3), "learned_pick": choice,
+            "learned_vs_greedy": round(t_learned / t_greedy, 4), "max_abs_diff_states": diff}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,6 +230,8 @@ def main() -> int:
     ap.add_argument("--no-trajectories", action="store_true")
     ap.add_argument("--no-planner", action="store_true",
                     help="skip timing the learned plan on the same workload")
+    ap.add_argument("--no-planner-32q", action="store_true",
+                    help="skip the learned-planner workload at 32 qubits (two 64 GiB registers)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with one device-to-device copy of the register (counter calibration)")
     args = ap.parse_args()
@@ -295,6 +329,8 @@ def main() -> int:
         adjoint = adjoint_bench(lq, W, args.seed, args.adjoint_circuits, args.ps_seconds)
     if not args.no_trajectories:
         trajectories = trajectories_bench(lq, W, args.seed, args.traj_qubits, args.traj_shots)
+    if planner is not None and not args.no_planner_32q:
+        planner["at_32_qubits"] = planner_32q(lq, W, args.seed)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(n, layers, args.seed, args.cpu_seconds)
 
