@@ -160,6 +160,12 @@ lqcs_status lqcs_plan_describe(uint32_t n_qubits, const lqcs_program* prog, cons
  * 0 = LDS tile. A switch for comparisons; results agree to rounding. */
 lqcs_status lqcs_set_tile_kernel(int kernel);
 
+/* Dense controlled unitaries (reference src/backend/kernels/gate_kernels.cpp:143-167)
+ * with k >= min_targets targets, k <= 6, run as complex GEMMs on FP64 MFMA;
+ * the others on the vector kernels. Default 4; 7 turns MFMA off. A switch
+ * for comparisons; results agree to rounding. */
+lqcs_status lqcs_set_dense_mfma_min_targets(uint32_t min_targets);
+
 /* Launch timing (no counterpart in the reference): while enabled, every
  * tiled pass and every gate-by-gate kernel of lqcs_state_apply is bracketed
  * by HIP events on the engine stream. lqcs_profile_read synchronises and

@@ -79,6 +79,9 @@ void launch_controlled_single(amp_t* s, uint32_t nq, uint32_t c, uint32_t t, con
 void launch_swap(amp_t* s, uint32_t nq, uint32_t a, uint32_t b, uint32_t batch = 1);
 void launch_two_qubit(amp_t* s, uint32_t nq, uint32_t q0, uint32_t q1, const Mat4& m,
                       uint32_t batch = 1);
+// Dense controlled unitaries with at least this many targets (4..6) run on
+// FP64 MFMA; larger values keep them on the vector kernels.
+extern uint32_t g_dense_mfma_min_targets;
 void launch_controlled_unitary(amp_t* s, uint32_t nq, const uint32_t* ctl, uint32_t nc,
                                const uint32_t* tgt, uint32_t nt, const amp_t* m_host,
                                uint32_t batch = 1);

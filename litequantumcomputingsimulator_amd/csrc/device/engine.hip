@@ -702,6 +702,13 @@ lqcs_status lqcs_set_tile_kernel(int kernel) {
     });
 }
 
+lqcs_status lqcs_set_dense_mfma_min_targets(uint32_t k) {
+    return guarded([&] {
+        if (k < 4) throw Error(LQCS_ERR_INVALID_ARGUMENT, "dense MFMA kernel needs at least 4 targets");
+        g_dense_mfma_min_targets = k;
+    });
+}
+
 lqcs_status lqcs_profile_enable(int enable) {
     return guarded([&] {
         ensure_ready();

@@ -15,9 +15,20 @@ namespace {
 
 constexpr int kItems = 2;
 
-__device__ __forceinline__ uint64_t group_index(int k) {
-    return (uint64_t(blockIdx.x) * kItems + k) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ uint64_t group_index(uint64_t blk, int k) {
+    return (blk * kItems + k) * kBlock + threadIdx.x;
 }
+// Grid-stride over blocks of kItems * kBlock groups: launches are capped at
+// kMaxGrid workgroups, so registers of any width stay within the launch limits.
+#define LQCS_FOR_BLOCKS(blk, groups) \
+    for (uint64_t blk = blockIdx.x; blk * (kItems * kBlock) < (groups); blk += gridDim.x)
+constexpr uint64_t kMaxGrid = uint64_t{1} << 22;
+inline unsigned capped(uint64_t blocks) { return static_cast<unsigned>(std::min<uint64_t>(blocks, kMaxGrid)); }
+inline unsigned grid_capped(uint64_t items, int per_thread = 1) { return capped(grid_for(items, per_thread)); }
+// Launch errors (a shape beyond the device limits, a missing code object)
+// surface at the launch, not at the next synchronisation.
+inline void launched() { LQCS_HIP(hipGetLastError()); }
+
 __device__ __forceinline__ amp_t* batch_base(amp_t* s, uint32_t nq) {
     return s + (uint64_t(blockIdx.y) << nq);
 }
@@ -27,20 +38,22 @@ __global__ void __launch_bounds__(kBlock) k_single(amp_t* __restrict__ s, uint32
                                                    Mat2 m) {
     amp_t* b = batch_base(s, nq);
     const uint64_t groups = uint64_t{1} << (nq - 1);
-    amp_t a0[kItems], a1[kItems];
-    uint64_t i0[kItems];
+    LQCS_FOR_BLOCKS(blk, groups) {
+        amp_t a0[kItems], a1[kItems];
+        uint64_t i0[kItems];
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t g = group_index(k);
-        i0[k] = insert_zero_bit(g < groups ? g : 0, t);
-        a0[k] = ld(b + i0[k]);
-        a1[k] = ld(b + (i0[k] | (uint64_t{1} << t)));
-    }
+        for (int k = 0; k < kItems; ++k) {
+            const uint64_t g = group_index(blk, k);
+            i0[k] = insert_zero_bit(g < groups ? g : 0, t);
+            a0[k] = ld(b + i0[k]);
+            a1[k] = ld(b + (i0[k] | (uint64_t{1} << t)));
+        }
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        if (group_index(k) >= groups) continue;
-        st(b + i0[k], cmac2(m.m[0], a0[k], m.m[1], a1[k]));
-        st(b + (i0[k] | (uint64_t{1} << t)), cmac2(m.m[2], a0[k], m.m[3], a1[k]));
+        for (int k = 0; k < kItems; ++k) {
+            if (group_index(blk, k) >= groups) continue;
+            st(b + i0[k], cmac2(m.m[0], a0[k], m.m[1], a1[k]));
+            st(b + (i0[k] | (uint64_t{1} << t)), cmac2(m.m[2], a0[k], m.m[3], a1[k]));
+        }
     }
 }
 
@@ -49,17 +62,19 @@ __global__ void __launch_bounds__(kBlock) k_diag_single(amp_t* __restrict__ s, u
                                                         uint32_t t, amp_t d0, amp_t d1) {
     amp_t* b = batch_base(s, nq);
     const uint64_t n = uint64_t{1} << nq;
-    amp_t a[kItems];
+    LQCS_FOR_BLOCKS(blk, n) {
+        amp_t a[kItems];
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t i = group_index(k);
-        a[k] = ld(b + (i < n ? i : 0));
-    }
+        for (int k = 0; k < kItems; ++k) {
+            const uint64_t i = group_index(blk, k);
+            a[k] = ld(b + (i < n ? i : 0));
+        }
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t i = group_index(k);
-        if (i >= n) continue;
-        st(b + i, cmul(a[k], ((i >> t) & 1) ? d1 : d0));
+        for (int k = 0; k < kItems; ++k) {
+            const uint64_t i = group_index(blk, k);
+            if (i >= n) continue;
+            st(b + i, cmul(a[k], ((i >> t) & 1) ? d1 : d0));
+        }
     }
 }
 
@@ -69,20 +84,22 @@ __global__ void __launch_bounds__(kBlock) k_controlled_single(amp_t* __restrict_
     amp_t* b = batch_base(s, nq);
     const unsigned lo = c < t ? c : t, hi = c < t ? t : c;
     const uint64_t groups = uint64_t{1} << (nq - 2);
-    amp_t a0[kItems], a1[kItems];
-    uint64_t i0[kItems];
+    LQCS_FOR_BLOCKS(blk, groups) {
+        amp_t a0[kItems], a1[kItems];
+        uint64_t i0[kItems];
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t g = group_index(k);
-        i0[k] = insert_two_zero_bits(g < groups ? g : 0, lo, hi) | (uint64_t{1} << c);
-        a0[k] = ld(b + i0[k]);
-        a1[k] = ld(b + (i0[k] | (uint64_t{1} << t)));
-    }
+        for (int k = 0; k < kItems; ++k) {
+            const uint64_t g = group_index(blk, k);
+            i0[k] = insert_two_zero_bits(g < groups ? g : 0, lo, hi) | (uint64_t{1} << c);
+            a0[k] = ld(b + i0[k]);
+            a1[k] = ld(b + (i0[k] | (uint64_t{1} << t)));
+        }
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        if (group_index(k) >= groups) continue;
-        st(b + i0[k], cmac2(m.m[0], a0[k], m.m[1], a1[k]));
-        st(b + (i0[k] | (uint64_t{1} << t)), cmac2(m.m[2], a0[k], m.m[3], a1[k]));
+        for (int k = 0; k < kItems; ++k) {
+            if (group_index(blk, k) >= groups) continue;
+            st(b + i0[k], cmac2(m.m[0], a0[k], m.m[1], a1[k]));
+            st(b + (i0[k] | (uint64_t{1} << t)), cmac2(m.m[2], a0[k], m.m[3], a1[k]));
+        }
     }
 }
 
@@ -92,20 +109,22 @@ __global__ void __launch_bounds__(kBlock) k_swap(amp_t* __restrict__ s, uint32_t
     amp_t* b = batch_base(s, nq);
     const unsigned lo = qa < qb ? qa : qb, hi = qa < qb ? qb : qa;
     const uint64_t groups = uint64_t{1} << (nq - 2);
-    amp_t x[kItems], y[kItems];
-    uint64_t base[kItems];
+    LQCS_FOR_BLOCKS(blk, groups) {
+        amp_t x[kItems], y[kItems];
+        uint64_t base[kItems];
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t g = group_index(k);
-        base[k] = insert_two_zero_bits(g < groups ? g : 0, lo, hi);
-        x[k] = ld(b + (base[k] | (uint64_t{1} << qa)));
-        y[k] = ld(b + (base[k] | (uint64_t{1} << qb)));
-    }
+        for (int k = 0; k < kItems; ++k) {
+            const uint64_t g = group_index(blk, k);
+            base[k] = insert_two_zero_bits(g < groups ? g : 0, lo, hi);
+            x[k] = ld(b + (base[k] | (uint64_t{1} << qa)));
+            y[k] = ld(b + (base[k] | (uint64_t{1} << qb)));
+        }
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        if (group_index(k) >= groups) continue;
-        st(b + (base[k] | (uint64_t{1} << qa)), y[k]);
-        st(b + (base[k] | (uint64_t{1} << qb)), x[k]);
+        for (int k = 0; k < kItems; ++k) {
+            if (group_index(blk, k) >= groups) continue;
+            st(b + (base[k] | (uint64_t{1} << qa)), y[k]);
+            st(b + (base[k] | (uint64_t{1} << qb)), x[k]);
+        }
     }
 }
 
@@ -115,19 +134,20 @@ __global__ void __launch_bounds__(kBlock) k_two_qubit(amp_t* __restrict__ s, uin
     amp_t* b = batch_base(s, nq);
     const unsigned lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
     const uint64_t groups = uint64_t{1} << (nq - 2);
-    const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (g >= groups) return;
-    const uint64_t base = insert_two_zero_bits(g, lo, hi);
-    const uint64_t idx[4] = {base, base | (uint64_t{1} << q0), base | (uint64_t{1} << q1),
-                             base | (uint64_t{1} << q0) | (uint64_t{1} << q1)};
-    amp_t a[4];
+    for (uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x; g < groups;
+         g += uint64_t(gridDim.x) * kBlock) {
+        const uint64_t base = insert_two_zero_bits(g, lo, hi);
+        const uint64_t idx[4] = {base, base | (uint64_t{1} << q0), base | (uint64_t{1} << q1),
+                                 base | (uint64_t{1} << q0) | (uint64_t{1} << q1)};
+        amp_t a[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) a[r] = ld(b + idx[r]);
+        for (int r = 0; r < 4; ++r) a[r] = ld(b + idx[r]);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        amp_t acc = cmac2(m.m[4 * r], a[0], m.m[4 * r + 1], a[1]);
-        acc = cadd(acc, cmac2(m.m[4 * r + 2], a[2], m.m[4 * r + 3], a[3]));
-        st(b + idx[r], acc);
+        for (int r = 0; r < 4; ++r) {
+            amp_t acc = cmac2(m.m[4 * r], a[0], m.m[4 * r + 1], a[1]);
+            acc = cadd(acc, cmac2(m.m[4 * r + 2], a[2], m.m[4 * r + 3], a[3]));
+            st(b + idx[r], acc);
+        }
     }
 }
 
@@ -156,18 +176,19 @@ __global__ void __launch_bounds__(kBlock) k_cu_small(amp_t* __restrict__ s, uint
     __syncthreads();
     amp_t* b = batch_base(s, nq);
     const uint64_t groups = uint64_t{1} << (nq - L.n_fixed);
-    const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (g >= groups) return;
-    const uint64_t base = expand(g, L);
-    amp_t a[D];
+    for (uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x; g < groups;
+         g += uint64_t(gridDim.x) * kBlock) {
+        const uint64_t base = expand(g, L);
+        amp_t a[D];
 #pragma unroll
-    for (int l = 0; l < D; ++l) a[l] = ld(b + (base | offs[l]));
+        for (int l = 0; l < D; ++l) a[l] = ld(b + (base | offs[l]));
 #pragma unroll
-    for (int r = 0; r < D; ++r) {
-        amp_t acc = make_double2(0.0, 0.0);
+        for (int r = 0; r < D; ++r) {
+            amp_t acc = make_double2(0.0, 0.0);
 #pragma unroll
-        for (int c = 0; c < D; ++c) acc = cadd(acc, cmul(sm[r * D + c], a[c]));
-        st(b + (base | offs[r]), acc);
+            for (int c = 0; c < D; ++c) acc = cadd(acc, cmul(sm[r * D + c], a[c]));
+            st(b + (base | offs[r]), acc);
+        }
     }
 }
 
@@ -178,19 +199,103 @@ __global__ void __launch_bounds__(kBlock) k_cu_large(amp_t* __restrict__ s, uint
     extern __shared__ amp_t grp[];
     const uint32_t D = 1u << L.nt;
     amp_t* b = batch_base(s, nq);
-    const uint64_t base = expand(blockIdx.x, L);
-    for (uint32_t l = threadIdx.x; l < D; l += blockDim.x) grp[l] = ld(b + (base | offs[l]));
-    __syncthreads();
-    amp_t out[LQCS_CU_LARGE_ROWS];
-    int k = 0;
-    for (uint32_t r = threadIdx.x; r < D; r += blockDim.x, ++k) {
-        amp_t acc = make_double2(0.0, 0.0);
-        for (uint32_t c = 0; c < D; ++c) acc = cadd(acc, cmul(mat[uint64_t(r) * D + c], grp[c]));
-        out[k] = acc;
+    const uint64_t groups = uint64_t{1} << (nq - L.n_fixed);
+    for (uint64_t g = blockIdx.x; g < groups; g += gridDim.x) {
+        const uint64_t base = expand(g, L);
+        for (uint32_t l = threadIdx.x; l < D; l += blockDim.x) grp[l] = ld(b + (base | offs[l]));
+        __syncthreads();
+        amp_t out[LQCS_CU_LARGE_ROWS];
+        int k = 0;
+        for (uint32_t r = threadIdx.x; r < D; r += blockDim.x, ++k) {
+            amp_t acc = make_double2(0.0, 0.0);
+            for (uint32_t c = 0; c < D; ++c) acc = cadd(acc, cmul(mat[uint64_t(r) * D + c], grp[c]));
+            out[k] = acc;
+        }
+        __syncthreads();  // every read of grp is done before the next group lands
+        k = 0;
+        for (uint32_t r = threadIdx.x; r < D; r += blockDim.x, ++k) st(b + (base | offs[r]), out[k]);
+    }
+}
+
+// :143-167 for dense blocks of k = 4..6 targets on FP64 MFMA.
+//
+// The op is a complex GEMM Y = U X: U is 2^k x 2^k and the columns of X are
+// the groups (one column = the 2^k amplitudes of one setting of the other
+// qubits, controls at 1). It runs as four real GEMMs, Yr = Ur Xr - Ui Xi and
+// Yi = Ui Xr + Ur Xi, on v_mfma_f64_16x16x4 tiles: 16 output rows x 16
+// columns, 4 rows of X per step. A wave owns S slabs of 16 columns; lane l
+// loads amplitude (4b + l/16) of column l%16 as one 16-byte (re, im) pair,
+// which is the B operand of the Xr and the Xi product of row block b. U sits
+// in LDS in fragment order: the A operand of output tile t and row block b
+// for lane l is U[16t + l%16][4b + l/16], so each lane's 16-byte read is
+// contiguous with its neighbours'. A tile's accumulators hold rows
+// l/16 + 4v, v = 0..3, of column l%16.
+struct DenseOffsets {
+    uint64_t blk[16];  // target offset of local index 4b, b < 2^k / 4
+    uint64_t lo[4];    // target offset of local index 0..3
+};
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4d mfma4(double a, double b, v4d c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <int K, int S>
+__global__ void __launch_bounds__(kBlock) k_cu_mfma(amp_t* __restrict__ s, uint32_t nq, Layout L,
+                                                    DenseOffsets o, const amp_t* __restrict__ mat,
+                                                    uint64_t iters) {
+    constexpr int D = 1 << K, NB = D / 4, NT = D / 16;
+    __shared__ amp_t frag[D * D];
+    for (int e = threadIdx.x; e < D * D; e += kBlock) {
+        const int lane = e & 63, tb = e >> 6, t = tb / NB, b = tb % NB;
+        frag[e] = mat[(16 * t + (lane & 15)) * D + 4 * b + (lane >> 4)];
     }
     __syncthreads();
-    k = 0;
-    for (uint32_t r = threadIdx.x; r < D; r += blockDim.x, ++k) st(b + (base | offs[r]), out[k]);
+    amp_t* base_ptr = batch_base(s, nq);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t ld_lane = o.lo[lane >> 4];
+    for (uint64_t it = uint64_t(blockIdx.x) * (kBlock / 64) + wave; it < iters;
+         it += uint64_t(gridDim.x) * (kBlock / 64)) {
+        uint64_t col[S];
+        amp_t x[S][NB];
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+            col[q] = expand((it * S + q) * 16 + (lane & 15), L);
+#pragma unroll
+            for (int b = 0; b < NB; ++b) x[q][b] = ld(base_ptr + (col[q] | o.blk[b] | ld_lane));
+        }
+        v4d yr[S][NT], yi[S][NT];
+        // U fits in registers up to k = 5 (the compiler hoists its LDS reads
+        // out of the loop); at k = 6 it is re-read from LDS every iteration.
+        int zero = 0;
+        if (K == 6) asm volatile("" : "+v"(zero));
+#pragma unroll
+        for (int q = 0; q < S; ++q)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) yr[q][t] = yi[q][t] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const amp_t u = frag[(t * NB + b) * 64 + lane + zero];
+#pragma unroll
+                for (int q = 0; q < S; ++q) {
+                    yr[q][t] = mfma4(u.x, x[q][b].x, yr[q][t]);
+                    yr[q][t] = mfma4(-u.y, x[q][b].y, yr[q][t]);
+                    yi[q][t] = mfma4(u.y, x[q][b].x, yi[q][t]);
+                    yi[q][t] = mfma4(u.x, x[q][b].y, yi[q][t]);
+                }
+            }
+#pragma unroll
+        for (int q = 0; q < S; ++q)
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    st(base_ptr + (col[q] | o.blk[4 * t] | ld_lane | o.blk[v]),
+                       make_double2(yr[q][t][v], yi[q][t][v]));
+    }
 }
 
 // :169-189 -- |l> -> |perm[l]> on the control subspace, staged through LDS.
@@ -200,10 +305,14 @@ __global__ void __launch_bounds__(kBlock) k_cperm(amp_t* __restrict__ s, uint32_
     extern __shared__ amp_t grp[];
     const uint32_t D = 1u << L.nt;
     amp_t* b = batch_base(s, nq);
-    const uint64_t base = expand(blockIdx.x, L);
-    for (uint32_t l = threadIdx.x; l < D; l += blockDim.x) grp[l] = ld(b + (base | offs[l]));
-    __syncthreads();
-    for (uint32_t l = threadIdx.x; l < D; l += blockDim.x) st(b + (base | offs[perm[l]]), grp[l]);
+    const uint64_t groups = uint64_t{1} << (nq - L.n_fixed);
+    for (uint64_t g = blockIdx.x; g < groups; g += gridDim.x) {
+        const uint64_t base = expand(g, L);
+        for (uint32_t l = threadIdx.x; l < D; l += blockDim.x) grp[l] = ld(b + (base | offs[l]));
+        __syncthreads();
+        for (uint32_t l = threadIdx.x; l < D; l += blockDim.x) st(b + (base | offs[perm[l]]), grp[l]);
+        __syncthreads();  // grp is reused by the next group
+    }
 }
 
 // :191-225 -- exp(-i theta/2 P): pairs (i, i^x) with i's lowest x bit clear
@@ -212,23 +321,25 @@ __global__ void __launch_bounds__(kBlock) k_pauli_pairs(amp_t* __restrict__ s, u
                                                         amp_t c, amp_t yphase) {
     amp_t* b = batch_base(s, nq);
     const uint64_t groups = uint64_t{1} << (nq - 1);
-    amp_t ai[kItems], aj[kItems];
-    uint64_t ii[kItems];
+    LQCS_FOR_BLOCKS(blk, groups) {
+        amp_t ai[kItems], aj[kItems];
+        uint64_t ii[kItems];
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t g = group_index(k);
-        ii[k] = insert_zero_bit(g < groups ? g : 0, lsb);
-        ai[k] = ld(b + ii[k]);
-        aj[k] = ld(b + (ii[k] ^ x));
-    }
+        for (int k = 0; k < kItems; ++k) {
+            const uint64_t g = group_index(blk, k);
+            ii[k] = insert_zero_bit(g < groups ? g : 0, lsb);
+            ai[k] = ld(b + ii[k]);
+            aj[k] = ld(b + (ii[k] ^ x));
+        }
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        if (group_index(k) >= groups) continue;
-        const uint64_t i = ii[k], j = i ^ x;
-        const double si = (__popcll(i & zy) & 1) ? -1.0 : 1.0;
-        const double sj = (__popcll(j & zy) & 1) ? -1.0 : 1.0;
-        st(b + i, cmac2(c, ai[k], cscale(yphase, sj), aj[k]));
-        st(b + j, cmac2(cscale(yphase, si), ai[k], c, aj[k]));
+        for (int k = 0; k < kItems; ++k) {
+            if (group_index(blk, k) >= groups) continue;
+            const uint64_t i = ii[k], j = i ^ x;
+            const double si = (__popcll(i & zy) & 1) ? -1.0 : 1.0;
+            const double sj = (__popcll(j & zy) & 1) ? -1.0 : 1.0;
+            st(b + i, cmac2(c, ai[k], cscale(yphase, sj), aj[k]));
+            st(b + j, cmac2(cscale(yphase, si), ai[k], c, aj[k]));
+        }
     }
 }
 
@@ -237,17 +348,19 @@ __global__ void __launch_bounds__(kBlock) k_pauli_diag(amp_t* __restrict__ s, ui
     amp_t* b = batch_base(s, nq);
     const uint64_t n = uint64_t{1} << nq;
     const amp_t dp = cadd(c, yphase), dm = cadd(c, cscale(yphase, -1.0));
-    amp_t a[kItems];
+    LQCS_FOR_BLOCKS(blk, n) {
+        amp_t a[kItems];
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t i = group_index(k);
-        a[k] = ld(b + (i < n ? i : 0));
-    }
+        for (int k = 0; k < kItems; ++k) {
+            const uint64_t i = group_index(blk, k);
+            a[k] = ld(b + (i < n ? i : 0));
+        }
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t i = group_index(k);
-        if (i >= n) continue;
-        st(b + i, cmul(a[k], (__popcll(i & zy) & 1) ? dm : dp));
+        for (int k = 0; k < kItems; ++k) {
+            const uint64_t i = group_index(blk, k);
+            if (i >= n) continue;
+            st(b + i, cmul(a[k], (__popcll(i & zy) & 1) ? dm : dp));
+        }
     }
 }
 
@@ -287,27 +400,34 @@ std::vector<uint64_t> target_offsets(const uint32_t* tgt, uint32_t nt) {
 
 }  // namespace
 
+uint32_t g_dense_mfma_min_targets = 4;
+
 void launch_single(amp_t* s, uint32_t nq, uint32_t t, const Mat2& m, uint32_t batch) {
-    k_single<<<dim3(grid_for(uint64_t{1} << (nq - 1), kItems), batch), kBlock, 0, stream()>>>(s, nq, t, m);
+    k_single<<<dim3(grid_capped(uint64_t{1} << (nq - 1), kItems), batch), kBlock, 0, stream()>>>(s, nq, t, m);
+    launched();
 }
 
 void launch_diag_single(amp_t* s, uint32_t nq, uint32_t t, amp_t d0, amp_t d1, uint32_t batch) {
-    k_diag_single<<<dim3(grid_for(uint64_t{1} << nq, kItems), batch), kBlock, 0, stream()>>>(s, nq, t, d0, d1);
+    k_diag_single<<<dim3(grid_capped(uint64_t{1} << nq, kItems), batch), kBlock, 0, stream()>>>(s, nq, t, d0, d1);
+    launched();
 }
 
 void launch_controlled_single(amp_t* s, uint32_t nq, uint32_t c, uint32_t t, const Mat2& m,
                               uint32_t batch) {
-    k_controlled_single<<<dim3(grid_for(uint64_t{1} << (nq - 2), kItems), batch), kBlock, 0, stream()>>>(
+    k_controlled_single<<<dim3(grid_capped(uint64_t{1} << (nq - 2), kItems), batch), kBlock, 0, stream()>>>(
         s, nq, c, t, m);
+    launched();
 }
 
 void launch_swap(amp_t* s, uint32_t nq, uint32_t a, uint32_t b, uint32_t batch) {
-    k_swap<<<dim3(grid_for(uint64_t{1} << (nq - 2), kItems), batch), kBlock, 0, stream()>>>(s, nq, a, b);
+    k_swap<<<dim3(grid_capped(uint64_t{1} << (nq - 2), kItems), batch), kBlock, 0, stream()>>>(s, nq, a, b);
+    launched();
 }
 
 void launch_two_qubit(amp_t* s, uint32_t nq, uint32_t q0, uint32_t q1, const Mat4& m,
                       uint32_t batch) {
-    k_two_qubit<<<dim3(grid_for(uint64_t{1} << (nq - 2)), batch), kBlock, 0, stream()>>>(s, nq, q0, q1, m);
+    k_two_qubit<<<dim3(grid_capped(uint64_t{1} << (nq - 2)), batch), kBlock, 0, stream()>>>(s, nq, q0, q1, m);
+    launched();
 }
 
 void launch_controlled_unitary(amp_t* s, uint32_t nq, const uint32_t* ctl, uint32_t nc,
@@ -317,19 +437,39 @@ void launch_controlled_unitary(amp_t* s, uint32_t nq, const uint32_t* ctl, uint3
     const std::vector<uint64_t> off = target_offsets(tgt, nt);
     const uint64_t D = uint64_t{1} << nt;
     const uint64_t groups = uint64_t{1} << (nq - L.n_fixed);
-    const uint64_t* d_off = stage(off.data(), off.size());
     const amp_t* d_mat = stage(m_host, D * D);
+    if (nt >= g_dense_mfma_min_targets && nt >= 4 && nt <= 6) {
+        const uint32_t S = nt == 4 ? 4 : nt == 5 ? 2 : 1;
+        const uint64_t slabs = groups / 16;
+        if (groups >= 16 * S) {
+            DenseOffsets o{};
+            for (uint64_t b = 0; b < D / 4; ++b) o.blk[b] = off[4 * b];
+            for (int v = 0; v < 4; ++v) o.lo[v] = off[v];
+            const uint64_t iters = slabs / S;
+            const uint64_t want = (iters + kBlock / 64 - 1) / (kBlock / 64);
+            const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(want, uint64_t(num_cus()) * 8));
+            const dim3 g(grid, batch);
+            switch (nt) {
+                case 4: k_cu_mfma<4, 4><<<g, kBlock, 0, stream()>>>(s, nq, L, o, d_mat, iters); launched(); return;
+                case 5: k_cu_mfma<5, 2><<<g, kBlock, 0, stream()>>>(s, nq, L, o, d_mat, iters); launched(); return;
+                default: k_cu_mfma<6, 1><<<g, kBlock, 0, stream()>>>(s, nq, L, o, d_mat, iters); launched(); return;
+            }
+        }
+    }
+    const uint64_t* d_off = stage(off.data(), off.size());
+    const dim3 g(grid_capped(groups), batch);
     switch (nt) {
-        case 1: k_cu_small<1><<<dim3(grid_for(groups), batch), kBlock, 0, stream()>>>(s, nq, L, d_off, d_mat); return;
-        case 2: k_cu_small<2><<<dim3(grid_for(groups), batch), kBlock, 0, stream()>>>(s, nq, L, d_off, d_mat); return;
-        case 3: k_cu_small<3><<<dim3(grid_for(groups), batch), kBlock, 0, stream()>>>(s, nq, L, d_off, d_mat); return;
-        case 4: k_cu_small<4><<<dim3(grid_for(groups), batch), kBlock, 0, stream()>>>(s, nq, L, d_off, d_mat); return;
+        case 1: k_cu_small<1><<<g, kBlock, 0, stream()>>>(s, nq, L, d_off, d_mat); launched(); return;
+        case 2: k_cu_small<2><<<g, kBlock, 0, stream()>>>(s, nq, L, d_off, d_mat); launched(); return;
+        case 3: k_cu_small<3><<<g, kBlock, 0, stream()>>>(s, nq, L, d_off, d_mat); launched(); return;
+        case 4: k_cu_small<4><<<g, kBlock, 0, stream()>>>(s, nq, L, d_off, d_mat); launched(); return;
         default: break;
     }
     if (D > uint64_t(kBlock) * LQCS_CU_LARGE_ROWS)
         throw Error(3, "controlled unitary wider than the engine supports");
-    k_cu_large<<<dim3(static_cast<unsigned>(groups), batch), kBlock, D * sizeof(amp_t), stream()>>>(
+    k_cu_large<<<dim3(capped(groups), batch), kBlock, D * sizeof(amp_t), stream()>>>(
         s, nq, L, d_off, d_mat);
+    launched();
 }
 
 void launch_controlled_permutation(amp_t* s, uint32_t nq, const uint32_t* ctl, uint32_t nc,
@@ -342,8 +482,9 @@ void launch_controlled_permutation(amp_t* s, uint32_t nq, const uint32_t* ctl, u
     const uint64_t groups = uint64_t{1} << (nq - L.n_fixed);
     const uint64_t* d_off = stage(off.data(), off.size());
     const uint64_t* d_perm = stage(perm_host, D);
-    k_cperm<<<dim3(static_cast<unsigned>(groups), batch), kBlock, D * sizeof(amp_t), stream()>>>(s, nq, L, d_off,
+    k_cperm<<<dim3(capped(groups), batch), kBlock, D * sizeof(amp_t), stream()>>>(s, nq, L, d_off,
                                                                                    d_perm);
+    launched();
 }
 
 void launch_pauli_rotation(amp_t* s, uint32_t nq, uint64_t x, uint64_t zy, uint32_t n_y,
@@ -355,19 +496,22 @@ void launch_pauli_rotation(amp_t* s, uint32_t nq, uint64_t x, uint64_t zy, uint3
     const amp_t ip = kIPow[n_y % 4];
     const amp_t yphase = make_double2(mis.x * ip.x - mis.y * ip.y, mis.x * ip.y + mis.y * ip.x);
     if (x == 0) {
-        k_pauli_diag<<<dim3(grid_for(uint64_t{1} << nq, kItems), batch), kBlock, 0, stream()>>>(s, nq, zy, c,
-                                                                                   yphase);
+        k_pauli_diag<<<dim3(grid_capped(uint64_t{1} << nq, kItems), batch), kBlock, 0, stream()>>>(s, nq, zy, c,
+                                                                                                  yphase);
+        launched();
         return;
     }
     const uint32_t lsb = static_cast<uint32_t>(__builtin_ctzll(x));
-    k_pauli_pairs<<<dim3(grid_for(uint64_t{1} << (nq - 1), kItems), batch), kBlock, 0, stream()>>>(
+    k_pauli_pairs<<<dim3(grid_capped(uint64_t{1} << (nq - 1), kItems), batch), kBlock, 0, stream()>>>(
         s, nq, x, zy, lsb, c, yphase);
+    launched();
 }
 
 void launch_fill_basis(amp_t* s, uint32_t nq, uint64_t index, uint32_t batch) {
     const uint64_t n = uint64_t{1} << nq;
     const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(grid_for(n), 65536));
     k_fill_basis<<<dim3(grid, batch), kBlock, 0, stream()>>>(s, nq, index);
+    launched();
 }
 
 }  // namespace lqcs_dev

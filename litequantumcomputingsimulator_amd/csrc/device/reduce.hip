@@ -263,18 +263,19 @@ __global__ void __launch_bounds__(kBlock) k_collapse(amp_t* __restrict__ s, uint
     const bool one = u < p;
     const double scale = 1.0 / sqrt(one ? p : 1.0 - p);
     const uint64_t half = uint64_t{1} << (nq - 1);
-    const uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (g == 0 && clbit >= 0 && creg) creg[uint64_t(r) * creg_stride + clbit] = one ? 1 : 0;
-    if (g >= half) return;
-    const uint64_t i0 = insert_zero_bit(g, q), i1 = i0 | (uint64_t{1} << q);
-    const amp_t a0 = b[i0], a1 = b[i1];
+    const uint64_t g0 = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (g0 == 0 && clbit >= 0 && creg) creg[uint64_t(r) * creg_stride + clbit] = one ? 1 : 0;
     const amp_t z = make_double2(0.0, 0.0);
-    if (one) {
-        if (reset) { b[i0] = cscale(a1, scale); b[i1] = z; }
-        else { b[i0] = z; b[i1] = cscale(a1, scale); }
-    } else {
-        b[i0] = cscale(a0, scale);
-        b[i1] = z;
+    for (uint64_t g = g0; g < half; g += uint64_t(gridDim.x) * kBlock) {
+        const uint64_t i0 = insert_zero_bit(g, q), i1 = i0 | (uint64_t{1} << q);
+        const amp_t a0 = b[i0], a1 = b[i1];
+        if (one) {
+            if (reset) { b[i0] = cscale(a1, scale); b[i1] = z; }
+            else { b[i0] = z; b[i1] = cscale(a1, scale); }
+        } else {
+            b[i0] = cscale(a0, scale);
+            b[i1] = z;
+        }
     }
 }
 
@@ -359,7 +360,7 @@ void reduce_qstats(const amp_t* s, uint32_t nq, uint32_t q, uint32_t batch, doub
 void launch_collapse(amp_t* s, uint32_t nq, uint32_t q, const double* d_p1, const double* d_draws,
                      uint64_t draw_stride, uint64_t draw_offset, int reset, uint8_t* d_creg,
                      uint32_t creg_stride, int32_t clbit, uint32_t batch) {
-    k_collapse<<<dim3(grid_for(uint64_t{1} << (nq - 1)), batch), kBlock, 0, stream()>>>(
+    k_collapse<<<dim3(std::min<unsigned>(grid_for(uint64_t{1} << (nq - 1)), 1u << 22), batch), kBlock, 0, stream()>>>(
         s, nq, q, d_p1, d_draws, draw_stride, draw_offset, reset, d_creg, creg_stride, clbit);
 }
 

@@ -35,14 +35,14 @@ __global__ void __launch_bounds__(T) k_tile_pass(amp_t* __restrict__ s, uint32_t
                                                  uint64_t smask, uint64_t smask_lo,
                                                  uint64_t smask_hi,
                                                  const TileOp* __restrict__ ops,
-                                                 uint32_t n_ops) {
+                                                 uint32_t n_ops, uint64_t tile0) {
     constexpr int N = 1 << L;
     constexpr int PER = N / T;
     __shared__ amp_t tile[N];
 
     amp_t* b = s + (uint64_t(blockIdx.y) << nq);
     const uint64_t full = (nq == 64) ? ~uint64_t{0} : ((uint64_t{1} << nq) - 1);
-    const uint64_t base = pdep64(blockIdx.x, full & ~smask);
+    const uint64_t base = pdep64(tile0 + blockIdx.x, full & ~smask);
     const uint64_t lo = pdep64(threadIdx.x, smask_lo);
 
     {
@@ -172,8 +172,15 @@ void launch_t(amp_t* s, uint32_t nq, uint64_t smask, const TileOp* ops, uint32_t
     uint64_t lo, hi;
     split_mask(smask, __builtin_ctz(T), lo, hi);
     const uint64_t tiles = uint64_t{1} << (nq - L);
-    k_tile_pass<L, T><<<dim3(static_cast<unsigned>(tiles), batch), T, 0, stream()>>>(
-        s, nq, smask, lo, hi, ops, n_ops);
+    // One workgroup per tile, launched in chunks that keep every grid within
+    // the device's 2^32 work-items.
+    constexpr uint64_t kChunk = uint64_t{1} << 22;
+    for (uint64_t t0 = 0; t0 < tiles; t0 += kChunk) {
+        const uint64_t n = std::min(kChunk, tiles - t0);
+        k_tile_pass<L, T><<<dim3(static_cast<unsigned>(n), batch), T, 0, stream()>>>(s, nq, smask, lo, hi, ops,
+                                                                                      n_ops, t0);
+        LQCS_HIP(hipGetLastError());
+    }
 }
 
 }  // namespace

@@ -447,6 +447,9 @@ PYBIND11_MODULE(_pylqcs, m) {
         if (code < 0) throw std::invalid_argument("tile kernel must be 'register', 'stream', 'land' or 'lds'");
         if (lqcs_set_tile_kernel(code) != LQCS_OK) throw std::runtime_error(lqcs_dev_last_error());
     }, "kernel"_a);
+    m.def("set_dense_mfma_min_targets", [](uint32_t k) {
+        if (lqcs_set_dense_mfma_min_targets(k) != LQCS_OK) throw std::invalid_argument(lqcs_dev_last_error());
+    }, "min_targets"_a);
     m.def("profile_read", [] {
         const LaunchProfile p = profile_read();
         return py::dict("tile_launches"_a = p.tile_launches, "tile_ms"_a = p.tile_ms,
