@@ -16,7 +16,9 @@ parameter-shift gradients on the same GPU (timed on a bounded sample). Two
 more objects report the noisy-trajectory workload (24 qubits, 2048 shots,
 trajectories/s) and the learned planner (ms per step of the cost model's plan
 next to greedy's, on the headline workload and on random U3/CX/CP layers of
-depth 40 at 32 qubits, with the two final states compared).
+depth 40 at 32 qubits, with the two final states compared). A last object
+times fused dense blocks of 5 and 6 qubits (complex GEMMs on FP64 MFMA) at
+30 qubits against the FP64 peak, with the reference beside them.
 
 `traffic` (HBM bytes per tiled pass) comes from the rocprofv3 counter summary
 committed under profiles/ for the same kernel and workload, when one exists.
@@ -209,6 +211,95 @@ def planner_32q(lq, W, seed: int, n: int = 32, depth: int = 40) -> dict:
             "learned_vs_greedy": round(t_learned / t_greedy, 4), "max_abs_diff_states": diff}
 
 
+PEAK_FP64_TFLOPS = 78.6
+
+
+def dense_blocks_bench(lq, seed: int, n: int = 30, reps: int = 4, cpu_seconds: float = 8.0,
+                       cpu: bool = True) -> dict:
+    """Fused dense blocks: the reference's generic controlled-unitary path
+    (src/backend/kernels/gate_kernels.cpp:143-167) for k = 5 and 6 targets,
+    here complex GEMMs on FP64 MFMA. `reps` Haar-random k-qubit unitaries on
+    qubits 8..8+k of an n-qubit register, one launch each, HIP-event timed.
+    Per launch: 32 * 2^n bytes (one read and one write of every amplitude) and
+    (8 * 2^k - 2) * 2^n flop; k = 6 is bound by FP64 MFMA, k = 5 by HBM. The
+    reference runs the same circuit on the host cores at 24 qubits, scaled by
+    2^(n - 24)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+
+    def haar(k):
+        d = 1 << k
+        q, r = np.linalg.qr(rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d)))
+        return q * (np.diag(r) / np.abs(np.diag(r)))
+
+    out = {"workload": f"{reps} Haar-random k-qubit unitaries on qubits 8..8+k, {n} qubits, one launch each",
+           "kernel": "k_cu_mfma"}
+    sv = lq.statevector(n)
+    mats = {}
+    for k in (5, 6):
+        mats[k] = [haar(k) for _ in range(reps)]
+        qc = lq.QuantumCircuit(n)
+        for m in mats[k]:
+            qc.unitary(m, list(range(8, 8 + k)))
+        comp = lq.CompiledCircuit(qc, False)
+        comp.apply(sv, plan="gate_by_gate")  # untimed
+        lq.synchronize()
+        lq.profile_enable(True)
+        lq.profile_read_launches()
+        comp.apply(sv, plan="gate_by_gate")
+        ms, _ = lq.profile_read_launches()
+        lq.profile_enable(False)
+        t = float(np.mean(ms))
+        gbs = 32 * 2**n / t / 1e6
+        tflops = (8 * 2**k - 2) * 2**n / t / 1e9
+        out[f"k{k}"] = {"avg_launch_ms": round(t, 4), "GB/s": round(gbs, 1), "TFLOP/s": round(tflops, 2),
+                        "hbm_frac": round(gbs / PEAK_HBM_GBPS, 4), "fp64_frac": round(tflops / PEAK_FP64_TFLOPS, 4)}
+        log(f"dense k={k}: {t:.3f} ms per launch, {gbs:.0f} GB/s, {tflops:.1f} TFLOP/s")
+    k6 = out["k6"]
+    traffic, src = None, None
+    try:
+        with open(os.path.join(ROOT, "profiles", f"k_cu_mfma_dense_{n}q.json")) as f:
+            pm = json.load(f)
+        if pm.get("workload", {}).get("qubits") == n and pm.get("workload", {}).get("blocks") == reps:
+            traffic, src = pm["per_k"]["6"]["traffic_bytes_per_dispatch"], f"profiles/k_cu_mfma_dense_{n}q.json"
+    except (OSError, ValueError, KeyError):
+        pass
+    out["roofline"] = {"bound": "mfma", "achieved": k6["TFLOP/s"], "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                       "frac": k6["fp64_frac"], "traffic": traffic, "traffic_source": src, "kernel": "k_cu_mfma",
+                       "flop_per_launch": (8 * 64 - 2) * 2**n, "avg_launch_ms": k6["avg_launch_ms"]}
+    del sv
+    if cpu:
+        try:
+            from oracle import oracle as O
+            n_s = 24
+            c = O.Circuit(n_s)
+            for m in mats[6]:
+                c.unitary(m, list(range(8, 14)))
+            cores = len(os.sched_getaffinity(0))
+            kind = "reference" if O.ref_available() else "port"
+            if kind == "reference":
+                O.ref_set_threads(cores)
+                run = lambda: O.ref_run_statevector(c, fuse=False)  # noqa: E731
+            else:
+                run = lambda: O.run_statevector(c, fuse=False)  # noqa: E731
+            reps_done, t_total = 0, 0.0
+            while t_total < cpu_seconds and reps_done < 20:
+                t0 = time.perf_counter()
+                run()
+                t_total += time.perf_counter() - t0
+                reps_done += 1
+            per_gate = t_total / (reps_done * reps) * (1 << (n - n_s))
+            out["cpu_baseline"] = {"value": round(1e3 * per_gate, 3), "unit": "ms per k=6 block at 30 qubits",
+                                   "cores": cores, "kind": kind,
+                                   "sample": f"{reps_done} x run_statevector of the {reps} k=6 blocks at {n_s} "
+                                             f"qubits; seconds per block scaled by 2^{n - n_s}"}
+            log(f"dense cpu baseline: {1e3 * per_gate:.1f} ms per k=6 block at {n} qubits ({kind})")
+        except Exception as e:  # noqa: BLE001
+            log(f"dense cpu baseline unavailable: {e}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +319,8 @@ def main() -> int:
     ap.add_argument("--traj-qubits", type=int, default=24)
     ap.add_argument("--traj-shots", type=int, default=2048)
     ap.add_argument("--no-trajectories", action="store_true")
+    ap.add_argument("--no-dense", action="store_true",
+                    help="skip the dense-block workload (k = 5, 6 targets on FP64 MFMA)")
     ap.add_argument("--no-planner", action="store_true",
                     help="skip timing the learned plan on the same workload")
     ap.add_argument("--no-planner-32q", action="store_true",
@@ -331,6 +424,9 @@ def main() -> int:
         trajectories = trajectories_bench(lq, W, args.seed, args.traj_qubits, args.traj_shots)
     if planner is not None and not args.no_planner_32q:
         planner["at_32_qubits"] = planner_32q(lq, W, args.seed)
+    dense = None
+    if not args.no_dense:
+        dense = dense_blocks_bench(lq, args.seed, cpu=not args.no_cpu_baseline)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(n, layers, args.seed, args.cpu_seconds)
 
@@ -358,6 +454,7 @@ def main() -> int:
         "adjoint": adjoint,
         "trajectories": trajectories,
         "planner": planner,
+        "dense_blocks": dense,
     }
     print(json.dumps(line), flush=True)
     return 0
