@@ -221,6 +221,20 @@ lqcs_status lqcs_state_norm_sq(const lqcs_dev_state* s, double* out);
 lqcs_status lqcs_state_max_abs_diff(const lqcs_dev_state* a, const lqcs_dev_state* b, double* out);
 /* Statevector::probabilities (statevector.cpp:33-37) into host memory. */
 lqcs_status lqcs_state_probabilities(const lqcs_dev_state* s, double* out);
+/* Copies amplitudes [first, first + count) to out (interleaved re, im):
+ * the device side of Statevector::operator[] and DensityMatrix::element. */
+lqcs_status lqcs_state_read(const lqcs_dev_state* s, uint64_t first, uint64_t count, double* out);
+
+/* Density matrices (reference include/lqcs/backend/density_matrix.hpp:17-44,
+ * src/backend/density_matrix.cpp:19-124): an n-qubit rho is a 2n-qubit register,
+ * rho[r][c] at index r | (c << n) (ket bits low, bra bits high), evolved by
+ * lqcs_state_apply with U on the ket qubits, conj(U) on the bra qubits and
+ * each Kraus channel as its 4x4 superoperator sum_i K_i (x) conj(K_i) on the
+ * qubit pair (q, q + n). */
+/* out[r] = Re rho[r][r], 2^n doubles (DensityMatrix::probabilities, density_matrix.cpp:100-105). */
+lqcs_status lqcs_dm_diagonal(const lqcs_dev_state* rho, double* out);
+/* <psi|rho|psi> for an n-qubit register psi (DensityMatrix::fidelity, density_matrix.cpp:107-124). */
+lqcs_status lqcs_dm_fidelity(const lqcs_dev_state* rho, const lqcs_dev_state* psi, double* out);
 /* Statevector::expectation_value (statevector.cpp:39-78) with the string
  * already turned into global masks (leftmost char = qubit n-1). */
 lqcs_status lqcs_state_expectation_pauli(const lqcs_dev_state* s, uint64_t x_mask,

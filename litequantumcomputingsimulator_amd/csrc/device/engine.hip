@@ -1038,6 +1038,49 @@ lqcs_status lqcs_state_probabilities(const lqcs_dev_state* s, double* out) {
     });
 }
 
+lqcs_status lqcs_state_read(const lqcs_dev_state* s, uint64_t first, uint64_t count, double* out) {
+    return guarded([&] {
+        need(s, "state");
+        need(out, "out");
+        if (first > amps_of(s->nq) || count > amps_of(s->nq) - first)
+            throw Error(LQCS_ERR_OUT_OF_RANGE, "read: amplitude range out of range");
+        ensure_ready();
+        if (count)
+            LQCS_HIP(hipMemcpyAsync(out, s->d + first, count * sizeof(amp_t), hipMemcpyDeviceToHost, stream()));
+        sync();
+    });
+}
+
+lqcs_status lqcs_dm_diagonal(const lqcs_dev_state* rho, double* out) {
+    return guarded([&] {
+        need(rho, "rho");
+        need(out, "out");
+        if (rho->nq % 2) throw Error(LQCS_ERR_INVALID_ARGUMENT, "density matrix: register width must be even");
+        const uint32_t n = rho->nq / 2;
+        ensure_ready();
+        arena_reset();
+        double* d = static_cast<double*>(arena(sizeof(double) * amps_of(n)));
+        launch_dm_diagonal(rho->d, n, d);
+        LQCS_HIP(hipMemcpyAsync(out, d, sizeof(double) * amps_of(n), hipMemcpyDeviceToHost, stream()));
+        sync();
+    });
+}
+
+lqcs_status lqcs_dm_fidelity(const lqcs_dev_state* rho, const lqcs_dev_state* psi, double* out) {
+    return guarded([&] {
+        need(rho, "rho");
+        need(psi, "psi");
+        need(out, "out");
+        if (rho->nq != 2 * psi->nq) throw Error(LQCS_ERR_INVALID_ARGUMENT, "fidelity: qubit count mismatch");
+        ensure_ready();
+        arena_reset();
+        double* d = static_cast<double*>(arena(sizeof(double)));
+        reduce_dm_fidelity(rho->d, psi->d, psi->nq, d);
+        LQCS_HIP(hipMemcpyAsync(out, d, sizeof(double), hipMemcpyDeviceToHost, stream()));
+        sync();
+    });
+}
+
 lqcs_status lqcs_state_expectation_pauli(const lqcs_dev_state* s, uint64_t x_mask, uint64_t zy_mask,
                                          uint32_t n_y, double* out) {
     const double one = 1.0;

@@ -319,7 +319,42 @@ unsigned red_blocks(uint32_t nq, uint32_t shift = 0) {
     return static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(kRedBlocks, want)));
 }
 
+// ---- density matrices held as 2n-qubit registers: rho[r][c] at r | (c << n) ----------
+// (reference include/lqcs/backend/density_matrix.hpp: ket bits low, bra bits high)
+__global__ void __launch_bounds__(kBlock) k_dm_diagonal(const amp_t* __restrict__ s, uint32_t n,
+                                                        double* __restrict__ out) {
+    const uint64_t d = uint64_t{1} << n;
+    for (uint64_t r = uint64_t(blockIdx.x) * kBlock + threadIdx.x; r < d; r += uint64_t(gridDim.x) * kBlock)
+        out[r] = s[(r << n) | r].x;
+}
+
+// Partial sums of Re(conj(psi_r) rho[r][c] psi_c) over all (r, c).
+__global__ void __launch_bounds__(kBlock) k_dm_fidelity_partial(const amp_t* __restrict__ rho,
+                                                                const amp_t* __restrict__ psi, uint32_t n,
+                                                                double* __restrict__ part) {
+    const uint64_t total = uint64_t{1} << (2 * n), mask = (uint64_t{1} << n) - 1;
+    double acc[1] = {0.0};
+    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < total; i += uint64_t(gridDim.x) * kBlock) {
+        const amp_t t = cmul(cmul(cconj(psi[i & mask]), rho[i]), psi[i >> n]);
+        acc[0] += t.x;
+    }
+    block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
 }  // namespace
+
+void launch_dm_diagonal(const amp_t* rho, uint32_t n, double* d_out) {
+    const unsigned g = std::min<unsigned>(grid_for(uint64_t{1} << n), 4096);
+    k_dm_diagonal<<<g, kBlock, 0, stream()>>>(rho, n, d_out);
+}
+
+void reduce_dm_fidelity(const amp_t* rho, const amp_t* psi, uint32_t n, double* d_out) {
+    const unsigned g = red_blocks(2 * n);
+    double* part = static_cast<double*>(arena(sizeof(double) * g));
+    k_dm_fidelity_partial<<<g, kBlock, 0, stream()>>>(rho, psi, n, part);
+    k_finish<1><<<1, kBlock, 0, stream()>>>(part, g, d_out);
+}
 
 void reduce_norm_sq(const amp_t* s, uint32_t nq, uint32_t batch, double* d_out) {
     const unsigned g = red_blocks(nq);

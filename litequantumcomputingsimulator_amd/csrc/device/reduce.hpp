@@ -21,5 +21,9 @@ void launch_collapse(amp_t* s, uint32_t nq, uint32_t q, const double* d_p1, cons
 void launch_probabilities(const amp_t* s, uint32_t nq, double* d_out);
 void launch_sample(const amp_t* s, uint32_t nq, const double* d_draws, uint64_t n_draws,
                    uint64_t* d_out);
+// Density matrices as 2n-qubit registers (rho[r][c] at r | c << n):
+// d_out[r] = Re rho[r][r]; and <psi|rho|psi> for an n-qubit psi.
+void launch_dm_diagonal(const amp_t* rho, uint32_t n, double* d_out);
+void reduce_dm_fidelity(const amp_t* rho, const amp_t* psi, uint32_t n, double* d_out);
 
 }  // namespace lqcs_dev

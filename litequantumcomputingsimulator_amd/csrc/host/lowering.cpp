@@ -99,6 +99,50 @@ uint32_t ProgramBuilder::add_instruction(const Instruction& inst) {
     return push(LQCS_OP_CONTROLLED_UNITARY, ctl, tgt, m);
 }
 
+uint32_t ProgramBuilder::add_instruction_conj(const Instruction& inst, qubit_t shift) {
+    const Gate& g = inst.gate;
+    if (!g.is_unitary()) throw std::logic_error("lowering: " + g.name() + " is not a unitary op");
+    const std::size_t nc = g.n_controls;
+    std::vector<qubit_t> q(inst.qubits);
+    for (qubit_t& v : q) v += shift;
+    const std::vector<qubit_t> ctl(q.begin(), q.begin() + nc);
+    const std::vector<qubit_t> tgt(q.begin() + nc, q.end());
+    const std::size_t k = g.base_qubits();
+
+    if (g.type == GateType::Permutation) {  // real 0/1 matrix: its own conjugate
+        const uint32_t idx = push(LQCS_OP_CONTROLLED_PERMUTATION, ctl, tgt, {});
+        ops_[idx].data_offset = perm_.size();
+        for (std::size_t v : g.perm) perm_.push_back(v);
+        return idx;
+    }
+    if (g.type == GateType::PauliRotation && nc == 0) {
+        // conj(exp(-i t/2 P)) = exp(i t/2 conj(P)) and conj(P) = (-1)^{#Y} P.
+        const Gate::PauliMasks pm = g.pauli_masks();
+        uint64_t x = 0, zy = 0;
+        for (std::size_t j = 0; j < k; ++j) {
+            if (pm.x & (std::size_t{1} << j)) x |= uint64_t{1} << tgt[j];
+            if (pm.zy & (std::size_t{1} << j)) zy |= uint64_t{1} << tgt[j];
+        }
+        const double t = g.params.at(0);
+        return add_pauli_rotation(x, zy, pm.n_y, (pm.n_y % 2) ? t : -t);
+    }
+    std::vector<complex_t> m = g.base_matrix();
+    for (complex_t& z : m) z = std::conj(z);
+    if (nc == 0 && k == 1) {
+        if (g.is_diagonal()) return push(LQCS_OP_DIAGONAL_SINGLE, {}, tgt, {m[0], m[3]});
+        return push(LQCS_OP_SINGLE, {}, tgt, m);
+    }
+    if (nc == 0 && g.type == GateType::SWAP) return push(LQCS_OP_SWAP, {}, tgt, {});
+    if (nc == 0 && k == 2) return push(LQCS_OP_TWO_QUBIT, {}, tgt, m);
+    if (nc == 1 && k == 1) return push(LQCS_OP_CONTROLLED_SINGLE, ctl, tgt, m);
+    return push(LQCS_OP_CONTROLLED_UNITARY, ctl, tgt, m);
+}
+
+uint32_t ProgramBuilder::add_two_qubit(qubit_t q0, qubit_t q1, const std::vector<complex_t>& m) {
+    if (m.size() != 16) throw std::invalid_argument("lowering: a two-qubit op needs 16 entries");
+    return push(LQCS_OP_TWO_QUBIT, {}, {q0, q1}, m);
+}
+
 lqcs_program ProgramBuilder::program() const {
     lqcs_program p{};
     p.ops = ops_.data();

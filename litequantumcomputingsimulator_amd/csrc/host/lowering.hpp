@@ -30,8 +30,13 @@ public:
     // Lowers one unitary instruction with the reference dispatch order
     // (statevector_simulator.cpp:22-85). Returns the op index.
     uint32_t add_instruction(const Instruction& inst);
+    // Lowers conj(U) of a unitary instruction onto its qubits + `shift` (the
+    // bra side of a density matrix), with the same dispatch.
+    uint32_t add_instruction_conj(const Instruction& inst, qubit_t shift);
     // Adds exp(-i theta/2 P) with global masks.
     uint32_t add_pauli_rotation(uint64_t x, uint64_t zy, uint32_t n_y, double theta);
+    // Adds a dense 4x4 op (unitary or not) on (q0, q1), local bit 0 <-> q0.
+    uint32_t add_two_qubit(qubit_t q0, qubit_t q1, const std::vector<complex_t>& m);
     std::size_t size() const { return ops_.size(); }
     lqcs_program program() const;
 

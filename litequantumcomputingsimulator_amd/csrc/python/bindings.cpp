@@ -285,6 +285,62 @@ PYBIND11_MODULE(_pylqcs, m) {
              "circuit"_a, "shots"_a = 1024)
         .def("name", &StatevectorSimulator::name);
 
+    // Density-matrix backend (reference python/src/bindings.cpp:416-443), on the device.
+    py::class_<DensityMatrix>(m, "DensityMatrix")
+        .def(py::init<std::size_t>(), "num_qubits"_a)
+        .def(py::init<std::size_t, std::size_t>(), "num_qubits"_a, "max_qubits"_a)
+        .def("num_qubits", &DensityMatrix::num_qubits)
+        .def("dim", &DensityMatrix::dim)
+        .def("element", &DensityMatrix::element, "row"_a, "col"_a)
+        .def("apply_channel", &DensityMatrix::apply_channel, "channel"_a, "qubit"_a,
+             py::call_guard<py::gil_scoped_release>())
+        .def("trace", &DensityMatrix::trace, py::call_guard<py::gil_scoped_release>())
+        .def("purity", &DensityMatrix::purity, py::call_guard<py::gil_scoped_release>())
+        .def("fidelity", &DensityMatrix::fidelity, "psi"_a, py::call_guard<py::gil_scoped_release>())
+        .def("probabilities", [](const DensityMatrix& rho) {
+            std::vector<double> p;
+            {
+                py::gil_scoped_release nogil;
+                p = rho.probabilities();
+            }
+            py::array_t<double> a(static_cast<py::ssize_t>(p.size()));
+            std::copy(p.begin(), p.end(), a.mutable_data());
+            return a;
+        })
+        // Engine extension: rho as a dim x dim matrix (a copy).
+        .def("to_numpy", [](const DensityMatrix& rho) {
+            const auto d = static_cast<py::ssize_t>(rho.dim());
+            py::array_t<std::complex<double>> a({d, d});
+            {
+                py::gil_scoped_release nogil;
+                // Device order is r | (c << n): row-major in (c, r), i.e. the transpose.
+                if (lqcs_state_read(rho.device(), 0, static_cast<uint64_t>(d) * d,
+                                    reinterpret_cast<double*>(a.mutable_data())) != LQCS_OK)
+                    throw std::runtime_error(lqcs_dev_last_error());
+            }
+            return a.attr("T").attr("copy")();
+        });
+
+    py::class_<DensityMatrixSimulator>(m, "DensityMatrixSimulator")
+        .def(py::init([](std::optional<std::uint64_t> seed, NoiseModel noise, const std::string& plan,
+                         int tile_qubits, int low_qubits, std::size_t max_qubits) {
+                 DensityMatrixSimulator::Options o;
+                 o.seed = seed;
+                 o.noise = std::move(noise);
+                 o.engine.plan = plan_of(plan);
+                 o.engine.tile_qubits = tile_qubits;
+                 o.engine.low_qubits = low_qubits;
+                 o.max_qubits = max_qubits;
+                 return DensityMatrixSimulator(std::move(o));
+             }),
+             "seed"_a = py::none(), "noise"_a = NoiseModel{}, "plan"_a = "auto", "tile_qubits"_a = 0,
+             "low_qubits"_a = 0, "max_qubits"_a = DensityMatrix::kMaxQubits)
+        .def("run", &DensityMatrixSimulator::run, "circuit"_a, "shots"_a = 1024,
+             py::call_guard<py::gil_scoped_release>())
+        .def("run_density_matrix", &DensityMatrixSimulator::run_density_matrix, "circuit"_a,
+             py::call_guard<py::gil_scoped_release>())
+        .def("name", &DensityMatrixSimulator::name);
+
     py::class_<ParametricCircuit>(m, "ParametricCircuit")
         .def(py::init<std::size_t, std::size_t>(), "num_qubits"_a, "num_clbits"_a = 0)
         .def("x", &ParametricCircuit::x, ri).def("y", &ParametricCircuit::y, ri)

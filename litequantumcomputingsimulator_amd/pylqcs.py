@@ -14,6 +14,8 @@ Result = _native.Result
 KrausChannel = _native.KrausChannel
 NoiseModel = _native.NoiseModel
 StatevectorSimulator = _native.StatevectorSimulator
+DensityMatrix = _native.DensityMatrix
+DensityMatrixSimulator = _native.DensityMatrixSimulator
 ParametricCircuit = _native.ParametricCircuit
 algorithms = _native.algorithms
 device_info = _native.device_info
@@ -40,7 +42,7 @@ last_plan_choice = _native.last_plan_choice
 
 __all__ = [
     "GateType", "Gate", "QuantumCircuit", "Statevector", "Result", "KrausChannel",
-    "NoiseModel", "StatevectorSimulator", "ParametricCircuit", "algorithms",
+    "NoiseModel", "StatevectorSimulator", "DensityMatrix", "DensityMatrixSimulator", "ParametricCircuit", "algorithms",
     "device_info", "last_plan_stats", "MAX_DEVICE_QUBITS", "CompiledCircuit",
     "statevector", "synchronize", "profile_enable", "profile_read", "set_tile_kernel", "ExpectationFunction",
     "expectation", "profile_read_launches", "COST_FEATURES", "mlp_num_params", "mlp_forward", "mlp_loss_grad",
