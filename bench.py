@@ -16,9 +16,10 @@ parameter-shift gradients on the same GPU (timed on a bounded sample). Two
 more objects report the noisy-trajectory workload (24 qubits, 2048 shots,
 trajectories/s) and the learned planner (ms per step of the cost model's plan
 next to greedy's, on the headline workload and on random U3/CX/CP layers of
-depth 40 at 32 qubits, with the two final states compared). A last object
-times fused dense blocks of 5 and 6 qubits (complex GEMMs on FP64 MFMA) at
-30 qubits against the FP64 peak, with the reference beside them.
+depth 40 at 32 qubits, with the two final states compared). Further objects
+time fused dense blocks of 5 and 6 qubits (complex GEMMs on FP64 MFMA) at
+30 qubits against the FP64 peak, and the noisy workload's density matrix at
+12 and 14 qubits, each with the reference beside it.
 
 `traffic` (HBM bytes per tiled pass) comes from the rocprofv3 counter summary
 committed under profiles/ for the same kernel and workload, when one exists.
@@ -180,6 +181,68 @@ def trajectories_bench(lq, W, seed: int, n: int, shots: int) -> dict:
             "trajectory_gates_per_s": round(shots * gates / dt, 1), "distinct_outcomes": len(dict(counts))}
 
 
+def density_matrix_bench(lq, W, seed: int, widths=(12, 14, 16), cpu: bool = True, cpu_width: int = 10) -> dict:
+    """The noisy workload's exact form: rho of GHZ prep + 10 random U3/CZ
+    layers with depolarizing(1e-3) then amplitude_damping(1e-3) after every
+    gate, on the device (a 2n-qubit register, tiled passes), at the
+    reference's 12-qubit cross-check width and beyond its 13-qubit cap. The
+    reference's DensityMatrixSimulator runs the same circuit on the host cores
+    at `cpu_width` qubits, scaled by 4^(12 - cpu_width) (its passes are linear
+    in 4^n)."""
+    noise = lq.NoiseModel()
+    noise.add_all_qubit_channel(lq.KrausChannel.depolarizing(1e-3))
+    noise.add_all_qubit_channel(lq.KrausChannel.amplitude_damping(1e-3))
+    out = {"workload": "rho of GHZ prep + 10 random U3/CZ layers, depolarizing(1e-3) + amplitude_damping(1e-3) "
+                       "after every gate"}
+    for n in widths:
+        qc = lq.QuantumCircuit(n, n)
+        W.noisy_ghz_layers(qc, n, 10, seed)
+        sim = lq.DensityMatrixSimulator(noise=noise, max_qubits=16)
+        sim.run_density_matrix(qc)  # warm-up
+        lq.synchronize()
+        lq.profile_enable(True)
+        lq.profile_read()
+        t0 = time.perf_counter()
+        rho = sim.run_density_matrix(qc)
+        lq.synchronize()
+        dt = time.perf_counter() - t0
+        prof = lq.profile_read()
+        lq.profile_enable(False)
+        passes = prof["tile_launches"] + prof["other_launches"]
+        kms = prof["tile_ms"] + prof["other_ms"]
+        out[f"n{n}"] = {"seconds": round(dt, 4), "gates": qc.num_instructions(), "launches": passes,
+                        "kernel_ms": round(kms, 3), "trace": rho.trace(),
+                        "GB/s": round(passes * 32 * 4**n / (kms * 1e6), 1) if kms else None}
+        log(f"density matrix {n} qubits: {dt:.3f} s, {passes} launches, {kms:.1f} ms of kernels")
+        del rho
+    if cpu:
+        try:
+            from oracle import oracle as O
+            c = O.Circuit(cpu_width, cpu_width)
+            W.noisy_ghz_layers(c, cpu_width, 10, seed)
+            chans = [O.depolarizing(1e-3), O.amplitude_damping(1e-3)]
+            kind = "reference" if O.ref_available() else "port"
+            cores = len(os.sched_getaffinity(0))
+            t0 = time.perf_counter()
+            if kind == "reference":
+                O.ref_set_threads(cores)
+                O.ref_density_probabilities(c, chans)
+            else:
+                O.density_probabilities(c, chans)
+            dt = time.perf_counter() - t0
+            scaled = dt * 4 ** (12 - cpu_width)
+            out["cpu_baseline"] = {"value": round(scaled, 3), "unit": "seconds for rho at 12 qubits", "cores": cores,
+                                   "kind": kind,
+                                   "sample": f"run_density_matrix of the same generator at {cpu_width} qubits "
+                                             f"({dt:.2f} s), scaled by 4^{12 - cpu_width}"}
+            if "n12" in out:
+                out["n12"]["speedup_vs_cpu"] = round(scaled / out["n12"]["seconds"], 1)
+            log(f"density matrix cpu baseline: {dt:.2f} s at {cpu_width} qubits ({kind})")
+        except Exception as e:  # noqa: BLE001
+            log(f"density matrix cpu baseline unavailable: {e}")
+    return out
+
+
 def planner_32q(lq, W, seed: int, n: int = 32, depth: int = 40) -> dict:
     """The learned-planner workload: random U3/CX/CP layers of depth 40 at 32
     qubits (64 GiB per register); one apply with greedy grouping and one with
@@ -319,6 +382,8 @@ def main() -> int:
     ap.add_argument("--traj-qubits", type=int, default=24)
     ap.add_argument("--traj-shots", type=int, default=2048)
     ap.add_argument("--no-trajectories", action="store_true")
+    ap.add_argument("--no-density-matrix", action="store_true",
+                    help="skip the density-matrix workload (12, 14 and 16 qubits)")
     ap.add_argument("--no-dense", action="store_true",
                     help="skip the dense-block workload (k = 5, 6 targets on FP64 MFMA)")
     ap.add_argument("--no-planner", action="store_true",
@@ -424,6 +489,9 @@ def main() -> int:
         trajectories = trajectories_bench(lq, W, args.seed, args.traj_qubits, args.traj_shots)
     if planner is not None and not args.no_planner_32q:
         planner["at_32_qubits"] = planner_32q(lq, W, args.seed)
+    dm = None
+    if not args.no_density_matrix:
+        dm = density_matrix_bench(lq, W, args.seed, cpu=not args.no_cpu_baseline)
     dense = None
     if not args.no_dense:
         dense = dense_blocks_bench(lq, args.seed, cpu=not args.no_cpu_baseline)
@@ -455,6 +523,7 @@ def main() -> int:
         "trajectories": trajectories,
         "planner": planner,
         "dense_blocks": dense,
+        "density_matrix": dm,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -43,9 +43,22 @@ std::vector<complex_t> superoperator(const KrausChannel& ch) {
     return s;
 }
 
-void add_unitary(detail::ProgramBuilder& b, const Instruction& inst, std::size_t n) {
+// 4x4 product a * b (row-major).
+std::vector<complex_t> mul4(const std::vector<complex_t>& a, const std::vector<complex_t>& b) {
+    std::vector<complex_t> c(16, complex_t{0, 0});
+    for (int i = 0; i < 4; ++i)
+        for (int k = 0; k < 4; ++k)
+            for (int j = 0; j < 4; ++j) c[4 * i + j] += a[4 * i + k] * b[4 * k + j];
+    return c;
+}
+
+void check_qubits(const Instruction& inst, std::size_t n) {
     for (qubit_t q : inst.qubits)
         if (q >= n) throw std::out_of_range("DensityMatrix: qubit index out of range");
+}
+
+void add_unitary(detail::ProgramBuilder& b, const Instruction& inst, std::size_t n) {
+    check_qubits(inst, n);
     b.add_instruction(inst);                                  // ket side: U
     b.add_instruction_conj(inst, static_cast<qubit_t>(n));    // bra side: conj(U)
 }
@@ -156,18 +169,39 @@ double DensityMatrix::fidelity(const Statevector& psi) const {
 // ---- DensityMatrixSimulator ----------------------------------------------------------------
 // density_matrix_simulator.cpp:13-28: Reset rejected, Measure/Barrier skipped,
 // every channel on every touched qubit after each unitary, channel-major. The
-// whole circuit becomes one op stream, planned into tiled passes.
+// whole circuit becomes one op stream, planned into tiled passes; with noise,
+// a single-qubit gate and its channels fold into one superoperator.
 DensityMatrix DensityMatrixSimulator::run_density_matrix(const QuantumCircuit& circuit) const {
     const std::size_t n = circuit.num_qubits();
     DensityMatrix rho(n, options_.max_qubits);
+    // The channels act on disjoint (ket, bra) pairs for different qubits, so
+    // their channel-major order reduces to one composed superoperator per
+    // touched qubit: S = S_last ... S_first.
+    const auto& channels = options_.noise.channels();
+    std::vector<complex_t> chain;
+    for (const KrausChannel& ch : channels) chain = chain.empty() ? superoperator(ch) : mul4(superoperator(ch), chain);
     detail::ProgramBuilder b;
     for (const Instruction& inst : circuit.instructions()) {
         if (inst.gate.type == GateType::Reset)
             throw std::invalid_argument("DensityMatrixSimulator: Reset is not supported yet");
-        if (!inst.gate.is_unitary()) continue;
+        const Gate& g = inst.gate;
+        if (!g.is_unitary()) continue;
+        if (!chain.empty() && g.n_controls == 0 && g.base_qubits() == 1 && g.type != GateType::Permutation) {
+            // A noisy single-qubit gate is one 4x4 op on (q, q + n):
+            // S * (U (x) conj(U)), the superoperator of U followed by the channels.
+            check_qubits(inst, n);
+            const std::vector<complex_t> u = g.base_matrix();
+            std::vector<complex_t> m(16);
+            for (int l = 0; l < 4; ++l)
+                for (int lp = 0; lp < 4; ++lp)
+                    m[4 * l + lp] = u[2 * (l & 1) + (lp & 1)] * std::conj(u[2 * (l >> 1) + (lp >> 1)]);
+            const qubit_t q = inst.qubits[0];
+            b.add_two_qubit(q, static_cast<qubit_t>(q + n), mul4(chain, m));
+            continue;
+        }
         add_unitary(b, inst, n);
-        for (const KrausChannel& ch : options_.noise.channels())
-            for (qubit_t q : inst.qubits) add_channel(b, ch, q, n);
+        if (!chain.empty())
+            for (qubit_t q : inst.qubits) b.add_two_qubit(q, static_cast<qubit_t>(q + n), chain);
     }
     apply_program(rho.device(), b, options_.engine);
     return rho;
