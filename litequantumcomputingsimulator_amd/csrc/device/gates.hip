@@ -299,19 +299,77 @@ __global__ void __launch_bounds__(kBlock) k_cu_mfma(amp_t* __restrict__ s, uint3
 }
 
 // :169-189 -- |l> -> |perm[l]> on the control subspace, staged through LDS.
+// A round stages max(D, kPermSlots) amplitudes: kPermSlots / D whole groups
+// when D is small, so that every thread has work; consecutive threads walk l
+// within a group (contiguous when the targets are the low qubits).
+constexpr uint32_t kPermSlots = 2048;
+
 __global__ void __launch_bounds__(kBlock) k_cperm(amp_t* __restrict__ s, uint32_t nq, Layout L,
                                                   const uint64_t* __restrict__ offs,
                                                   const uint64_t* __restrict__ perm) {
+    // Dynamic LDS: the round's amplitudes, then (several groups per round)
+    // the first index of each group, ~0 past the last group.
     extern __shared__ amp_t grp[];
-    const uint32_t D = 1u << L.nt;
+    const uint32_t log_d = L.nt, D = 1u << log_d;
+    const uint32_t log_g = log_d < 11 ? 11 - log_d : 0, per_round = D << log_g;
+    uint64_t* gbase = reinterpret_cast<uint64_t*>(grp + per_round);
     amp_t* b = batch_base(s, nq);
     const uint64_t groups = uint64_t{1} << (nq - L.n_fixed);
-    for (uint64_t g = blockIdx.x; g < groups; g += gridDim.x) {
-        const uint64_t base = expand(g, L);
-        for (uint32_t l = threadIdx.x; l < D; l += blockDim.x) grp[l] = ld(b + (base | offs[l]));
+    const uint64_t rounds = (groups + (uint64_t{1} << log_g) - 1) >> log_g;
+    for (uint64_t r = blockIdx.x; r < rounds; r += gridDim.x) {
+        const uint64_t g0 = r << log_g;
+        uint64_t only = 0;  // one group per round: its first index
+        if (log_g == 0) {
+            only = expand(g0, L);
+        } else {
+            for (uint32_t gi = threadIdx.x; gi < (1u << log_g); gi += kBlock)
+                gbase[gi] = g0 + gi < groups ? expand(g0 + gi, L) : ~uint64_t{0};
+            __syncthreads();
+        }
+        for (uint32_t e = threadIdx.x; e < per_round; e += kBlock) {
+            const uint64_t base = log_g == 0 ? only : gbase[e >> log_d];
+            if (base != ~uint64_t{0}) grp[e] = ld(b + (base | offs[e & (D - 1)]));
+        }
         __syncthreads();
-        for (uint32_t l = threadIdx.x; l < D; l += blockDim.x) st(b + (base | offs[perm[l]]), grp[l]);
-        __syncthreads();  // grp is reused by the next group
+        for (uint32_t e = threadIdx.x; e < per_round; e += kBlock) {
+            const uint64_t base = log_g == 0 ? only : gbase[e >> log_d];
+            if (base != ~uint64_t{0}) st(b + (base | offs[perm[e & (D - 1)]]), grp[e]);
+        }
+        __syncthreads();  // grp and gbase are reused by the next round
+    }
+}
+
+// The same by cycles, for target and control sets that leave the lowest
+// qubits free: one thread per group, so consecutive threads touch consecutive
+// amplitudes. Each thread rotates every cycle of the permutation in place,
+// c_0 -> c_1 -> ... (the amplitude at c_i moves to c_{i+1} = perm[c_i]),
+// reading 16 cycle entries ahead of its writes; fixed points are not touched.
+// cyc holds the cycles back to back, starts[k]..starts[k+1] the k-th.
+__global__ void __launch_bounds__(kBlock) k_cperm_cycles(amp_t* __restrict__ s, uint32_t nq, Layout L,
+                                                         const uint64_t* __restrict__ offs,
+                                                         const uint32_t* __restrict__ cyc,
+                                                         const uint32_t* __restrict__ starts, uint32_t n_cycles) {
+    constexpr int kAhead = 16;
+    amp_t* b = batch_base(s, nq);
+    const uint64_t groups = uint64_t{1} << (nq - L.n_fixed);
+    for (uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x; g < groups; g += uint64_t(gridDim.x) * kBlock) {
+        amp_t* base = b + expand(g, L);
+        for (uint32_t k = 0; k < n_cycles; ++k) {
+            const uint32_t first = starts[k], len = starts[k + 1] - first;
+            amp_t carry = ld(base + offs[cyc[first + len - 1]]);  // moves to c_0
+            for (uint32_t i = 0; i < len; i += kAhead) {
+                amp_t x[kAhead];
+#pragma unroll
+                for (int j = 0; j < kAhead; ++j)
+                    if (i + j < len) x[j] = ld(base + offs[cyc[first + i + j]]);
+#pragma unroll
+                for (int j = 0; j < kAhead; ++j)
+                    if (i + j < len) {
+                        st(base + offs[cyc[first + i + j]], carry);
+                        carry = x[j];
+                    }
+            }
+        }
     }
 }
 
@@ -481,9 +539,33 @@ void launch_controlled_permutation(amp_t* s, uint32_t nq, const uint32_t* ctl, u
     if (D * sizeof(amp_t) > 64 * 1024) throw Error(3, "permutation wider than the engine supports");
     const uint64_t groups = uint64_t{1} << (nq - L.n_fixed);
     const uint64_t* d_off = stage(off.data(), off.size());
+    // Lowest qubits free and enough groups for one thread each: rotate cycles.
+    if (L.n_fixed > 0 && L.fixed[0] >= 4 && groups >= uint64_t(kBlock)) {
+        std::vector<uint32_t> cyc, starts{0};
+        std::vector<uint8_t> seen(D, 0);
+        for (uint64_t l = 0; l < D; ++l) {
+            if (seen[l] || perm_host[l] == l) continue;
+            for (uint64_t c = l; !seen[c]; c = perm_host[c]) {
+                seen[c] = 1;
+                cyc.push_back(static_cast<uint32_t>(c));
+            }
+            starts.push_back(static_cast<uint32_t>(cyc.size()));
+        }
+        const uint32_t n_cycles = static_cast<uint32_t>(starts.size() - 1);
+        if (n_cycles == 0) return;  // the identity
+        const uint32_t* d_cyc = stage(cyc.data(), cyc.size());
+        const uint32_t* d_starts = stage(starts.data(), starts.size());
+        k_cperm_cycles<<<dim3(grid_capped(groups), batch), kBlock, 0, stream()>>>(s, nq, L, d_off, d_cyc, d_starts,
+                                                                                 n_cycles);
+        launched();
+        return;
+    }
     const uint64_t* d_perm = stage(perm_host, D);
-    k_cperm<<<dim3(capped(groups), batch), kBlock, D * sizeof(amp_t), stream()>>>(s, nq, L, d_off,
-                                                                                   d_perm);
+    const uint64_t per_round = std::max<uint64_t>(D, kPermSlots);
+    const uint64_t groups_per_round = per_round / D;
+    const uint64_t rounds = (groups + groups_per_round - 1) / groups_per_round;
+    const size_t lds = per_round * sizeof(amp_t) + (groups_per_round > 1 ? groups_per_round * sizeof(uint64_t) : 0);
+    k_cperm<<<dim3(capped(rounds), batch), kBlock, lds, stream()>>>(s, nq, L, d_off, d_perm);
     launched();
 }
 
