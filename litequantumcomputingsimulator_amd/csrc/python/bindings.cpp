@@ -285,6 +285,11 @@ PYBIND11_MODULE(_pylqcs, m) {
              "circuit"_a, "shots"_a = 1024)
         .def("name", &StatevectorSimulator::name);
 
+    // OpenQASM 2.0 (reference python/src/bindings.cpp:410-413).
+    auto io_mod = m.def_submodule("io", "OpenQASM import/export");
+    io_mod.def("to_qasm", &io::to_qasm, "circuit"_a);
+    io_mod.def("from_qasm", [](const std::string& src) { return io::from_qasm(src); }, "source"_a);
+
     // Density-matrix backend (reference python/src/bindings.cpp:416-443), on the device.
     py::class_<DensityMatrix>(m, "DensityMatrix")
         .def(py::init<std::size_t>(), "num_qubits"_a)

@@ -18,6 +18,7 @@ DensityMatrix = _native.DensityMatrix
 DensityMatrixSimulator = _native.DensityMatrixSimulator
 ParametricCircuit = _native.ParametricCircuit
 algorithms = _native.algorithms
+io = _native.io
 device_info = _native.device_info
 last_plan_stats = _native.last_plan_stats
 MAX_DEVICE_QUBITS = _native.MAX_DEVICE_QUBITS
@@ -42,7 +43,7 @@ last_plan_choice = _native.last_plan_choice
 
 __all__ = [
     "GateType", "Gate", "QuantumCircuit", "Statevector", "Result", "KrausChannel",
-    "NoiseModel", "StatevectorSimulator", "DensityMatrix", "DensityMatrixSimulator", "ParametricCircuit", "algorithms",
+    "NoiseModel", "StatevectorSimulator", "DensityMatrix", "DensityMatrixSimulator", "ParametricCircuit", "algorithms", "io",
     "device_info", "last_plan_stats", "MAX_DEVICE_QUBITS", "CompiledCircuit",
     "statevector", "synchronize", "profile_enable", "profile_read", "set_tile_kernel", "ExpectationFunction",
     "expectation", "profile_read_launches", "COST_FEATURES", "mlp_num_params", "mlp_forward", "mlp_loss_grad",
