@@ -20,6 +20,9 @@ struct EngineOptions {
     // Tiled-pass kernel: 0 engine default, 1 LDS tile, 2 register tile,
     // 3 register tile streaming, 4 register tile with an LDS landing buffer.
     int tile_kernel = 0;
+    // Remove adjacent inverse pairs (RedundantGateElimination) before fusion
+    // and planning. Off by default: the reference's simulator applies fusion only.
+    bool eliminate_redundant = false;
 };
 
 }  // namespace lqcs

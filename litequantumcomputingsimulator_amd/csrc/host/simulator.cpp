@@ -15,6 +15,7 @@
 #include "lqcs/backend/statevector_simulator.hpp"
 #include "lqcs/core/random.hpp"
 #include "lqcs/transpiler/gate_fusion.hpp"
+#include "lqcs/transpiler/redundant_elimination.hpp"
 
 namespace lqcs {
 
@@ -173,8 +174,9 @@ std::string key_of(const std::vector<std::optional<qubit_t>>& src, uint64_t idx)
 }  // namespace
 
 QuantumCircuit StatevectorSimulator::maybe_fuse(const QuantumCircuit& circuit) const {
-    if (!options_.fuse_gates) return circuit;
-    return SingleQubitGateFusion().run(circuit);
+    const QuantumCircuit c = options_.engine.eliminate_redundant ? RedundantGateElimination().run(circuit) : circuit;
+    if (!options_.fuse_gates) return c;
+    return SingleQubitGateFusion().run(c);
 }
 
 Statevector StatevectorSimulator::run_statevector(const QuantumCircuit& circuit) const {

@@ -252,8 +252,9 @@ PYBIND11_MODULE(_pylqcs, m) {
     py::class_<StatevectorSimulator>(m, "StatevectorSimulator")
         .def(py::init([](std::optional<std::uint64_t> seed, bool fuse_gates, int num_threads, NoiseModel noise,
                          const std::string& plan, int tile_qubits, int low_qubits, std::size_t max_qubits,
-                         std::uint32_t max_batch) {
+                         std::uint32_t max_batch, bool eliminate_redundant) {
                  StatevectorSimulator::Options o;
+                 o.engine.eliminate_redundant = eliminate_redundant;
                  o.seed = seed;
                  o.fuse_gates = fuse_gates;
                  o.num_threads = num_threads;
@@ -266,7 +267,8 @@ PYBIND11_MODULE(_pylqcs, m) {
                  return StatevectorSimulator(std::move(o));
              }),
              "seed"_a = py::none(), "fuse_gates"_a = true, "num_threads"_a = 0, "noise"_a = NoiseModel{},
-             "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0, "max_qubits"_a = 30, "max_batch"_a = 0)
+             "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0, "max_qubits"_a = 30, "max_batch"_a = 0,
+             "eliminate_redundant"_a = false)
         .def("run", &StatevectorSimulator::run, "circuit"_a, "shots"_a = 1024,
              py::call_guard<py::gil_scoped_release>())
         .def("run_statevector", &StatevectorSimulator::run_statevector, "circuit"_a,
@@ -284,6 +286,16 @@ PYBIND11_MODULE(_pylqcs, m) {
              },
              "circuit"_a, "shots"_a = 1024)
         .def("name", &StatevectorSimulator::name);
+
+    // Circuit passes (C++ API of the reference's include/lqcs/transpiler/*.hpp).
+    py::class_<SingleQubitGateFusion>(m, "SingleQubitGateFusion")
+        .def(py::init<>())
+        .def("run", &SingleQubitGateFusion::run, "circuit"_a)
+        .def("name", &SingleQubitGateFusion::name);
+    py::class_<RedundantGateElimination>(m, "RedundantGateElimination")
+        .def(py::init<>())
+        .def("run", &RedundantGateElimination::run, "circuit"_a)
+        .def("name", &RedundantGateElimination::name);
 
     // OpenQASM 2.0 (reference python/src/bindings.cpp:410-413).
     auto io_mod = m.def_submodule("io", "OpenQASM import/export");

@@ -19,6 +19,8 @@ DensityMatrixSimulator = _native.DensityMatrixSimulator
 ParametricCircuit = _native.ParametricCircuit
 algorithms = _native.algorithms
 io = _native.io
+SingleQubitGateFusion = _native.SingleQubitGateFusion
+RedundantGateElimination = _native.RedundantGateElimination
 device_info = _native.device_info
 last_plan_stats = _native.last_plan_stats
 MAX_DEVICE_QUBITS = _native.MAX_DEVICE_QUBITS
@@ -43,7 +45,8 @@ last_plan_choice = _native.last_plan_choice
 
 __all__ = [
     "GateType", "Gate", "QuantumCircuit", "Statevector", "Result", "KrausChannel",
-    "NoiseModel", "StatevectorSimulator", "DensityMatrix", "DensityMatrixSimulator", "ParametricCircuit", "algorithms", "io",
+    "NoiseModel", "StatevectorSimulator", "DensityMatrix", "DensityMatrixSimulator", "ParametricCircuit", "algorithms", "io", "SingleQubitGateFusion",
+    "RedundantGateElimination",
     "device_info", "last_plan_stats", "MAX_DEVICE_QUBITS", "CompiledCircuit",
     "statevector", "synchronize", "profile_enable", "profile_read", "set_tile_kernel", "ExpectationFunction",
     "expectation", "profile_read_launches", "COST_FEATURES", "mlp_num_params", "mlp_forward", "mlp_loss_grad",
