@@ -100,7 +100,7 @@ def measured_traffic(kernel: str, n: int, layers: int, plan: str, tile: int, low
     return kib * 1024.0, f"profiles/{name}"
 
 
-def adjoint_bench(lq, W, seed: int, n_circuits: int, ps_seconds: float) -> dict:
+def adjoint_bench(lq, W, seed: int, n_circuits: int, ps_seconds: float, cpu: bool = True) -> dict:
     """Batched adjoint gradients of n_circuits Pauli-rotation ansatze of uneven
     width (16-24 qubits, 400 rotations each, 200-term Hamiltonians) against
     the parameter-shift gradient, both on this GPU. Parameter shift is timed
@@ -142,21 +142,72 @@ def adjoint_bench(lq, W, seed: int, n_circuits: int, ps_seconds: float) -> dict:
     err = max(float(np.max(np.abs(np.array(g) - np.array(p)))) for (_, g), p in zip(same, ps))
     err_full = max(float(np.max(np.abs(np.array(g) - np.array(h)))) for (_, g), (_, h) in zip(same, full[:k]))
     sample_grads = sum(len(t) for t in thetas[:k])
-    return {
+    out = {
         "workload": (f"{n_circuits} Pauli-rotation ansatze, 16-24 qubits (uneven), 400 rotations of weight 2-8, "
                      "200-term Hamiltonians, theta ~ U(-pi, pi)"),
-        "adjoint_grads_per_s": round(n_grads / t_adj, 1),
+        "adjoint_gradient_vectors_per_s": round(n_circuits / t_adj, 2),
+        "adjoint_components_per_s": round(n_grads / t_adj, 1),
         "adjoint_s": round(t_adj, 3),
-        "param_shift_grads_per_s": round(sample_grads / t_ps, 2),
-        "param_shift_sample": f"first {k} circuit(s) of the batch, widths {sizes[:k]}, {sample_grads} gradients",
-        "adjoint_grads_per_s_on_sample": round(sample_grads / t_adj_sample, 1),
+        "param_shift_gradient_vectors_per_s": round(k / t_ps, 3),
+        "param_shift_components_per_s": round(sample_grads / t_ps, 2),
+        "param_shift_sample": f"first {k} circuit(s) of the batch, widths {sizes[:k]}, {sample_grads} components",
+        "adjoint_components_per_s_on_sample": round(sample_grads / t_adj_sample, 1),
         "speedup_on_sample": round((sample_grads / t_adj_sample) / (sample_grads / t_ps), 1),
         "max_abs_diff_vs_param_shift": err,
         "max_abs_diff_batch_vs_sample_run": err_full,
     }
+    if cpu:
+        out["cpu_baseline"] = _ps_cpu_baseline(W, sizes, seed)
+    return out
 
 
-def trajectories_bench(lq, W, seed: int, n: int, shots: int) -> dict:
+def _ps_cpu_baseline(W, sizes, seed: int, budget_s: float = 30.0) -> dict | None:
+    """The reference's parameter_shift_gradient (vqe.cpp:68-94, OpenMP on the
+    host cores) for the batch's first 16-qubit circuit: 2 x 400 evaluations.
+    Two evaluations (run_statevector + expectation) are timed first; when the
+    full gradient would exceed budget_s, those stand in, scaled by 400."""
+    try:
+        from oracle import oracle as O
+    except Exception as e:  # noqa: BLE001
+        log(f"parameter-shift cpu baseline unavailable: {e}")
+        return None
+    i = next((j for j, n in enumerate(sizes) if n == 16), 0)
+    n = sizes[i]
+    rots, theta = W.random_pauli_ansatz(n, 400, seed + 1 + i)
+    h = W.random_hamiltonian(n, 200, seed + 100_000 + i)
+    t = O.Circuit(n)
+    for (p, qs), th in zip(rots, theta):
+        t.rp(float(th), p, qs)
+    ref = O.ref_available()
+    kind = "reference" if ref else "port"
+    cores = len(os.sched_getaffinity(0))
+    if ref:
+        O.ref_set_threads(cores)
+    run = O.ref_run_statevector if ref else O.run_statevector
+    expect = O.ref_expectation if ref else (lambda st, nq, terms: O.expectation(st, nq, terms))
+    t0 = time.perf_counter()
+    for _ in range(2):
+        expect(run(t, fuse=False), n, h)
+    t_two = time.perf_counter() - t0
+    if 400 * t_two <= budget_s:
+        theta_l = [float(v) for v in theta]
+        t0 = time.perf_counter()
+        if ref:
+            O.ref_parameter_shift_gradient(t, list(range(400)), theta_l, h)
+        else:
+            O.parameter_shift_gradient(t, list(range(400)), theta_l, h)
+        per_vector = time.perf_counter() - t0
+        sample = f"parameter_shift_gradient of circuit {i} ({n} qubits, 400 rotations, 200 terms): {per_vector:.2f} s"
+    else:
+        per_vector = 400 * t_two
+        sample = (f"2 evaluations (run_statevector + 200-term expectation) of circuit {i} ({n} qubits, 400 "
+                  f"rotations): {t_two:.3f} s, scaled by 400 to the 800 evaluations of one gradient")
+    log(f"parameter-shift cpu baseline: {per_vector:.2f} s per gradient vector at {n} qubits ({kind})")
+    return {"value": round(1.0 / per_vector, 5), "unit": "gradient vectors/s (400 parameters, one circuit)",
+            "cores": cores, "kind": kind, "sample": sample}
+
+
+def trajectories_bench(lq, W, seed: int, n: int, shots: int, cpu: bool = True, cpu_shots: int = 1) -> dict:
     """The noisy-trajectory workload: GHZ prep + 10 random U3/CZ layers at n
     qubits, depolarizing(1e-3) then amplitude_damping(1e-3) after every gate,
     measure_all, `shots` trajectories in batched launches."""
@@ -175,10 +226,33 @@ def trajectories_bench(lq, W, seed: int, n: int, shots: int) -> dict:
     dt = time.perf_counter() - t0
     gates = qc.num_instructions() - n
     log(f"trajectories: {shots} shots of {n} qubits in {dt:.3f} s")
-    return {"workload": (f"GHZ prep + 10 random U3/CZ layers, {n} qubits, {gates} gates, depolarizing(1e-3) + "
-                         f"amplitude_damping(1e-3) after every gate, measure_all"),
-            "shots": shots, "seconds": round(dt, 3), "trajectories_per_s": round(shots / dt, 2),
-            "trajectory_gates_per_s": round(shots * gates / dt, 1), "distinct_outcomes": len(dict(counts))}
+    out = {"workload": (f"GHZ prep + 10 random U3/CZ layers, {n} qubits, {gates} gates, depolarizing(1e-3) + "
+                        f"amplitude_damping(1e-3) after every gate, measure_all"),
+           "shots": shots, "seconds": round(dt, 3), "trajectories_per_s": round(shots / dt, 2),
+           "trajectory_gates_per_s": round(shots * gates / dt, 1), "distinct_outcomes": len(dict(counts))}
+    if cpu:
+        try:
+            from oracle import oracle as O
+            c = O.Circuit(n, n)
+            W.noisy_ghz_layers(c, n, 10, seed)
+            c.measure_all()
+            chans = [O.depolarizing(1e-3), O.amplitude_damping(1e-3)]
+            kind = "reference" if O.ref_available() else "port"
+            cores = len(os.sched_getaffinity(0))
+            t0 = time.perf_counter()
+            if kind == "reference":
+                O.ref_set_threads(cores)
+                O.ref_run_counts(c, seed, cpu_shots, channels=chans)
+            else:
+                O.run_counts(c, seed, cpu_shots, channels=chans)
+            dtc = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": round(cpu_shots / dtc, 4), "unit": "trajectories/s", "cores": cores,
+                                   "kind": kind,
+                                   "sample": f"run() of {cpu_shots} shot(s) of the same circuit and noise, {dtc:.2f} s"}
+            log(f"trajectories cpu baseline: {cpu_shots} shot(s) in {dtc:.2f} s ({kind})")
+        except Exception as e:  # noqa: BLE001
+            log(f"trajectories cpu baseline unavailable: {e}")
+    return out
 
 
 def density_matrix_bench(lq, W, seed: int, widths=(12, 14, 16), cpu: bool = True, cpu_width: int = 10) -> dict:
@@ -484,9 +558,10 @@ def main() -> int:
     trajectories = None
     del sv
     if not args.no_adjoint:
-        adjoint = adjoint_bench(lq, W, args.seed, args.adjoint_circuits, args.ps_seconds)
+        adjoint = adjoint_bench(lq, W, args.seed, args.adjoint_circuits, args.ps_seconds, cpu=not args.no_cpu_baseline)
     if not args.no_trajectories:
-        trajectories = trajectories_bench(lq, W, args.seed, args.traj_qubits, args.traj_shots)
+        trajectories = trajectories_bench(lq, W, args.seed, args.traj_qubits, args.traj_shots,
+                                          cpu=not args.no_cpu_baseline)
     if planner is not None and not args.no_planner_32q:
         planner["at_32_qubits"] = planner_32q(lq, W, args.seed)
     dm = None
