@@ -137,6 +137,98 @@ __global__ void __launch_bounds__(kBlock) k_collapse_stats(amp_t* __restrict__ s
     }
 }
 
+// Terminal measurements (every step after them is a measurement too): the
+// state is not needed afterwards, so instead of collapsing it, the sums for
+// each outcome run over the amplitudes that agree with the member's outcomes
+// so far (fixed qubits F, values V[member]), split by the measured qubit q.
+// p1 = S1 / (S0 + S1) is the collapsed state's probability up to rounding:
+// collapse and renormalisation only zero the other amplitudes and scale the
+// kept ones by one factor. When F is the low qubits 0..k-1 (measure_all), the
+// amplitudes read are (g << k) | V.
+__device__ __forceinline__ double tsum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_term_partial(const amp_t* __restrict__ s, uint32_t nq, uint64_t fixed,
+                                                         const uint64_t* __restrict__ vals, uint32_t q,
+                                                         double* __restrict__ part) {
+    const uint32_t r = blockIdx.y;
+    const amp_t* b = s + (uint64_t(r) << nq);
+    const uint64_t v = vals[r], full = (uint64_t{1} << nq) - 1, free_bits = full & ~fixed;
+    const uint32_t k = static_cast<uint32_t>(__popcll(fixed));
+    const bool prefix = fixed == ((uint64_t{1} << k) - 1);
+    const uint64_t n = uint64_t{1} << (nq - k), qbit = uint64_t{1} << q;
+    double acc0 = 0.0, acc1 = 0.0;
+    for (uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x; g < n; g += uint64_t(gridDim.x) * kBlock) {
+        uint64_t i;
+        if (prefix) {
+            i = (g << k) | v;
+        } else {
+            i = v;
+            uint64_t m = free_bits;
+            for (uint64_t bit = 1; m; bit <<= 1) {
+                const uint64_t low = m & (~m + 1);
+                if (g & bit) i |= low;
+                m ^= low;
+            }
+        }
+        const double w = cnorm(b[i]);
+        if (i & qbit) acc1 += w; else acc0 += w;
+    }
+    __shared__ double red[2][kBlock / 64];
+    acc0 = tsum(acc0);
+    acc1 = tsum(acc1);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = acc0;
+        red[1][threadIdx.x >> 6] = acc1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t0 = 0.0, t1 = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            t0 += red[0][w];
+            t1 += red[1][w];
+        }
+        part[2 * (uint64_t(r) * gridDim.x + blockIdx.x)] = t0;
+        part[2 * (uint64_t(r) * gridDim.x + blockIdx.x) + 1] = t1;
+    }
+}
+
+// Sums each member's partials (fixed order), decides the outcome with the
+// member's draw (u < p1) and records it.
+__global__ void __launch_bounds__(kBlock) k_term_decide(const double* __restrict__ part, uint32_t blocks,
+                                                        const double* __restrict__ draws, uint64_t draw_stride,
+                                                        uint64_t draw_offset, uint32_t q, uint64_t* __restrict__ vals,
+                                                        uint8_t* __restrict__ creg, uint32_t creg_stride,
+                                                        uint32_t clbit) {
+    const uint32_t r = blockIdx.x;
+    double a0 = 0.0, a1 = 0.0;
+    for (uint32_t j = threadIdx.x; j < blocks; j += kBlock) {
+        a0 += part[2 * (uint64_t(r) * blocks + j)];
+        a1 += part[2 * (uint64_t(r) * blocks + j) + 1];
+    }
+    __shared__ double red[2][kBlock / 64];
+    a0 = tsum(a0);
+    a1 = tsum(a1);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a0;
+        red[1][threadIdx.x >> 6] = a1;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double s0 = 0.0, s1 = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        s0 += red[0][w];
+        s1 += red[1][w];
+    }
+    const double p1 = s1 / (s0 + s1);
+    const bool one = draws[uint64_t(r) * draw_stride + draw_offset] < p1;
+    creg[uint64_t(r) * creg_stride + clbit] = one ? 1 : 0;
+    vals[r] = one ? (vals[r] | (uint64_t{1} << q)) : (vals[r] & ~(uint64_t{1} << q));
+}
+
 unsigned pass_blocks(uint32_t nq) {
     const uint64_t want = ((uint64_t{1} << (nq - 1)) + kBlock * 4 - 1) / (kBlock * 4);
     return static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(want, 1u << 16)));
@@ -299,6 +391,11 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
         }
         if (d != D) throw Error(LQCS_ERR_INVALID_ARGUMENT, "draws_per_shot does not match the steps");
 
+        // Trailing measurements (nothing but measurements after them) sample
+        // without collapsing (k_term_*).
+        uint64_t term = n_steps;
+        while (term > 0 && steps[term - 1].kind == LQCS_TOP_MEASURE) --term;
+
         // Batch size: at most max_batch and ~1/2 of free HBM; and small enough
         // that the batch's events leave runs of ~25 noise sites between them
         // (every event splits the shared op sequence for the whole batch).
@@ -321,12 +418,15 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
         double* d_stats = dev_array<double>(B * 4);
         uint32_t* d_members = dev_array<uint32_t>(B);
         uint8_t* d_skip = dev_array<uint8_t>(B);
+        const unsigned term_blocks = static_cast<unsigned>(std::min<uint64_t>(pass_blocks(nq), 1024));
+        uint64_t* d_vals = dev_array<uint64_t>(B);
+        double* d_term = dev_array<double>(B * term_blocks * 2);
         struct Free {
             std::vector<void*> p;
             ~Free() {
                 for (void* x : p) device_free(x);
             }
-        } guard{{states, d_creg, d_draws, d_stats, d_members, d_skip}};
+        } guard{{states, d_creg, d_draws, d_stats, d_members, d_skip, d_vals, d_term}};
         std::vector<uint8_t> host_creg(B * creg_w);
         const TileChoice tc = default_tile(nq);
 
@@ -350,6 +450,21 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
                                         hipMemcpyHostToDevice, stream()));
             for (uint64_t i = 0; i < n_steps; ++i) {
                 const lqcs_traj_step& st = steps[i];
+                if (i >= term) {  // a trailing measurement
+                    if (i == term) {
+                        flush();
+                        LQCS_HIP(hipMemsetAsync(d_vals, 0, uint64_t(nb) * sizeof(uint64_t), stream()));
+                    }
+                    uint64_t fixed = 0;
+                    for (uint64_t j = term; j < i; ++j) fixed |= uint64_t{1} << steps[j].qubit;
+                    k_term_partial<<<dim3(term_blocks, nb), kBlock, 0, stream()>>>(states, nq, fixed, d_vals, st.qubit,
+                                                                                  d_term);
+                    LQCS_HIP(hipGetLastError());
+                    k_term_decide<<<nb, kBlock, 0, stream()>>>(d_term, term_blocks, d_draws, D, off[i], st.qubit,
+                                                               d_vals, d_creg, creg_w, st.clbit);
+                    LQCS_HIP(hipGetLastError());
+                    continue;
+                }
                 if (st.kind != LQCS_TOP_UNITARY) {
                     flush();
                     reduce_qstats(states, nq, st.qubit, nb, d_stats);

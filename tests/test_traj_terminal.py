@@ -1,0 +1,68 @@
+"""Trailing measurements of the trajectory path (reference
+statevector_simulator.cpp:156-190, collapse_qubit gate_kernels.cpp:227-247):
+the engine samples them from sums over the amplitudes consistent with the
+outcomes so far instead of collapsing the register; the counts equal the
+oracle's, which collapses, draw for draw. Orders that are not the low-qubit
+prefix, repeated qubits, clbit remapping, and a measurement followed by
+gates (so only the last ones are trailing) are covered."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def noisy_body(n, seed):
+    rng = np.random.default_rng(seed)
+    c = O.Circuit(n, n)
+    for q in range(n):
+        c.u(*(float(v) for v in rng.uniform(0, 2 * np.pi, 3)), q)
+    for q in range(n - 1):
+        c.cx(q, q + 1)
+    for q in range(n):
+        c.ry(float(rng.uniform(0, 3)), q)
+    return c
+
+
+def counts(lq, c, seed, shots, chans):
+    noise = lq.NoiseModel()
+    for name, p in chans:
+        noise.add_all_qubit_channel(getattr(lq.KrausChannel, name)(p))
+    return dict(lq.StatevectorSimulator(seed=seed, noise=noise).run(c.to_product(lq), shots).counts())
+
+
+CH = [("depolarizing", 0.02), ("amplitude_damping", 0.03)]
+
+
+@pytest.mark.parametrize("order", ["prefix", "reversed", "scrambled", "repeat"])
+def test_trailing_measurements_match_oracle(lq, order):
+    n, seed, shots = 9, 17, 300
+    c = noisy_body(n, 5)
+    qs = list(range(n))
+    if order == "reversed":
+        qs = qs[::-1]
+    elif order == "scrambled":
+        qs = [int(q) for q in np.random.default_rng(1).permutation(n)]
+    elif order == "repeat":
+        qs = [0, 3, 0, 5, 3, 8, 1]
+    for k, q in enumerate(qs):
+        c.measure(q, (k * 5) % n)
+    want = O.run_counts(c, seed=seed, shots=shots, channels=[getattr(O, a)(b) for a, b in CH])
+    assert counts(lq, c, seed, shots, CH) == want
+
+
+def test_measurement_then_gates_then_trailing(lq):
+    n, seed, shots = 8, 23, 300
+    c = noisy_body(n, 9)
+    c.measure(2, 0).h(2).cx(2, 5).measure(4, 1)
+    c.rx(0.4, 4).measure(5, 2).measure(4, 3).measure(0, 4)
+    want = O.run_counts(c, seed=seed, shots=shots, channels=[getattr(O, a)(b) for a, b in CH])
+    assert counts(lq, c, seed, shots, CH) == want
+
+
+def test_noiseless_midcircuit_trailing(lq):
+    c = O.Circuit(3, 3)
+    c.h(0).cx(0, 1).measure(0, 0).h(0).t(2).h(2).measure(2, 1).measure(1, 2)
+    want = O.run_counts(c, seed=4, shots=400)
+    assert dict(lq.StatevectorSimulator(seed=4).run(c.to_product(lq), 400).counts()) == want
