@@ -281,6 +281,19 @@ DecodedOp decode(const lqcs_program& p, uint64_t k, uint32_t nq) {
 
 // ---- gate-by-gate execution -------------------------------------------------
 void run_single_op(amp_t* s, uint32_t nq, const DecodedOp& d, uint32_t batch) {
+    if (d.member >= 0) {  // one batch register, or all but one: contiguous ranges
+        DecodedOp all = d;
+        all.member = -1;
+        const uint32_t m = static_cast<uint32_t>(d.member);
+        if (m >= batch) throw Error(LQCS_ERR_INVALID_ARGUMENT, "batch member out of range");
+        if (d.only) {
+            run_single_op(s + (uint64_t(m) << nq), nq, all, 1);
+        } else {
+            if (m > 0) run_single_op(s, nq, all, m);
+            if (m + 1 < batch) run_single_op(s + (uint64_t(m + 1) << nq), nq, all, batch - m - 1);
+        }
+        return;
+    }
     const uint32_t* t = d.targets.data();
     switch (d.kind) {
         case LQCS_OP_SINGLE: {
@@ -360,6 +373,8 @@ static uint32_t to_local_mask(uint64_t smask, uint64_t global) {
 TileOp make_tile_op(const DecodedOp& d, uint64_t smask) {
     TileOp t;
     std::memset(&t, 0, sizeof t);
+    t.member = d.member;
+    t.only = d.only ? 1u : 0u;
     t.lctrl = to_local_mask(smask, d.cmask);
     t.gctrl = d.cmask & ~smask;
     auto set_diag_qubit = [&](int slot, uint32_t q) {

@@ -27,6 +27,10 @@ struct DecodedOp {
     uint32_t n_y = 0;
     double theta = 0.0;
     bool diagonal = false;       // exact zeros off the diagonal
+    // Batch-member restriction (trajectories): member >= 0 with only: that
+    // register alone; without: every register of the batch but that one.
+    int32_t member = -1;
+    bool only = false;
 };
 
 DecodedOp decode(const lqcs_program& p, uint64_t k, uint32_t nq);

@@ -63,7 +63,7 @@ using Regs = amp_t[kRR];
 
 // An op's uniform fields, decoded from its header.
 struct Hdr {
-    uint32_t kind, b0, b1, xr, rctl, tctl, la, lb, gctrl, ga, gb;
+    uint32_t kind, b0, b1, xr, rctl, tctl, la, lb, gctrl, ga, gb, member;
 };
 
 __device__ __forceinline__ uint32_t rfl(uint32_t v) {
@@ -219,7 +219,8 @@ __device__ __forceinline__ void r_pauli_diag(const Regs& a, Regs& o, const Hdr& 
 
 __device__ __forceinline__ void apply_reg_op(const Regs& a, Regs& o, const Hdr& h, const amp_t* m, uint32_t jt,
                                              const RPhase& ph, uint32_t tile) {
-    if ((tile & h.gctrl) != h.gctrl) {  // tile-uniform control not met
+    if ((tile & h.gctrl) != h.gctrl ||  // tile-uniform control not met
+        (h.member && ((h.member & 0x7fffffffu) - 1 == blockIdx.y) != (h.member >> 31 != 0))) {  // batch register
         copy_regs(a, o);
         return;
     }
@@ -333,6 +334,7 @@ __device__ __forceinline__ uint32_t apply_op(const Regs& a, Regs& o, const amp_t
     h.gctrl = rfl(h1.x);
     h.ga = rfl(h1.y);
     h.gb = rfl(h1.z);
+    h.member = rfl(h1.w);
     apply_reg_op(a, o, h, words + off + 2, jt, ph, tile);
     return off + 2 + (code >> 16);
 }
@@ -734,6 +736,7 @@ bool build_rtile(const std::vector<TileOp>& tops, uint32_t L, uint64_t smask, ui
                 if (t.lctrl & (1u << rb[q])) rctl |= 1u << q;
             h.tctl = t.lctrl & ph.tmask;
             h.gctrl = pext_host(t.gctrl, outside);
+            if (t.member >= 0) h.member = (static_cast<uint32_t>(t.member) + 1) | (t.only ? 1u << 31 : 0u);
             switch (t.kind) {
                 case TK_DENSE1:
                     kind = is_x(t) ? RK_X : RK_DENSE1;

@@ -57,6 +57,7 @@ __global__ void __launch_bounds__(T) k_tile_pass(amp_t* __restrict__ s, uint32_t
     for (uint32_t o = 0; o < n_ops; ++o) {
         const TileOp& op = ops[o];
         if ((base & op.gctrl) != op.gctrl) continue;  // tile-uniform control
+        if (op.member >= 0 && (op.member == static_cast<int32_t>(blockIdx.y)) != (op.only != 0)) continue;
         const uint32_t lctrl = op.lctrl;
         switch (op.kind) {
             case TK_DENSE1: {

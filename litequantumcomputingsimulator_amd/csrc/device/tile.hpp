@@ -19,6 +19,10 @@ enum TileKind : uint32_t {
 struct TileOp {
     uint32_t kind;
     uint32_t t0, t1;
+    // Batch-member restriction (trajectories): member >= 0 with only = 1:
+    // that register alone; with only = 0: every register but that one.
+    int32_t member;
+    uint32_t only;
     uint32_t lctrl;
     uint64_t gctrl;
     uint32_t dl0, dl1;
@@ -42,7 +46,8 @@ struct RegOpHdr {
     uint32_t la, lb;  // DIAG: tile-local diagonal bits; PAULI: la = tile-local Z bits
     uint32_t gctrl;   // controls on the tile number
     uint32_t ga, gb;  // DIAG: tile-number diagonal bits; PAULI: ga = tile-number Z bits
-    uint32_t pad;
+    uint32_t member;  // 0: every batch register; else (index + 1) | 1 << 31 for that one alone,
+                      // (index + 1) for every one but it
 };
 static_assert(sizeof(RegOpHdr) == 2 * sizeof(amp_t), "op header is two 16-byte words");
 

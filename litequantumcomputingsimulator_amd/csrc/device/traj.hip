@@ -21,10 +21,13 @@
 // draw the bound does not settle (one 2x2-statistics read of that shot's
 // register, then the reference's rule on the normalised probabilities).
 //
-// Between events the whole batch shares one op sequence (the circuit's gates
-// and the default branches), which runs as tiled passes over the batch: a
-// trajectory costs a few HBM passes per tile set instead of two per noise
-// site. Branch choices equal the reference's except for draws within
+// Between state-dependent events the whole batch shares one op sequence (the
+// circuit's gates, the default branches, and the other branches of
+// state-independent channels as ops restricted to one batch register), which
+// runs as tiled passes over the batch: a trajectory costs a few HBM passes
+// per tile set instead of two per noise site. Trailing measurements are
+// sampled without collapsing. Branch choices equal the reference's except for
+// draws within
 // rounding of a cumulative boundary.
 #include "common.hpp"
 #include "engine.hpp"
@@ -307,6 +310,21 @@ int mixture_branch(const ChannelPlan& cp, double u) {
     return chosen;
 }
 
+// a * b^dagger of two 2x2 matrices (row-major).
+Mat2 mul_adj(const Mat2& a, const Mat2& b) {
+    Mat2 r{};
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            amp_t acc = make_double2(0.0, 0.0);
+            for (int k = 0; k < 2; ++k) {
+                const amp_t bc = make_double2(b.m[2 * j + k].x, -b.m[2 * j + k].y);  // conj(b[j][k])
+                acc = cadd(acc, cmul(a.m[2 * i + k], bc));
+            }
+            r.m[2 * i + j] = acc;
+        }
+    return r;
+}
+
 DecodedOp single_op(uint32_t q, const Mat2& m) {
     DecodedOp d;
     d.targets = {q};
@@ -373,9 +391,10 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
                 d += uint64_t(n_channels) * st.n_noise_qubits;
                 sites += uint64_t(n_channels) * st.n_noise_qubits;
                 for (uint32_t c = 0; c < n_channels; ++c) {
+                    // Only state-dependent events split the shared run (mixture
+                    // branches ride along as single-register ops).
                     const ChannelPlan& cp = cplan[c];
-                    const double p_event = cp.mixture ? 1.0 - (cp.def >= 0 ? cp.c[cp.def] : 0.0) : 1.0 - cp.bound;
-                    events_per_shot += st.n_noise_qubits * std::max(0.0, p_event);
+                    if (!cp.mixture) events_per_shot += st.n_noise_qubits * std::max(0.0, 1.0 - cp.bound);
                 }
                 for (uint32_t j = 0; j < st.n_noise_qubits; ++j)
                     if (noise_qubits[st.noise_qubit_offset + j] >= nq)
@@ -397,8 +416,8 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
         while (term > 0 && steps[term - 1].kind == LQCS_TOP_MEASURE) --term;
 
         // Batch size: at most max_batch and ~1/2 of free HBM; and small enough
-        // that the batch's events leave runs of ~25 noise sites between them
-        // (every event splits the shared op sequence for the whole batch).
+        // that the batch's state-dependent events leave runs of ~25 noise sites
+        // between them (each splits the shared op sequence for the whole batch).
         size_t free_b = 0, total_b = 0;
         LQCS_HIP(hipMemGetInfo(&free_b, &total_b));
         const uint64_t reg_bytes = (uint64_t{1} << nq) * sizeof(amp_t);
@@ -498,6 +517,20 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
                             if (!cp.def_identity) pending.push_back(single_op(q, cp.def_op));
                             continue;
                         }
+                        if (cp.mixture) {
+                            // State-independent branches stay in the shared run: the
+                            // default on every register, then on each event register
+                            // its branch times the default's inverse (both unitary).
+                            if (!cp.def_identity) pending.push_back(single_op(q, cp.def_op));
+                            for (const auto& e : ev) {
+                                DecodedOp op = single_op(q, cp.def_identity ? cp.unit[e.second]
+                                                                            : mul_adj(cp.unit[e.second], cp.def_op));
+                                op.member = static_cast<int32_t>(e.first);
+                                op.only = true;
+                                pending.push_back(std::move(op));
+                            }
+                            continue;
+                        }
                         flush();
                         // Undecided members: statistics of their own registers first.
                         std::vector<uint32_t> undecided;
@@ -505,20 +538,20 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
                             if (e.second < 0) undecided.push_back(e.first);
                         for (size_t k = 0; k < undecided.size(); ++k)
                             reduce_qstats(states + (uint64_t(undecided[k]) << nq), nq, q, 1, d_stats + 4 * k);
-                        // Default branch on everyone else.
-                        if (!cp.def_identity) {
+                        // Default branch on everyone else: with one event register, as
+                        // the next shared run's first op; otherwise now, skipping them.
+                        if (!cp.def_identity && ev.size() == 1) {
+                            DecodedOp op = single_op(q, cp.def_op);
+                            op.member = static_cast<int32_t>(ev[0].first);
+                            op.only = false;
+                            pending.push_back(std::move(op));
+                        } else if (!cp.def_identity) {
                             std::vector<uint8_t> skip(nb, 0);
                             for (const auto& e : ev) skip[e.first] = 1;
                             LQCS_HIP(hipMemcpyAsync(d_skip, skip.data(), nb, hipMemcpyHostToDevice, stream()));
                             k_single_skip<<<dim3(pass_blocks(nq), nb), kBlock, 0, stream()>>>(states, nq, q,
                                                                                              cp.def_op, d_skip);
                             LQCS_HIP(hipGetLastError());
-                        }
-                        // Other branches of mixtures: their unitary.
-                        for (const auto& e : ev) {
-                            if (e.second < 0) continue;
-                            const DecodedOp op = single_op(q, cp.unit[e.second]);
-                            run_single_op(states + (uint64_t(e.first) << nq), nq, op, 1);
                         }
                         // Undecided: the reference's rule on their statistics.
                         if (!undecided.empty()) {

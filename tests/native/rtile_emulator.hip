@@ -130,6 +130,7 @@ int main(int argc, char** argv) {
         for (int k = 0; k < (narrow ? 400 : n_ops); ++k) {
             TileOp t;
             std::memset(&t, 0, sizeof t);
+            t.member = -1;  // every batch register
             const int kind = static_cast<int>(rng() % 10);
             auto rand_m = [&](int n) {
                 for (int e = 0; e < n; ++e) t.m[e] = make_double2(U(rng), U(rng));

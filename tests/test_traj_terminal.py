@@ -1,10 +1,12 @@
-"""Trailing measurements of the trajectory path (reference
-statevector_simulator.cpp:156-190, collapse_qubit gate_kernels.cpp:227-247):
+"""Trajectory path (reference statevector_simulator.cpp:122-190,
+collapse_qubit gate_kernels.cpp:227-247). Trailing measurements:
 the engine samples them from sums over the amplitudes consistent with the
 outcomes so far instead of collapsing the register; the counts equal the
 oracle's, which collapses, draw for draw. Orders that are not the low-qubit
 prefix, repeated qubits, clbit remapping, and a measurement followed by
-gates (so only the last ones are trailing) are covered."""
+gates (so only the last ones are trailing) are covered. Events: branches of
+state-independent channels (unitary default or not) and state-dependent
+draws, at batch sizes that put one, few or many events on a site."""
 import numpy as np
 import pytest
 
@@ -66,3 +68,39 @@ def test_noiseless_midcircuit_trailing(lq):
     c.h(0).cx(0, 1).measure(0, 0).h(0).t(2).h(2).measure(2, 1).measure(1, 2)
     want = O.run_counts(c, seed=4, shots=400)
     assert dict(lq.StatevectorSimulator(seed=4).run(c.to_product(lq), 400).counts()) == want
+
+
+def test_mixture_with_unitary_default_branch(lq):
+    """A state-independent channel whose default branch is not the identity
+    (sqrt(1-p) S, sqrt(p) X): the default rides on every register and an
+    event register takes X S^dagger after it, in the shared run."""
+    import math
+
+    p = 0.08
+    a, b = math.sqrt(1 - p), math.sqrt(p)
+    ops = [[a, 0, 0, 1j * a], [0, b, b, 0]]
+    n, seed, shots = 7, 31, 300
+    c = noisy_body(n, 3)
+    c.measure_all()
+    want = O.run_counts(c, seed=seed, shots=shots, channels=[ops, O.amplitude_damping(0.04)])
+    noise = lq.NoiseModel()
+    noise.add_all_qubit_channel(lq.KrausChannel([np.array(o, complex) for o in ops]))
+    noise.add_all_qubit_channel(lq.KrausChannel.amplitude_damping(0.04))
+    got = dict(lq.StatevectorSimulator(seed=seed, noise=noise).run(c.to_product(lq), shots).counts())
+    assert got == want
+
+
+@pytest.mark.parametrize("max_batch", [1, 3, 64])
+def test_event_handling_independent_of_batch_size(lq, max_batch):
+    """Single-register ops, deferred defaults and skip launches depend on how
+    many events share a site within a batch; the counts do not."""
+    n, seed, shots = 8, 12, 200
+    c = noisy_body(n, 4)
+    c.measure_all()
+    ch = [O.depolarizing(0.06), O.amplitude_damping(0.06)]
+    want = O.run_counts(c, seed=seed, shots=shots, channels=ch)
+    noise = lq.NoiseModel()
+    noise.add_all_qubit_channel(lq.KrausChannel.depolarizing(0.06))
+    noise.add_all_qubit_channel(lq.KrausChannel.amplitude_damping(0.06))
+    sim = lq.StatevectorSimulator(seed=seed, noise=noise, max_batch=max_batch)
+    assert dict(sim.run(c.to_product(lq), shots).counts()) == want
