@@ -341,6 +341,57 @@ DecodedOp single_op(uint32_t q, const Mat2& m) {
     return d;
 }
 
+// Greedy single-qubit fusion of a shared run (the rule of the reference's
+// gate_fusion.cpp:32-68, applied to the run's ops, whose noise branches are
+// not unitary): uncontrolled 2x2 ops on a qubit multiply into one pending
+// matrix, emitted before the next op that touches the qubit (or at the end).
+// Ops restricted to one batch register touch their qubits like any other.
+// The default branch of amplitude damping after every gate then folds into
+// the gates around it instead of costing an op each.
+void fuse_run(std::vector<DecodedOp>& ops) {
+    std::vector<DecodedOp> out;
+    out.reserve(ops.size());
+    std::vector<Mat2> acc(64);
+    uint64_t have = 0;
+    auto as_mat = [](const DecodedOp& d) {
+        Mat2 m{};
+        if (d.kind == LQCS_OP_DIAGONAL_SINGLE) {
+            m.m[0] = d.m[0];
+            m.m[3] = d.m[1];
+        } else {
+            for (int e = 0; e < 4; ++e) m.m[e] = d.m[e];
+        }
+        return m;
+    };
+    auto emit = [&](uint32_t q) {
+        out.push_back(single_op(q, acc[q]));
+        have &= ~(uint64_t{1} << q);
+    };
+    for (DecodedOp& d : ops) {
+        const bool one = (d.kind == LQCS_OP_SINGLE || d.kind == LQCS_OP_DIAGONAL_SINGLE) && d.controls.empty() &&
+                         d.member < 0 && d.targets.size() == 1;
+        if (one) {
+            const uint32_t q = d.targets[0];
+            const Mat2 m = as_mat(d);
+            if (have & (uint64_t{1} << q)) {
+                Mat2 r{};
+                for (int i = 0; i < 2; ++i)
+                    for (int j = 0; j < 2; ++j)
+                        r.m[2 * i + j] = cadd(cmul(m.m[2 * i], acc[q].m[j]), cmul(m.m[2 * i + 1], acc[q].m[2 + j]));
+                acc[q] = r;
+            } else {
+                acc[q] = m;
+                have |= uint64_t{1} << q;
+            }
+            continue;
+        }
+        for (uint64_t t = d.touched & have; t; t &= t - 1) emit(static_cast<uint32_t>(__builtin_ctzll(t)));
+        out.push_back(std::move(d));
+    }
+    for (uint64_t t = have; t; t &= t - 1) emit(static_cast<uint32_t>(__builtin_ctzll(t)));
+    ops.swap(out);
+}
+
 template <typename T>
 T* dev_array(size_t n) {
     return static_cast<T*>(device_alloc(std::max<size_t>(n, 1) * sizeof(T)));
@@ -453,6 +504,7 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
         uint32_t nb = 0;
         auto flush = [&] {
             if (pending.empty()) return;
+            fuse_run(pending);
             const Plan plan = make_plan(tc.L ? LQCS_PLAN_GREEDY : LQCS_PLAN_GATE_BY_GATE, pending, nq, tc.L, tc.low);
             execute_plan(states, nq, tc.L, plan, pending, nb);
             pending.clear();
