@@ -90,6 +90,31 @@ __device__ __forceinline__ amp_t pick(bool c, amp_t p, amp_t q) {
     return make_double2(c ? p.x : q.x, c ? p.y : q.y);
 }
 
+// Register sets in fixed physical registers: set A in v[2:65], set B in
+// v[66:129]. Pinning an op's output set at the end of the op makes every case
+// of the op dispatch produce its results in the same registers, so the joins
+// of the dispatch need no copies (register allocation otherwise tends to
+// leave results where one case put them and copy whole sets at the joins).
+__device__ __forceinline__ void pin_a(Regs& x) {
+    asm volatile("" : "+{v[2:3]}"(x[0].x), "+{v[4:5]}"(x[0].y), "+{v[6:7]}"(x[1].x), "+{v[8:9]}"(x[1].y), "+{v[10:11]}"(x[2].x), "+{v[12:13]}"(x[2].y), "+{v[14:15]}"(x[3].x), "+{v[16:17]}"(x[3].y));
+    asm volatile("" : "+{v[18:19]}"(x[4].x), "+{v[20:21]}"(x[4].y), "+{v[22:23]}"(x[5].x), "+{v[24:25]}"(x[5].y), "+{v[26:27]}"(x[6].x), "+{v[28:29]}"(x[6].y), "+{v[30:31]}"(x[7].x), "+{v[32:33]}"(x[7].y));
+    asm volatile("" : "+{v[34:35]}"(x[8].x), "+{v[36:37]}"(x[8].y), "+{v[38:39]}"(x[9].x), "+{v[40:41]}"(x[9].y), "+{v[42:43]}"(x[10].x), "+{v[44:45]}"(x[10].y), "+{v[46:47]}"(x[11].x), "+{v[48:49]}"(x[11].y));
+    asm volatile("" : "+{v[50:51]}"(x[12].x), "+{v[52:53]}"(x[12].y), "+{v[54:55]}"(x[13].x), "+{v[56:57]}"(x[13].y), "+{v[58:59]}"(x[14].x), "+{v[60:61]}"(x[14].y), "+{v[62:63]}"(x[15].x), "+{v[64:65]}"(x[15].y));
+}
+
+__device__ __forceinline__ void pin_b(Regs& x) {
+    asm volatile("" : "+{v[66:67]}"(x[0].x), "+{v[68:69]}"(x[0].y), "+{v[70:71]}"(x[1].x), "+{v[72:73]}"(x[1].y), "+{v[74:75]}"(x[2].x), "+{v[76:77]}"(x[2].y), "+{v[78:79]}"(x[3].x), "+{v[80:81]}"(x[3].y));
+    asm volatile("" : "+{v[82:83]}"(x[4].x), "+{v[84:85]}"(x[4].y), "+{v[86:87]}"(x[5].x), "+{v[88:89]}"(x[5].y), "+{v[90:91]}"(x[6].x), "+{v[92:93]}"(x[6].y), "+{v[94:95]}"(x[7].x), "+{v[96:97]}"(x[7].y));
+    asm volatile("" : "+{v[98:99]}"(x[8].x), "+{v[100:101]}"(x[8].y), "+{v[102:103]}"(x[9].x), "+{v[104:105]}"(x[9].y), "+{v[106:107]}"(x[10].x), "+{v[108:109]}"(x[10].y), "+{v[110:111]}"(x[11].x), "+{v[112:113]}"(x[11].y));
+    asm volatile("" : "+{v[114:115]}"(x[12].x), "+{v[116:117]}"(x[12].y), "+{v[118:119]}"(x[13].x), "+{v[120:121]}"(x[13].y), "+{v[122:123]}"(x[14].x), "+{v[124:125]}"(x[14].y), "+{v[126:127]}"(x[15].x), "+{v[128:129]}"(x[15].y));
+}
+
+template <int OUT>
+__device__ __forceinline__ void pin_out(Regs& x) {
+    if constexpr (OUT == 0) pin_a(x);
+    else pin_b(x);
+}
+
 __device__ __forceinline__ void copy_regs(const Regs& a, Regs& o) {
 #pragma unroll
     for (int r = 0; r < kRR; ++r) o[r] = a[r];
@@ -217,11 +242,12 @@ __device__ __forceinline__ void r_pauli_diag(const Regs& a, Regs& o, const Hdr& 
     }
 }
 
+template <int OUT>
 __device__ __forceinline__ void apply_reg_op(const Regs& a, Regs& o, const Hdr& h, const amp_t* m, uint32_t jt,
                                              const RPhase& ph, uint32_t tile) {
     if ((tile & h.gctrl) != h.gctrl ||  // tile-uniform control not met
         (h.member && ((h.member & 0x7fffffffu) - 1 == blockIdx.y) != (h.member >> 31 != 0))) {  // batch register
-        copy_regs(a, o);
+        copy_regs(a, o); pin_out<OUT>(o);
         return;
     }
     const bool free = (h.rctl | h.tctl) == 0;
@@ -230,93 +256,94 @@ __device__ __forceinline__ void apply_reg_op(const Regs& a, Regs& o, const Hdr& 
         case RK_DENSE1:
             if (free) {
                 switch (h.b0) {
-                    case 0: d1_all<0>(a, o, m); return;
-                    case 1: d1_all<1>(a, o, m); return;
-                    case 2: d1_all<2>(a, o, m); return;
-                    default: d1_all<3>(a, o, m); return;
+                    case 0: d1_all<0>(a, o, m); pin_out<OUT>(o); return;
+                    case 1: d1_all<1>(a, o, m); pin_out<OUT>(o); return;
+                    case 2: d1_all<2>(a, o, m); pin_out<OUT>(o); return;
+                    default: d1_all<3>(a, o, m); pin_out<OUT>(o); return;
                 }
             }
             switch (h.b0) {
-                case 0: d1_masked<0>(a, o, m, h.rctl, tp); return;
-                case 1: d1_masked<1>(a, o, m, h.rctl, tp); return;
-                case 2: d1_masked<2>(a, o, m, h.rctl, tp); return;
-                default: d1_masked<3>(a, o, m, h.rctl, tp); return;
+                case 0: d1_masked<0>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 1: d1_masked<1>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 2: d1_masked<2>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                default: d1_masked<3>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
             }
         case RK_X:
             if (free) {
                 switch (h.b0) {
-                    case 0: x_all<0>(a, o); return;
-                    case 1: x_all<1>(a, o); return;
-                    case 2: x_all<2>(a, o); return;
-                    default: x_all<3>(a, o); return;
+                    case 0: x_all<0>(a, o); pin_out<OUT>(o); return;
+                    case 1: x_all<1>(a, o); pin_out<OUT>(o); return;
+                    case 2: x_all<2>(a, o); pin_out<OUT>(o); return;
+                    default: x_all<3>(a, o); pin_out<OUT>(o); return;
                 }
             }
             if (h.tctl == 0 && __builtin_popcount(h.rctl) == 1) {
                 switch (h.b0 * 4 + __builtin_ctz(h.rctl)) {
-                    case 1: x_rc<0, 1>(a, o); return;
-                    case 2: x_rc<0, 2>(a, o); return;
-                    case 3: x_rc<0, 3>(a, o); return;
-                    case 4: x_rc<1, 0>(a, o); return;
-                    case 6: x_rc<1, 2>(a, o); return;
-                    case 7: x_rc<1, 3>(a, o); return;
-                    case 8: x_rc<2, 0>(a, o); return;
-                    case 9: x_rc<2, 1>(a, o); return;
-                    case 11: x_rc<2, 3>(a, o); return;
-                    case 12: x_rc<3, 0>(a, o); return;
-                    case 13: x_rc<3, 1>(a, o); return;
-                    default: x_rc<3, 2>(a, o); return;
+                    case 1: x_rc<0, 1>(a, o); pin_out<OUT>(o); return;
+                    case 2: x_rc<0, 2>(a, o); pin_out<OUT>(o); return;
+                    case 3: x_rc<0, 3>(a, o); pin_out<OUT>(o); return;
+                    case 4: x_rc<1, 0>(a, o); pin_out<OUT>(o); return;
+                    case 6: x_rc<1, 2>(a, o); pin_out<OUT>(o); return;
+                    case 7: x_rc<1, 3>(a, o); pin_out<OUT>(o); return;
+                    case 8: x_rc<2, 0>(a, o); pin_out<OUT>(o); return;
+                    case 9: x_rc<2, 1>(a, o); pin_out<OUT>(o); return;
+                    case 11: x_rc<2, 3>(a, o); pin_out<OUT>(o); return;
+                    case 12: x_rc<3, 0>(a, o); pin_out<OUT>(o); return;
+                    case 13: x_rc<3, 1>(a, o); pin_out<OUT>(o); return;
+                    default: x_rc<3, 2>(a, o); pin_out<OUT>(o); return;
                 }
             }
             switch (h.b0) {
-                case 0: x_masked<0>(a, o, h.rctl, tp); return;
-                case 1: x_masked<1>(a, o, h.rctl, tp); return;
-                case 2: x_masked<2>(a, o, h.rctl, tp); return;
-                default: x_masked<3>(a, o, h.rctl, tp); return;
+                case 0: x_masked<0>(a, o, h.rctl, tp); pin_out<OUT>(o); return;
+                case 1: x_masked<1>(a, o, h.rctl, tp); pin_out<OUT>(o); return;
+                case 2: x_masked<2>(a, o, h.rctl, tp); pin_out<OUT>(o); return;
+                default: x_masked<3>(a, o, h.rctl, tp); pin_out<OUT>(o); return;
             }
         case RK_DENSE2:
             switch (h.b0 * 4 + h.b1) {
-                case 1: d2<0, 1>(a, o, m, h.rctl, tp); return;
-                case 2: d2<0, 2>(a, o, m, h.rctl, tp); return;
-                case 3: d2<0, 3>(a, o, m, h.rctl, tp); return;
-                case 4: d2<1, 0>(a, o, m, h.rctl, tp); return;
-                case 6: d2<1, 2>(a, o, m, h.rctl, tp); return;
-                case 7: d2<1, 3>(a, o, m, h.rctl, tp); return;
-                case 8: d2<2, 0>(a, o, m, h.rctl, tp); return;
-                case 9: d2<2, 1>(a, o, m, h.rctl, tp); return;
-                case 11: d2<2, 3>(a, o, m, h.rctl, tp); return;
-                case 12: d2<3, 0>(a, o, m, h.rctl, tp); return;
-                case 13: d2<3, 1>(a, o, m, h.rctl, tp); return;
-                default: d2<3, 2>(a, o, m, h.rctl, tp); return;
+                case 1: d2<0, 1>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 2: d2<0, 2>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 3: d2<0, 3>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 4: d2<1, 0>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 6: d2<1, 2>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 7: d2<1, 3>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 8: d2<2, 0>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 9: d2<2, 1>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 11: d2<2, 3>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 12: d2<3, 0>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 13: d2<3, 1>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                default: d2<3, 2>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
             }
         case RK_DIAG:
-            r_diag(a, o, h, m, jt, ph, tile, tp);
+            r_diag(a, o, h, m, jt, ph, tile, tp); pin_out<OUT>(o);
             return;
         case RK_PAULI:
             switch (h.xr) {
-                case 0: r_pauli_diag(a, o, h, m, jt, ph, tile); return;
-                case 1: r_pauli<1>(a, o, h, m, jt, ph, tile); return;
-                case 2: r_pauli<2>(a, o, h, m, jt, ph, tile); return;
-                case 3: r_pauli<3>(a, o, h, m, jt, ph, tile); return;
-                case 4: r_pauli<4>(a, o, h, m, jt, ph, tile); return;
-                case 5: r_pauli<5>(a, o, h, m, jt, ph, tile); return;
-                case 6: r_pauli<6>(a, o, h, m, jt, ph, tile); return;
-                case 7: r_pauli<7>(a, o, h, m, jt, ph, tile); return;
-                case 8: r_pauli<8>(a, o, h, m, jt, ph, tile); return;
-                case 9: r_pauli<9>(a, o, h, m, jt, ph, tile); return;
-                case 10: r_pauli<10>(a, o, h, m, jt, ph, tile); return;
-                case 11: r_pauli<11>(a, o, h, m, jt, ph, tile); return;
-                case 12: r_pauli<12>(a, o, h, m, jt, ph, tile); return;
-                case 13: r_pauli<13>(a, o, h, m, jt, ph, tile); return;
-                case 14: r_pauli<14>(a, o, h, m, jt, ph, tile); return;
-                default: r_pauli<15>(a, o, h, m, jt, ph, tile); return;
+                case 0: r_pauli_diag(a, o, h, m, jt, ph, tile); pin_out<OUT>(o); return;
+                case 1: r_pauli<1>(a, o, h, m, jt, ph, tile); pin_out<OUT>(o); return;
+                case 2: r_pauli<2>(a, o, h, m, jt, ph, tile); pin_out<OUT>(o); return;
+                case 3: r_pauli<3>(a, o, h, m, jt, ph, tile); pin_out<OUT>(o); return;
+                case 4: r_pauli<4>(a, o, h, m, jt, ph, tile); pin_out<OUT>(o); return;
+                case 5: r_pauli<5>(a, o, h, m, jt, ph, tile); pin_out<OUT>(o); return;
+                case 6: r_pauli<6>(a, o, h, m, jt, ph, tile); pin_out<OUT>(o); return;
+                case 7: r_pauli<7>(a, o, h, m, jt, ph, tile); pin_out<OUT>(o); return;
+                case 8: r_pauli<8>(a, o, h, m, jt, ph, tile); pin_out<OUT>(o); return;
+                case 9: r_pauli<9>(a, o, h, m, jt, ph, tile); pin_out<OUT>(o); return;
+                case 10: r_pauli<10>(a, o, h, m, jt, ph, tile); pin_out<OUT>(o); return;
+                case 11: r_pauli<11>(a, o, h, m, jt, ph, tile); pin_out<OUT>(o); return;
+                case 12: r_pauli<12>(a, o, h, m, jt, ph, tile); pin_out<OUT>(o); return;
+                case 13: r_pauli<13>(a, o, h, m, jt, ph, tile); pin_out<OUT>(o); return;
+                case 14: r_pauli<14>(a, o, h, m, jt, ph, tile); pin_out<OUT>(o); return;
+                default: r_pauli<15>(a, o, h, m, jt, ph, tile); pin_out<OUT>(o); return;
             }
         default:
-            copy_regs(a, o);
+            copy_regs(a, o); pin_out<OUT>(o);
             return;
     }
 }
 
 // Applies the op at word `off` of the pass's words; returns the next op's word.
+template <int OUT>
 __device__ __forceinline__ uint32_t apply_op(const Regs& a, Regs& o, const amp_t* words, uint32_t off,
                                                  uint32_t jt, const RPhase& ph, uint32_t tile) {
     const uint4* hp = reinterpret_cast<const uint4*>(words + off);
@@ -335,7 +362,7 @@ __device__ __forceinline__ uint32_t apply_op(const Regs& a, Regs& o, const amp_t
     h.ga = rfl(h1.y);
     h.gb = rfl(h1.z);
     h.member = rfl(h1.w);
-    apply_reg_op(a, o, h, words + off + 2, jt, ph, tile);
+    apply_reg_op<OUT>(a, o, h, words + off + 2, jt, ph, tile);
     return off + 2 + (code >> 16);
 }
 
@@ -374,17 +401,19 @@ __device__ __forceinline__ void run_phases(Regs& v, Regs& w, amp_t* lds, const R
                 lds_used = true;
             }
             copy_regs(w, v);
+            pin_a(v);
         }
         jt_prev = jt;
         uint32_t off = ph.op_off;
         uint32_t i = 0;
         for (; i + 1 < ph.n_ops; i += 2) {
-            off = apply_op(v, w, words, off, jt, ph, tile);
-            off = apply_op(w, v, words, off, jt, ph, tile);
+            off = apply_op<1>(v, w, words, off, jt, ph, tile);
+            off = apply_op<0>(w, v, words, off, jt, ph, tile);
         }
         if (i < ph.n_ops) {
-            apply_op(v, w, words, off, jt, ph, tile);
+            apply_op<1>(v, w, words, off, jt, ph, tile);
             copy_regs(w, v);
+            pin_a(v);
         }
     }
 }
