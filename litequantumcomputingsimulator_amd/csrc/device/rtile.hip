@@ -124,8 +124,7 @@ __device__ __forceinline__ void copy_regs(const Regs& a, Regs& o) {
 // results land in their final registers without copies. `m` points at the
 // op's matrix entries in LDS.
 template <int B>
-__device__ __forceinline__ void d1_all(const Regs& a, Regs& o, const amp_t* m) {
-    const amp_t m0 = uni(m), m1 = uni(m + 1), m2 = uni(m + 2), m3 = uni(m + 3);
+__device__ __forceinline__ void d1_vals(const Regs& a, Regs& o, amp_t m0, amp_t m1, amp_t m2, amp_t m3) {
 #pragma unroll
     for (int r = 0; r < kRR; ++r) {
         if (r & (1 << B)) continue;
@@ -133,6 +132,33 @@ __device__ __forceinline__ void d1_all(const Regs& a, Regs& o, const amp_t* m) {
         o[r] = cmac2(m0, a0, m1, a1);
         o[r | (1 << B)] = cmac2(m2, a0, m3, a1);
     }
+}
+
+template <int B>
+__device__ __forceinline__ void d1_all(const Regs& a, Regs& o, const amp_t* m) {
+    d1_vals<B>(a, o, uni(m), uni(m + 1), uni(m + 2), uni(m + 3));
+}
+
+// 2x2 op on register bit B with the matrix chosen by register bit S: the
+// first four entries where bit S is 0, the next four where it is 1.
+template <int B, int S>
+__device__ __forceinline__ void d1_sel(const Regs& a, Regs& o, const amp_t* m) {
+#pragma unroll
+    for (int r = 0; r < kRR; ++r) {
+        if (r & (1 << B)) continue;
+        const amp_t* mr = (r & (1 << S)) ? m + 4 : m;
+        const amp_t a0 = a[r], a1 = a[r | (1 << B)];
+        o[r] = cmac2(uni(mr), a0, uni(mr + 1), a1);
+        o[r | (1 << B)] = cmac2(uni(mr + 2), a0, uni(mr + 3), a1);
+    }
+}
+
+// The same with the matrix chosen per lane (select bit among the thread's
+// tile bits): the entries are selected once into vector registers.
+template <int B>
+__device__ __forceinline__ void d1_lanesel(const Regs& a, Regs& o, const amp_t* m, bool s) {
+    d1_vals<B>(a, o, pick(s, uni(m + 4), uni(m)), pick(s, uni(m + 5), uni(m + 1)), pick(s, uni(m + 6), uni(m + 2)),
+               pick(s, uni(m + 7), uni(m + 3)));
 }
 
 // Controlled variant: pairs whose register bits miss a control, or lanes
@@ -252,14 +278,22 @@ __device__ __forceinline__ void apply_reg_op(const Regs& a, Regs& o, const Hdr& 
     }
     const bool free = (h.rctl | h.tctl) == 0;
     const bool tp = (jt & h.tctl) == h.tctl;
-    switch (h.kind) {
+    // A select op whose select bit is outside the tile is a plain 2x2 op with
+    // one of its two matrices, the same for the whole tile.
+    uint32_t kind = h.kind;
+    const amp_t* mp = m;
+    if (kind == RK_SEL1 && h.xr == 0 && h.la == 0) {
+        kind = RK_DENSE1;
+        if (tile & h.ga) mp = m + 4;
+    }
+    switch (kind) {
         case RK_DENSE1:
             if (free) {
                 switch (h.b0) {
-                    case 0: d1_all<0>(a, o, m); pin_out<OUT>(o); return;
-                    case 1: d1_all<1>(a, o, m); pin_out<OUT>(o); return;
-                    case 2: d1_all<2>(a, o, m); pin_out<OUT>(o); return;
-                    default: d1_all<3>(a, o, m); pin_out<OUT>(o); return;
+                    case 0: d1_all<0>(a, o, mp); pin_out<OUT>(o); return;
+                    case 1: d1_all<1>(a, o, mp); pin_out<OUT>(o); return;
+                    case 2: d1_all<2>(a, o, mp); pin_out<OUT>(o); return;
+                    default: d1_all<3>(a, o, mp); pin_out<OUT>(o); return;
                 }
             }
             switch (h.b0) {
@@ -313,6 +347,32 @@ __device__ __forceinline__ void apply_reg_op(const Regs& a, Regs& o, const Hdr& 
                 case 12: d2<3, 0>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
                 case 13: d2<3, 1>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
                 default: d2<3, 2>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+            }
+        case RK_SEL1:
+            if (h.xr) {
+                switch (h.b0 * 4 + h.xr - 1) {
+                    case 1: d1_sel<0, 1>(a, o, m); pin_out<OUT>(o); return;
+                    case 2: d1_sel<0, 2>(a, o, m); pin_out<OUT>(o); return;
+                    case 3: d1_sel<0, 3>(a, o, m); pin_out<OUT>(o); return;
+                    case 4: d1_sel<1, 0>(a, o, m); pin_out<OUT>(o); return;
+                    case 6: d1_sel<1, 2>(a, o, m); pin_out<OUT>(o); return;
+                    case 7: d1_sel<1, 3>(a, o, m); pin_out<OUT>(o); return;
+                    case 8: d1_sel<2, 0>(a, o, m); pin_out<OUT>(o); return;
+                    case 9: d1_sel<2, 1>(a, o, m); pin_out<OUT>(o); return;
+                    case 11: d1_sel<2, 3>(a, o, m); pin_out<OUT>(o); return;
+                    case 12: d1_sel<3, 0>(a, o, m); pin_out<OUT>(o); return;
+                    case 13: d1_sel<3, 1>(a, o, m); pin_out<OUT>(o); return;
+                    default: d1_sel<3, 2>(a, o, m); pin_out<OUT>(o); return;
+                }
+            }
+            {
+                const bool s = (jt & h.la) != 0;
+                switch (h.b0) {
+                    case 0: d1_lanesel<0>(a, o, m, s); pin_out<OUT>(o); return;
+                    case 1: d1_lanesel<1>(a, o, m, s); pin_out<OUT>(o); return;
+                    case 2: d1_lanesel<2>(a, o, m, s); pin_out<OUT>(o); return;
+                    default: d1_lanesel<3>(a, o, m, s); pin_out<OUT>(o); return;
+                }
             }
         case RK_DIAG:
             r_diag(a, o, h, m, jt, ph, tile, tp); pin_out<OUT>(o);
@@ -657,16 +717,96 @@ uint32_t matrix_words(uint32_t kind) {
         case RK_DENSE1: case RK_DIAG: return 4;
         case RK_DENSE2: return 16;
         case RK_PAULI: return 2;
+        case RK_SEL1: return 8;
         default: return 0;
     }
 }
 
+// Merges each singly-controlled X (a CX) with the uncontrolled dense 2x2 op
+// U on its target that comes next on that qubit, or else with the one just
+// before it: U CX (or CX U) is one 2x2 op on the target whose matrix the
+// control bit selects, U where it is 0 and U X (or X U) where it is 1. The
+// merged op stands where the CX stood; U moves there past ops that do not
+// touch its qubit. It costs what U alone costs, and the CX's register moves
+// are gone. `sel` marks the merged ops (their entries 4..7 hold the second
+// matrix). Returns how many ops were absorbed.
+uint32_t merge_selects(std::vector<TileOp>& ops, std::vector<uint8_t>& sel) {
+    const size_t n = ops.size();
+    std::vector<uint8_t> dead(n, 0);
+    sel.assign(n, 0);
+    std::vector<uint32_t> touched(n), dense(n);
+    for (size_t i = 0; i < n; ++i) local_masks(ops[i], touched[i], dense[i]);
+    auto plain_u = [&](size_t i, uint32_t t) {
+        const TileOp& u = ops[i];
+        return !sel[i] && u.kind == TK_DENSE1 && u.t0 == t && u.lctrl == 0 && u.gctrl == 0 && u.member < 0;
+    };
+    uint32_t merged = 0;
+    for (size_t k = 0; k < n; ++k) {
+        TileOp& x = ops[k];
+        if (dead[k] || x.kind != TK_DENSE1 || !is_x(x) || x.member >= 0) continue;
+        if (popc(x.lctrl) + popc(x.gctrl) != 1) continue;
+        const uint32_t t = x.t0, tb = 1u << t;
+        size_t u = n;
+        bool fwd = false;
+        for (size_t j = k + 1; j < n; ++j) {
+            if (dead[j] || !(touched[j] & tb)) continue;
+            if (plain_u(j, t)) {
+                u = j;
+                fwd = true;
+            }
+            break;
+        }
+        if (!fwd) {
+            for (size_t j = k; j-- > 0;) {
+                if (dead[j] || !(touched[j] & tb)) continue;
+                if (plain_u(j, t)) u = j;
+                break;
+            }
+        }
+        if (u == n) continue;
+        const amp_t u00 = ops[u].m[0], u01 = ops[u].m[1], u10 = ops[u].m[2], u11 = ops[u].m[3];
+        x.m[0] = u00;
+        x.m[1] = u01;
+        x.m[2] = u10;
+        x.m[3] = u11;
+        if (fwd) {  // U X: columns swapped
+            x.m[4] = u01;
+            x.m[5] = u00;
+            x.m[6] = u11;
+            x.m[7] = u10;
+        } else {  // X U: rows swapped
+            x.m[4] = u10;
+            x.m[5] = u11;
+            x.m[6] = u00;
+            x.m[7] = u01;
+        }
+        sel[k] = 1;
+        dead[u] = 1;
+        ++merged;
+    }
+    if (merged) {
+        size_t w = 0;
+        for (size_t i = 0; i < n; ++i) {
+            if (dead[i]) continue;
+            ops[w] = ops[i];
+            sel[w] = sel[i];
+            ++w;
+        }
+        ops.resize(w);
+        sel.resize(w);
+    }
+    return merged;
+}
+
 }  // namespace
 
-bool build_rtile(const std::vector<TileOp>& tops, uint32_t L, uint64_t smask, uint32_t nq, RTileProgram& out) {
+bool build_rtile(const std::vector<TileOp>& tops_in, uint32_t L, uint64_t smask, uint32_t nq, RTileProgram& out) {
     if (L != static_cast<uint32_t>(kRL)) return false;
     out.phases.clear();
     out.words.clear();
+    std::vector<TileOp> tops = tops_in;
+    std::vector<uint8_t> sel;
+    out.merged = merge_selects(tops, sel);
     const uint32_t all = (1u << kRL) - 1;
     const uint64_t outside = ((nq >= 64 ? ~uint64_t{0} : (uint64_t{1} << nq) - 1)) & ~smask;
     std::vector<uint32_t> touched(tops.size()), dense(tops.size());
@@ -768,8 +908,21 @@ bool build_rtile(const std::vector<TileOp>& tops, uint32_t L, uint64_t smask, ui
             if (t.member >= 0) h.member = (static_cast<uint32_t>(t.member) + 1) | (t.only ? 1u << 31 : 0u);
             switch (t.kind) {
                 case TK_DENSE1:
-                    kind = is_x(t) ? RK_X : RK_DENSE1;
                     b0 = ridx(t.t0);
+                    if (sel[i]) {  // the control bit selects the matrix; no control left
+                        kind = RK_SEL1;
+                        if (t.lctrl & rmask)
+                            xr = 1 + ridx(static_cast<uint32_t>(__builtin_ctz(t.lctrl)));
+                        else if (t.lctrl)
+                            h.la = t.lctrl;
+                        else
+                            h.ga = pext_host(t.gctrl, outside);
+                        rctl = 0;
+                        h.tctl = 0;
+                        h.gctrl = 0;
+                        break;
+                    }
+                    kind = is_x(t) ? RK_X : RK_DENSE1;
                     break;
                 case TK_DENSE2:
                     kind = RK_DENSE2;

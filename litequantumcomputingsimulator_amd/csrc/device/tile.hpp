@@ -33,11 +33,15 @@ struct TileOp {
 };
 
 // Register-resident tile (rtile.hip): per-phase layout and in-register ops.
-enum RegKind : uint32_t { RK_DENSE1 = 0, RK_DENSE2 = 1, RK_X = 2, RK_DIAG = 3, RK_PAULI = 4 };
+enum RegKind : uint32_t { RK_DENSE1 = 0, RK_DENSE2 = 1, RK_X = 2, RK_DIAG = 3, RK_PAULI = 4, RK_SEL1 = 5 };
 
 // Header of one in-register op in a pass's op words: two 16-byte words, then
 // the op's matrix entries (4 for DENSE1 and DIAG, 16 for DENSE2, 2 for
-// PAULI, none for X). Masks named "tile number" are over the qubits outside
+// PAULI, none for X, 8 for SEL1). SEL1 is a 2x2 op on register bit b0 whose
+// matrix is chosen by one select bit: the first four entries where it is 0,
+// the next four where it is 1. The select bit is register bit xr - 1 when xr
+// is nonzero, else the tile-local thread bit la when nonzero, else the
+// tile-number bit ga. Masks named "tile number" are over the qubits outside
 // the tile, compressed to the tile's index. The kernel reads them with
 // scalar loads (the same address in every lane).
 struct RegOpHdr {
@@ -80,6 +84,7 @@ __host__ __device__ __forceinline__ uint32_t rtile_slot(uint32_t j, uint32_t h) 
 struct RTileProgram {
     std::vector<RPhase> phases;
     std::vector<amp_t> words;  // the pass's op headers and matrices, 16 bytes each
+    uint32_t merged = 0;       // tile ops absorbed into SEL1 ops
 };
 
 // Splits a tiled step into register phases; false when some op needs more

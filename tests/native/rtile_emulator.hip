@@ -96,6 +96,7 @@ int main(int argc, char** argv) {
     std::mt19937_64 rng(20240611);
     std::uniform_real_distribution<double> U(-1, 1);
     int total_phases = 0, total_transposes = 0;
+    size_t merged_total = 0;
     for (int c = 0; c < cases; ++c) {
         const uint32_t nq = 12 + rng() % 9;  // 12..20
         // Tile set: `low` contiguous low qubits plus random others.
@@ -127,7 +128,32 @@ int main(int argc, char** argv) {
         }
         auto pick_bit = [&]() { return narrow ? four[rng() % 4] : static_cast<uint32_t>(rng() % kL); };
         std::vector<TileOp> ops;
-        for (int k = 0; k < (narrow ? 400 : n_ops); ++k) {
+        // Every tenth case is layers of dense 2x2 ops and singly-controlled X
+        // (controls on register, thread or outside bits), which the builder
+        // merges into select ops.
+        const bool layered = c % 10 == 6;
+        for (int k = 0; layered && k < 160; ++k) {
+            TileOp t;
+            std::memset(&t, 0, sizeof t);
+            t.member = -1;
+            t.kind = lqcs_dev::TK_DENSE1;
+            t.t0 = static_cast<uint32_t>(rng() % kL);
+            if (rng() % 3 == 0) {
+                t.m[1] = make_double2(1, 0);
+                t.m[2] = make_double2(1, 0);
+                if (rng() % 4 == 0 && outside) {
+                    t.gctrl = outside_bit();
+                } else {
+                    uint32_t b;
+                    do b = static_cast<uint32_t>(rng() % kL); while (b == t.t0);
+                    t.lctrl = 1u << b;
+                }
+            } else {
+                for (int e = 0; e < 4; ++e) t.m[e] = make_double2(U(rng), U(rng));
+            }
+            ops.push_back(t);
+        }
+        for (int k = 0; !layered && k < (narrow ? 400 : n_ops); ++k) {
             TileOp t;
             std::memset(&t, 0, sizeof t);
             t.member = -1;  // every batch register
@@ -272,6 +298,14 @@ int main(int argc, char** argv) {
                                     case lqcs_dev::RK_X:
                                         if (on) o[J(r)] = a[J(r ^ (1 << b0))];
                                         break;
+                                    case lqcs_dev::RK_SEL1: {
+                                        const bool s = xr ? ((r >> (xr - 1)) & 1) != 0
+                                                          : h.la ? (J(r) & h.la) != 0 : (tile & h.ga) != 0;
+                                        const amp_t* ms = s ? m + 4 : m;
+                                        const int row = (r >> b0) & 1, r0 = r & ~(1 << b0), r1 = r | (1 << b0);
+                                        o[J(r)] = C(ms[2 * row]) * a[J(r0)] + C(ms[2 * row + 1]) * a[J(r1)];
+                                        break;
+                                    }
                                     case lqcs_dev::RK_DENSE2: {
                                         const int z = r & ~((1 << b0) | (1 << b1));
                                         const bool onz = tp && ((uint32_t(z) & rctl) == rctl);
@@ -309,7 +343,8 @@ int main(int argc, char** argv) {
                     ++seen_ops;
                 }
             }
-            if (seen_ops != ops.size()) { why = "op count"; return -1; }
+            if (seen_ops + prog.merged != ops.size()) { why = "op count"; return -1; }
+            merged_total += prog.merged;
             // Relative to the largest amplitude: the random matrices are not
             // unitary, so magnitudes (and rounding) grow along the sequence.
             double err = 0, scale = 0;
@@ -337,6 +372,6 @@ int main(int argc, char** argv) {
             return 1;
         }
     }
-    std::printf("ok %d phases=%d transposes=%d\n", cases, total_phases, total_transposes);
+    std::printf("ok %d phases=%d transposes=%d merged=%zu\n", cases, total_phases, total_transposes, merged_total);
     return 0;
 }

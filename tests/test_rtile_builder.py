@@ -32,3 +32,6 @@ def test_register_programs_match_direct_application(tmp_path):
     assert line.startswith("ok 400"), line
     fields = dict(kv.split("=") for kv in line.split()[2:])
     assert int(fields["transposes"]) > 1000
+    # The layered cases merge singly-controlled X into the neighbouring 2x2
+    # ops (select ops, executed and compared like every other op).
+    assert int(fields["merged"]) > 500
