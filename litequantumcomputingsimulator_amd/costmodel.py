@@ -107,6 +107,18 @@ def random_circuit(lq, n: int, family: str, depth: int, seed: int):
 
 FAMILIES = ("u3cx", "u3cxcp", "pauli", "local", "qft")
 CANDIDATES = tuple((plan, low) for plan in ("greedy", "gain") for low in (2, 3, 4))
+# The engine keeps greedy grouping at the default low qubits (4 at these
+# widths) unless another candidate is predicted faster by this fraction.
+MARGIN = 0.02
+BASELINE = ("greedy", 4)
+
+
+def pick(pred: dict) -> tuple:
+    """The engine's LEARNED choice among predicted totals {(plan, low): ms}."""
+    best = min(pred, key=pred.get)
+    if BASELINE in pred and pred[best] > (1.0 - MARGIN) * pred[BASELINE]:
+        return BASELINE
+    return best
 
 
 def time_plan(lq, comp, sv, plan: str, low: int, reps: int = 1):
@@ -158,11 +170,11 @@ def evaluate(lq, model: "Model", records):
     model) and of the best candidate in hindsight."""
     out = []
     for rec in records:
-        pred = [float(model.predict_ms(lq, f).sum()) for (_, _, f, _) in rec["cands"]]
-        k = int(np.argmin(pred))
+        pred = {(p, lw): float(model.predict_ms(lq, f).sum()) for (p, lw, f, _) in rec["cands"]}
+        k = pick(pred)
         meas = {(p, lw): t for (p, lw, _, t) in rec["cands"]}
-        out.append({"spec": list(rec["spec"]), "pick": list(rec["cands"][k][:2]),
-                    "learned_ms": rec["cands"][k][3], "greedy_ms": meas[("greedy", 4)],
+        out.append({"spec": list(rec["spec"]), "pick": list(k),
+                    "learned_ms": meas[k], "greedy_ms": meas[BASELINE],
                     "best_ms": min(meas.values()), "predicted_ms": pred[k]})
     return out
 

@@ -102,6 +102,9 @@ Plan plan_gain(const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint3
 
 namespace {
 
+// Relative predicted gain a learned pick needs over greedy grouping.
+constexpr double kLearnedMargin = 0.02;
+
 struct Choice {
     int plan = LQCS_PLAN_GREEDY;
     int low = 0;
@@ -147,6 +150,7 @@ void step_features(const PlanStep& st, const std::vector<DecodedOp>& ops, uint32
                 switch (kind) {
                     case RK_DENSE1: f[free ? 6 : 7] += 1.0; break;
                     case RK_X: f[free ? 8 : 9] += 1.0; break;
+                    case RK_SEL1: f[h.la ? 7 : 6] += 1.0; break;  // a 2x2 op; per-lane select like a control
                     case RK_DENSE2: f[10] += 1.0; break;
                     case RK_DIAG: f[11] += 1.0; break;
                     default: f[12] += 1.0; break;
@@ -215,13 +219,17 @@ Plan choose_plan(const lqcs_exec_options* opts, const std::vector<DecodedOp>& op
     }
     first[cands.size()] = rows.size() / F;
     const std::vector<double> ms = cost_model_predict_ms(rows, rows.size() / F);
-    size_t best = 0;
+    size_t best = 0, base = cands.size();
     for (size_t c = 0; c < cands.size(); ++c) {
         double t = 0.0;
         for (size_t r = first[c]; r < first[c + 1]; ++r) t += ms[r];
         meta[c].predicted_ms = t;
         if (t < meta[best].predicted_ms || c == 0) best = c;
+        if (meta[c].plan == LQCS_PLAN_GREEDY && meta[c].low == static_cast<int>(low)) base = c;
     }
+    // The plan greedy grouping would run stays unless another is predicted
+    // to be faster by kLearnedMargin: a near tie is within the model's error.
+    if (base < cands.size() && meta[best].predicted_ms > (1.0 - kLearnedMargin) * meta[base].predicted_ms) best = base;
     g_last_choice = meta[best];
     return std::move(cands[best]);
 }

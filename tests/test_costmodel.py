@@ -84,8 +84,22 @@ def test_plan_features_describe_the_plan(lq):
         assert [int(x) for x in f[:, 3]] == [p["phases"] for p in d]
         assert [int(x) for x in f[:, 17]] == [p["ops"] for p in d]
         assert np.all(f[:, 16] >= low)  # contiguous low qubits of every tile
-        # op counts by kind add up to the ops of each pass
-        assert np.all(f[:, 6:13].sum(axis=1) == f[:, 17])
+        # op counts by kind add up to the ops of each pass, less the
+        # singly-controlled X ops merged into select ops (one per select op;
+        # the merge count itself is checked exactly by the register-tile
+        # emulator)
+        kinds = f[:, 6:13].sum(axis=1)
+        assert np.all(kinds <= f[:, 17]) and np.all(f[:, 17] - kinds <= kinds)
+    # with no X in the circuit nothing merges: the counts add up exactly
+    qd = lq.QuantumCircuit(n)
+    rng = np.random.default_rng(3)
+    for _ in range(6):
+        for q in range(n):
+            qd.u(*(float(v) for v in rng.uniform(0, 6.28, 3)), q)
+        for q in range(0, n - 1, 2):
+            qd.cp(float(rng.uniform(0, 6.28)), q, q + 1)
+    fd = lq.CompiledCircuit(qd, True).features(plan="greedy", tile_qubits=12, low_qubits=4)
+    assert np.all(fd[:, 6:13].sum(axis=1) == fd[:, 17])
     # without an installed model the learned plan is the greedy one
     lq.costmodel_uninstall()
     g = comp.features(plan="greedy", tile_qubits=12, low_qubits=4)
@@ -183,11 +197,13 @@ def test_trained_model_gains_over_greedy(lq):
             sv.set_basis(0)
             comp.apply(sv, plan=plan, tile_qubits=12, low_qubits=low)
             assert sv.max_abs_diff(ref) <= 1e-12
-        pick = min(pred, key=pred.get)
+        pick = CM.pick(pred)
         sv.set_basis(0)
         comp.apply(sv, plan="learned")  # the engine makes the same pick (barring a near tie)
         ranked = sorted(pred.values())
-        if ranked[1] - ranked[0] > 1e-9 * ranked[0]:
+        best = min(pred, key=pred.get)
+        margin_tie = abs(pred[best] - (1 - CM.MARGIN) * pred[CM.BASELINE]) <= 1e-9 * pred[best]
+        if ranked[1] - ranked[0] > 1e-9 * ranked[0] and not margin_tie:
             assert (lq.last_plan_choice()["plan"], lq.last_plan_choice()["low_qubits"]) == pick
         tot["learned"] += meas[pick]
         tot["greedy"] += meas[("greedy", 4)]
