@@ -73,3 +73,21 @@ def test_per_call_tile_kernels_agree(lq):
         sv = lq.statevector(n)
         comp.apply(sv, plan="greedy", tile_qubits=12, low_qubits=4, tile_kernel=kernel)
         assert sv.max_abs_diff(ref) <= 1e-12, kernel
+
+
+@pytest.mark.gpu
+def test_register_tile_on_a_1gib_register_matches_gate_by_gate(lq):
+    """At 26 qubits (1 GiB) the register tile streams the state with
+    non-temporal loads and stores; its state equals the gate-by-gate kernels'
+    (a separate set of kernels) within 1e-12, with U3/CX layers whose CX merge
+    into select ops."""
+    n = 26
+    qc = lq.QuantumCircuit(n)
+    W.random_u3_cx_layers(qc, n, 4, seed=26)
+    comp = lq.CompiledCircuit(qc, True)
+    a = lq.statevector(n)
+    comp.apply(a, plan="greedy", tile_qubits=12, low_qubits=4, tile_kernel="register")
+    b = lq.statevector(n)
+    comp.apply(b, plan="gate_by_gate")
+    assert a.max_abs_diff(b) <= 1e-12
+    assert abs(a.norm() - 1.0) <= 1e-10
