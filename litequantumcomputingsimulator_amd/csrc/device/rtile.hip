@@ -460,12 +460,21 @@ __device__ __forceinline__ void run_phases(Regs& v, Regs& w, amp_t* lds, const R
                 }
                 lds_used = true;
             }
-            copy_regs(w, v);
-            pin_a(v);
         }
         jt_prev = jt;
         uint32_t off = ph.op_off;
         uint32_t i = 0;
+        if (p > 0 && ph.transpose) {
+            // The transposed tile is in w: the phase's first op reads it from
+            // there (no copy back), or it is copied when the phase has none.
+            if (ph.n_ops > 0) {
+                off = apply_op<0>(w, v, words, off, jt, ph, tile);
+                i = 1;
+            } else {
+                copy_regs(w, v);
+                pin_a(v);
+            }
+        }
         for (; i + 1 < ph.n_ops; i += 2) {
             off = apply_op<1>(v, w, words, off, jt, ph, tile);
             off = apply_op<0>(w, v, words, off, jt, ph, tile);
