@@ -758,14 +758,17 @@ uint32_t matrix_words(uint32_t kind) {
     }
 }
 
-// Merges each singly-controlled X (a CX) with the uncontrolled dense 2x2 op
-// U on its target that comes next on that qubit, or else with the one just
-// before it: U CX (or CX U) is one 2x2 op on the target whose matrix the
-// control bit selects, U where it is 0 and U X (or X U) where it is 1. The
-// merged op stands where the CX stood; U moves there past ops that do not
-// touch its qubit. It costs what U alone costs, and the CX's register moves
-// are gone. `sel` marks the merged ops (their entries 4..7 hold the second
-// matrix). Returns how many ops were absorbed.
+// Merges each singly-controlled 2x2 op C-V (V = X, as in CX, or a diagonal,
+// as in CZ and CP) with the uncontrolled dense 2x2 op U on its target that
+// comes next on that qubit, or else with the one just before it: U C-V (or
+// C-V U) is one 2x2 op on the target whose matrix the control bit selects, U
+// where it is 0 and U V (or V U) where it is 1. The merged op stands where
+// C-V stood; U moves there past ops that do not touch its qubit. A
+// controlled phase (V = diag(1, p)) is symmetric in its two qubits, so it may
+// also merge with a U on its control, selected by its target. The merged op
+// costs what U alone costs: the X's register moves, or the diagonal's
+// multiplications, are gone. `sel` marks the merged ops (their entries 4..7
+// hold the second matrix). Returns how many ops were absorbed.
 uint32_t merge_selects(std::vector<TileOp>& ops, std::vector<uint8_t>& sel) {
     const size_t n = ops.size();
     std::vector<uint8_t> dead(n, 0);
@@ -776,46 +779,79 @@ uint32_t merge_selects(std::vector<TileOp>& ops, std::vector<uint8_t>& sel) {
         const TileOp& u = ops[i];
         return !sel[i] && u.kind == TK_DENSE1 && u.t0 == t && u.lctrl == 0 && u.gctrl == 0 && u.member < 0;
     };
-    uint32_t merged = 0;
-    for (size_t k = 0; k < n; ++k) {
-        TileOp& x = ops[k];
-        if (dead[k] || x.kind != TK_DENSE1 || !is_x(x) || x.member >= 0) continue;
-        if (popc(x.lctrl) + popc(x.gctrl) != 1) continue;
-        const uint32_t t = x.t0, tb = 1u << t;
-        size_t u = n;
-        bool fwd = false;
+    // The plain 2x2 op on tile bit t next to op k: the next op on t if it is
+    // one, else the previous op on t if that is one; n when neither is.
+    auto partner = [&](size_t k, uint32_t t, bool& fwd) -> size_t {
+        const uint32_t tb = 1u << t;
         for (size_t j = k + 1; j < n; ++j) {
             if (dead[j] || !(touched[j] & tb)) continue;
             if (plain_u(j, t)) {
-                u = j;
                 fwd = true;
+                return j;
             }
             break;
         }
-        if (!fwd) {
-            for (size_t j = k; j-- > 0;) {
-                if (dead[j] || !(touched[j] & tb)) continue;
-                if (plain_u(j, t)) u = j;
-                break;
+        for (size_t j = k; j-- > 0;) {
+            if (dead[j] || !(touched[j] & tb)) continue;
+            if (plain_u(j, t)) {
+                fwd = false;
+                return j;
+            }
+            break;
+        }
+        return n;
+    };
+    auto mul2 = [](const amp_t* a, const amp_t* b, amp_t* c) {
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) c[2 * i + j] = cadd(cmul(a[2 * i], b[j]), cmul(a[2 * i + 1], b[2 + j]));
+    };
+    const amp_t zero = make_double2(0.0, 0.0), one = make_double2(1.0, 0.0);
+    uint32_t merged = 0;
+    for (size_t k = 0; k < n; ++k) {
+        const TileOp& x = ops[k];
+        if (dead[k] || sel[k] || x.member >= 0) continue;
+        if (popc(x.lctrl) + popc(x.gctrl) != 1) continue;
+        amp_t V[4];
+        uint32_t t = 0;
+        bool symmetric = false;
+        if (x.kind == TK_DENSE1 && is_x(x)) {
+            t = x.t0;
+            V[0] = zero, V[1] = one, V[2] = one, V[3] = zero;
+        } else if (x.kind == TK_DIAG && popc(x.dl0) == 1 && x.dl1 == 0 && x.dg0 == 0 && x.dg1 == 0) {
+            t = static_cast<uint32_t>(__builtin_ctz(x.dl0));
+            V[0] = x.m[0], V[1] = zero, V[2] = zero, V[3] = x.m[1];
+            symmetric = x.m[0].x == 1.0 && x.m[0].y == 0.0 && x.lctrl != 0;
+        } else {
+            continue;
+        }
+        uint32_t lsel = x.lctrl;
+        uint64_t gsel = x.gctrl;
+        bool fwd = false;
+        size_t u = partner(k, t, fwd);
+        if (u == n && symmetric) {  // diag(1, p) on the control, selected by the target
+            const uint32_t t2 = static_cast<uint32_t>(__builtin_ctz(x.lctrl));
+            u = partner(k, t2, fwd);
+            if (u != n) {
+                lsel = 1u << t;
+                gsel = 0;
+                t = t2;
             }
         }
         if (u == n) continue;
-        const amp_t u00 = ops[u].m[0], u01 = ops[u].m[1], u10 = ops[u].m[2], u11 = ops[u].m[3];
-        x.m[0] = u00;
-        x.m[1] = u01;
-        x.m[2] = u10;
-        x.m[3] = u11;
-        if (fwd) {  // U X: columns swapped
-            x.m[4] = u01;
-            x.m[5] = u00;
-            x.m[6] = u11;
-            x.m[7] = u10;
-        } else {  // X U: rows swapped
-            x.m[4] = u10;
-            x.m[5] = u11;
-            x.m[6] = u00;
-            x.m[7] = u01;
-        }
+        TileOp m = x;
+        m.kind = TK_DENSE1;
+        m.t0 = t;
+        m.lctrl = lsel;
+        m.gctrl = gsel;
+        m.dl0 = m.dl1 = 0;
+        m.dg0 = m.dg1 = 0;
+        for (int e = 0; e < 4; ++e) m.m[e] = ops[u].m[e];
+        if (fwd)  // U after C-V: U V where the bit is 1
+            mul2(ops[u].m, V, m.m + 4);
+        else      // U before C-V: V U
+            mul2(V, ops[u].m, m.m + 4);
+        ops[k] = m;
+        local_masks(ops[k], touched[k], dense[k]);
         sel[k] = 1;
         dead[u] = 1;
         ++merged;

@@ -138,7 +138,21 @@ int main(int argc, char** argv) {
             t.member = -1;
             t.kind = lqcs_dev::TK_DENSE1;
             t.t0 = static_cast<uint32_t>(rng() % kL);
-            if (rng() % 3 == 0) {
+            const int two = static_cast<int>(rng() % 6);
+            if (two == 1 || two == 2) {  // controlled diagonal (CZ, CP or a general one)
+                t.kind = lqcs_dev::TK_DIAG;
+                t.dl0 = 1u << t.t0;
+                t.m[0] = two == 1 ? make_double2(1, 0) : make_double2(U(rng), U(rng));
+                t.m[1] = make_double2(U(rng), U(rng));
+                if (rng() % 4 == 0 && outside) {
+                    t.gctrl = outside_bit();
+                } else {
+                    uint32_t b;
+                    do b = static_cast<uint32_t>(rng() % kL); while (b == t.t0);
+                    t.lctrl = 1u << b;
+                }
+                t.t0 = 0;
+            } else if (two == 0) {
                 t.m[1] = make_double2(1, 0);
                 t.m[2] = make_double2(1, 0);
                 if (rng() % 4 == 0 && outside) {
