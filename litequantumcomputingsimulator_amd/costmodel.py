@@ -110,14 +110,14 @@ CANDIDATES = tuple((plan, low) for plan in ("greedy", "gain") for low in (2, 3, 
 # The engine keeps greedy grouping at the default low qubits (4 at these
 # widths) unless another candidate is predicted faster by this fraction.
 MARGIN = 0.02
-BASELINE = ("greedy", 4)
+GREEDY_DEFAULT = ("greedy", 4)
 
 
 def pick(pred: dict) -> tuple:
     """The engine's LEARNED choice among predicted totals {(plan, low): ms}."""
     best = min(pred, key=pred.get)
-    if BASELINE in pred and pred[best] > (1.0 - MARGIN) * pred[BASELINE]:
-        return BASELINE
+    if GREEDY_DEFAULT in pred and pred[best] > (1.0 - MARGIN) * pred[GREEDY_DEFAULT]:
+        return GREEDY_DEFAULT
     return best
 
 
@@ -174,7 +174,7 @@ def evaluate(lq, model: "Model", records):
         k = pick(pred)
         meas = {(p, lw): t for (p, lw, _, t) in rec["cands"]}
         out.append({"spec": list(rec["spec"]), "pick": list(k),
-                    "learned_ms": meas[k], "greedy_ms": meas[BASELINE],
+                    "learned_ms": meas[k], "greedy_ms": meas[GREEDY_DEFAULT],
                     "best_ms": min(meas.values()), "predicted_ms": pred[k]})
     return out
 

@@ -202,7 +202,7 @@ def test_trained_model_gains_over_greedy(lq):
         comp.apply(sv, plan="learned")  # the engine makes the same pick (barring a near tie)
         ranked = sorted(pred.values())
         best = min(pred, key=pred.get)
-        margin_tie = abs(pred[best] - (1 - CM.MARGIN) * pred[CM.BASELINE]) <= 1e-9 * pred[best]
+        margin_tie = abs(pred[best] - (1 - CM.MARGIN) * pred[CM.GREEDY_DEFAULT]) <= 1e-9 * pred[best]
         if ranked[1] - ranked[0] > 1e-9 * ranked[0] and not margin_tie:
             assert (lq.last_plan_choice()["plan"], lq.last_plan_choice()["low_qubits"]) == pick
         tot["learned"] += meas[pick]
