@@ -90,14 +90,15 @@ def test_plan_features_describe_the_plan(lq):
         # emulator)
         kinds = f[:, 6:13].sum(axis=1)
         assert np.all(kinds <= f[:, 17]) and np.all(f[:, 17] - kinds <= kinds)
-    # with no X in the circuit nothing merges: the counts add up exactly
+    # with no singly-controlled gate in the circuit nothing merges: the counts
+    # add up exactly
     qd = lq.QuantumCircuit(n)
     rng = np.random.default_rng(3)
     for _ in range(6):
         for q in range(n):
             qd.u(*(float(v) for v in rng.uniform(0, 6.28, 3)), q)
         for q in range(0, n - 1, 2):
-            qd.cp(float(rng.uniform(0, 6.28)), q, q + 1)
+            qd.iswap(q, q + 1)
     fd = lq.CompiledCircuit(qd, True).features(plan="greedy", tile_qubits=12, low_qubits=4)
     assert np.all(fd[:, 6:13].sum(axis=1) == fd[:, 17])
     # without an installed model the learned plan is the greedy one
