@@ -432,7 +432,7 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
         }
         // Draw offsets per step and a consistency check against D.
         std::vector<uint64_t> off(n_steps);
-        uint64_t d = 0, sites = 0;
+        uint64_t d = 0, n_unitary = 0;
         double events_per_shot = 0.0;
         for (uint64_t i = 0; i < n_steps; ++i) {
             off[i] = d;
@@ -440,7 +440,7 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
             if (st.kind == LQCS_TOP_UNITARY) {
                 if (st.op_index >= ops.size()) throw Error(LQCS_ERR_INVALID_ARGUMENT, "step op index out of range");
                 d += uint64_t(n_channels) * st.n_noise_qubits;
-                sites += uint64_t(n_channels) * st.n_noise_qubits;
+                ++n_unitary;
                 for (uint32_t c = 0; c < n_channels; ++c) {
                     // Only state-dependent events split the shared run (mixture
                     // branches ride along as single-register ops).
@@ -466,9 +466,14 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
         uint64_t term = n_steps;
         while (term > 0 && steps[term - 1].kind == LQCS_TOP_MEASURE) --term;
 
-        // Batch size: at most max_batch and ~1/2 of free HBM; and small enough
-        // that the batch's state-dependent events leave runs of ~25 noise sites
-        // between them (each splits the shared op sequence for the whole batch).
+        // Batch size: at most max_batch and ~1/2 of free HBM. Each
+        // state-dependent event of any register ends the batch's shared run,
+        // one more pass over every register: a batch of B registers takes
+        // about P0 + e B passes (P0 without events, e events per shot), which
+        // per shot grows with B. So B is the fewest registers whose tiles fill
+        // the GPU a few times over, or, where events are rarer, the most that
+        // keep e B within a fifth of P0. (2048 shots of the 24-qubit noisy
+        // workload: 115 trajectories/s at B = 79, 274 at B = 4.)
         size_t free_b = 0, total_b = 0;
         LQCS_HIP(hipMemGetInfo(&free_b, &total_b));
         const uint64_t reg_bytes = (uint64_t{1} << nq) * sizeof(amp_t);
@@ -476,8 +481,12 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
         B = std::min<uint64_t>(B, max_batch ? max_batch : 65535);
         B = std::min<uint64_t>(B, n_shots);
         B = std::min<uint64_t>(B, 65535);
-        if (!max_batch && events_per_shot > 0.0 && nq >= 14) {
-            const double target = std::max(1.0, double(sites) / (25.0 * events_per_shot));
+        if (!max_batch && events_per_shot > 0.0) {
+            const TileChoice t = default_tile(nq);
+            const double tiles = std::ldexp(1.0, static_cast<int>(nq) - static_cast<int>(t.L ? t.L : nq));
+            const double fill = std::ceil(4.0 * 3.0 * num_cus() / tiles);
+            const double p0 = std::max(1.0, double(n_unitary) / 30.0);
+            const double target = std::max({1.0, fill, std::floor(0.2 * p0 / events_per_shot)});
             B = std::min<uint64_t>(B, static_cast<uint64_t>(target));
         }
         amp_t* states = nullptr;
