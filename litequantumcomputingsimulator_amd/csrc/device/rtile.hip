@@ -203,21 +203,36 @@ __device__ __forceinline__ void x_masked(const Regs& a, Regs& o, uint32_t rc, bo
     }
 }
 
-template <int B0, int B1>
+// Row k of a 4x4 op times x: one chain of multiply-adds per component.
+__device__ __forceinline__ amp_t row4(const amp_t* m, const amp_t (&x)[4]) {
+    const amp_t m0 = uni(m), m1 = uni(m + 1), m2 = uni(m + 2), m3 = uni(m + 3);
+    return make_double2(m0.x * x[0].x - m0.y * x[0].y + m1.x * x[1].x - m1.y * x[1].y + m2.x * x[2].x -
+                            m2.y * x[2].y + m3.x * x[3].x - m3.y * x[3].y,
+                        m0.x * x[0].y + m0.y * x[0].x + m1.x * x[1].y + m1.y * x[1].x + m2.x * x[2].y +
+                            m2.y * x[2].x + m3.x * x[3].y + m3.y * x[3].x);
+}
+
+// 4x4 op on register bits B0 (local bit 0) and B1. `masked` keeps the
+// amplitudes of groups whose register bits miss a control, or of lanes whose
+// thread bits miss one; without controls there is nothing to select.
+template <int B0, int B1, bool MASKED>
 __device__ __forceinline__ void d2(const Regs& a, Regs& o, const amp_t* m, uint32_t rc, bool tp) {
 #pragma unroll
     for (int r = 0; r < kRR; ++r) {
         if (r & ((1 << B0) | (1 << B1))) continue;
         const int idx[4] = {r, r | (1 << B0), r | (1 << B1), r | (1 << B0) | (1 << B1)};
-        const bool on = tp && ((uint32_t(r) & rc) == rc);
         amp_t x[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) x[k] = a[idx[k]];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const amp_t acc = cmac2(uni(m + 4 * k), x[0], uni(m + 4 * k + 1), x[1]);
-            const amp_t n = cadd(acc, cmac2(uni(m + 4 * k + 2), x[2], uni(m + 4 * k + 3), x[3]));
-            o[idx[k]] = pick(on, n, x[k]);
+            const amp_t n = row4(m + 4 * k, x);
+            if constexpr (MASKED) {
+                const bool on = tp && ((uint32_t(r) & rc) == rc);
+                o[idx[k]] = pick(on, n, x[k]);
+            } else {
+                o[idx[k]] = n;
+            }
         }
     }
 }
@@ -334,19 +349,35 @@ __device__ __forceinline__ void apply_reg_op(const Regs& a, Regs& o, const Hdr& 
                 default: x_masked<3>(a, o, h.rctl, tp); pin_out<OUT>(o); return;
             }
         case RK_DENSE2:
+            if (free) {
+                switch (h.b0 * 4 + h.b1) {
+                    case 1: d2<0, 1, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 2: d2<0, 2, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 3: d2<0, 3, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 4: d2<1, 0, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 6: d2<1, 2, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 7: d2<1, 3, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 8: d2<2, 0, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 9: d2<2, 1, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 11: d2<2, 3, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 12: d2<3, 0, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 13: d2<3, 1, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    default: d2<3, 2, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                }
+            }
             switch (h.b0 * 4 + h.b1) {
-                case 1: d2<0, 1>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
-                case 2: d2<0, 2>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
-                case 3: d2<0, 3>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
-                case 4: d2<1, 0>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
-                case 6: d2<1, 2>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
-                case 7: d2<1, 3>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
-                case 8: d2<2, 0>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
-                case 9: d2<2, 1>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
-                case 11: d2<2, 3>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
-                case 12: d2<3, 0>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
-                case 13: d2<3, 1>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
-                default: d2<3, 2>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 1: d2<0, 1, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 2: d2<0, 2, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 3: d2<0, 3, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 4: d2<1, 0, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 6: d2<1, 2, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 7: d2<1, 3, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 8: d2<2, 0, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 9: d2<2, 1, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 11: d2<2, 3, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 12: d2<3, 0, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 13: d2<3, 1, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                default: d2<3, 2, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
             }
         case RK_SEL1:
             if (h.xr) {
