@@ -320,8 +320,8 @@ def density_matrix_bench(lq, W, seed: int, widths=(12, 14, 16), cpu: bool = True
 def planner_32q(lq, W, seed: int, n: int = 32, depth: int = 40) -> dict:
     """The learned-planner workload: random U3/CX/CP layers of depth 40 at 32
     qubits (64 GiB per register); applies with greedy grouping and with the
-    learned plan on two registers, alternating, two of each, the faster of
-    each timed pair reported, and the two states compared on the device."""
+    learned plan on two registers, three of each in alternating order, the
+    fastest of each reported, and the two states compared on the device."""
     qc = lq.QuantumCircuit(n)
     W.random_u3_cx_cp(qc, n, depth, seed)
     comp = lq.CompiledCircuit(qc, True)
@@ -333,25 +333,29 @@ def planner_32q(lq, W, seed: int, n: int = 32, depth: int = 40) -> dict:
     b.set_basis(0)
     lq.synchronize()
     t_greedy = t_learned = float("inf")
-    for rep in range(2):
+    for rep in range(3):  # alternating order: greedy first, then learned first, ...
         if rep:
             a.set_basis(0)
             b.set_basis(0)
             lq.synchronize()
-        t0 = time.perf_counter()
-        comp.apply(a, plan="greedy")
-        lq.synchronize()
-        t_greedy = min(t_greedy, time.perf_counter() - t0)
-        t0 = time.perf_counter()
-        comp.apply(b, plan="learned")
-        lq.synchronize()
-        t_learned = min(t_learned, time.perf_counter() - t0)
+        for plan in (("greedy", "learned") if rep % 2 == 0 else ("learned", "greedy")):
+            reg = a if plan == "greedy" else b
+            t0 = time.perf_counter()
+            comp.apply(reg, plan=plan)
+            lq.synchronize()
+            dt = time.perf_counter() - t0
+            if plan == "greedy":
+                t_greedy = min(t_greedy, dt)
+            else:
+                t_learned = min(t_learned, dt)
     choice = lq.last_plan_choice()
     diff = a.max_abs_diff(b)
     log(f"planner at {n} qubits: greedy {t_greedy:.3f} s, learned {t_learned:.3f} s ({choice}), max diff {diff:.2e}")
     return {"workload": f"random U3/CX/CP layers, {n} qubits, depth {depth}, {qc.num_instructions()} gates",
             "greedy_s": round(t_greedy, 3), "learned_s": round(t_learned, 3), "learned_pick": choice,
-            "learned_vs_greedy": round(t_learned / t_greedy, 4), "max_abs_diff_states": diff}
+            "learned_vs_greedy": round(t_learned / t_greedy, 4),
+            "same_plan_as_greedy": choice["plan"] == "greedy" and choice["low_qubits"] == 4,
+            "max_abs_diff_states": diff}
 
 
 PEAK_FP64_TFLOPS = 78.6

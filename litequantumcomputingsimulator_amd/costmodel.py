@@ -237,7 +237,7 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description="time plans on this GPU, train the cost model on it, evaluate")
     ap.add_argument("--out", default=default_model_path())
     ap.add_argument("--summary", default=None, help="JSON file for the held-out evaluation")
-    ap.add_argument("--widths", default="24,26,28,30")
+    ap.add_argument("--widths", default="24,26,28,30,32")
     ap.add_argument("--epochs", type=int, default=400)
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args(argv)
