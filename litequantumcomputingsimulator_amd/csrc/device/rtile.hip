@@ -252,6 +252,46 @@ __device__ __forceinline__ void r_diag(const Regs& a, Regs& o, const Hdr& h, con
     }
 }
 
+// diag(d0, d1) on register bit B. ONE0: d0 is exactly 1, so the registers
+// whose bit B is 0 pass through. MASKED: controls select per lane and
+// register which amplitudes change.
+template <int B, bool ONE0, bool MASKED>
+__device__ __forceinline__ void dg1_reg(const Regs& a, Regs& o, const amp_t* m, uint32_t rc, bool tp) {
+    const amp_t d0 = uni(m), d1 = uni(m + 1);
+#pragma unroll
+    for (int r = 0; r < kRR; ++r) {
+        const bool hi = ((r >> B) & 1) != 0;
+        if (ONE0 && !hi) {
+            o[r] = a[r];
+            continue;
+        }
+        const amp_t n = cmul(a[r], hi ? d1 : d0);
+        if constexpr (MASKED) {
+            const bool on = tp && ((uint32_t(r) & rc) == rc);
+            o[r] = pick(on, n, a[r]);
+        } else {
+            o[r] = n;
+        }
+    }
+}
+
+// diag(d0, d1) on a qubit that is the same for all of a lane's registers (a
+// thread bit, or a bit of the tile number): one factor per lane.
+template <bool MASKED>
+__device__ __forceinline__ void dg1_uniform(const Regs& a, Regs& o, const amp_t* m, bool s, uint32_t rc, bool tp) {
+    const amp_t d = pick(s, uni(m + 1), uni(m));
+#pragma unroll
+    for (int r = 0; r < kRR; ++r) {
+        const amp_t n = cmul(a[r], d);
+        if constexpr (MASKED) {
+            const bool on = tp && ((uint32_t(r) & rc) == rc);
+            o[r] = pick(on, n, a[r]);
+        } else {
+            o[r] = n;
+        }
+    }
+}
+
 template <int XR>
 __device__ __forceinline__ void r_pauli(const Regs& a, Regs& o, const Hdr& h, const amp_t* m, uint32_t jt,
                                         const RPhase& ph, uint32_t tile) {
@@ -408,6 +448,38 @@ __device__ __forceinline__ void apply_reg_op(const Regs& a, Regs& o, const Hdr& 
         case RK_DIAG:
             r_diag(a, o, h, m, jt, ph, tile, tp); pin_out<OUT>(o);
             return;
+        case RK_DIAG1:
+            if (h.xr) {
+                const bool one0 = h.b1 != 0;
+                switch ((h.xr - 1) | (one0 ? 4u : 0u) | (free ? 0u : 8u)) {
+                    case 0: dg1_reg<0, false, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 1: dg1_reg<1, false, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 2: dg1_reg<2, false, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 3: dg1_reg<3, false, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 4: dg1_reg<0, true, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 5: dg1_reg<1, true, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 6: dg1_reg<2, true, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 7: dg1_reg<3, true, false>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 8: dg1_reg<0, false, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 9: dg1_reg<1, false, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 10: dg1_reg<2, false, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 11: dg1_reg<3, false, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 12: dg1_reg<0, true, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 13: dg1_reg<1, true, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    case 14: dg1_reg<2, true, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                    default: dg1_reg<3, true, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                }
+            }
+            {
+                const bool s = h.la ? (jt & h.la) != 0 : (tile & h.ga) != 0;
+                if (free) {
+                    dg1_uniform<false>(a, o, m, s, h.rctl, tp);
+                } else {
+                    dg1_uniform<true>(a, o, m, s, h.rctl, tp);
+                }
+                pin_out<OUT>(o);
+                return;
+            }
         case RK_PAULI:
             switch (h.xr) {
                 case 0: r_pauli_diag(a, o, h, m, jt, ph, tile); pin_out<OUT>(o); return;
@@ -785,6 +857,7 @@ uint32_t matrix_words(uint32_t kind) {
         case RK_DENSE2: return 16;
         case RK_PAULI: return 2;
         case RK_SEL1: return 8;
+        case RK_DIAG1: return 2;
         default: return 0;
     }
 }
@@ -1033,6 +1106,17 @@ bool build_rtile(const std::vector<TileOp>& tops_in, uint32_t L, uint64_t smask,
                     b1 = ridx(t.t1);
                     break;
                 case TK_DIAG:
+                    if (t.dl1 == 0 && t.dg1 == 0 && popc(t.dl0) + popc(t.dg0) == 1) {  // one qubit
+                        kind = RK_DIAG1;
+                        if (t.dl0 & rmask)
+                            xr = 1 + ridx(static_cast<uint32_t>(__builtin_ctz(t.dl0)));
+                        else if (t.dl0)
+                            h.la = t.dl0;
+                        else
+                            h.ga = pext_host(t.dg0, outside);
+                        b1 = (t.m[0].x == 1.0 && t.m[0].y == 0.0) ? 1u : 0u;
+                        break;
+                    }
                     kind = RK_DIAG;
                     h.la = t.dl0;
                     h.lb = t.dl1;

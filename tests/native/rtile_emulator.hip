@@ -205,6 +205,7 @@ int main(int argc, char** argv) {
             } else if (kind <= 7) {
                 t.kind = lqcs_dev::TK_DIAG;
                 rand_m(4);
+                if (rng() % 3 == 0) t.m[0] = make_double2(1, 0);  // diag(1, p): CZ, CP, P, T
                 if (rng() % 3) t.dl0 = 1u << (rng() % kL); else t.dg0 = outside_bit();
                 if (rng() % 2) {
                     if (rng() % 3) {
@@ -336,6 +337,16 @@ int main(int argc, char** argv) {
                                         const uint32_t g = ((tile & h.ga) ? 1u : 0u) | ((tile & h.gb) ? 2u : 0u);
                                         const uint32_t sel = g | ((J(r) & h.la) ? 1u : 0u) | ((J(r) & h.lb) ? 2u : 0u);
                                         if (on) o[J(r)] = a[J(r)] * C(m[sel]);
+                                        break;
+                                    }
+                                    case lqcs_dev::RK_DIAG1: {
+                                        const bool hi = xr ? ((r >> (xr - 1)) & 1) != 0
+                                                           : h.la ? (J(r) & h.la) != 0 : (tile & h.ga) != 0;
+                                        if (b1 && !(m[0].x == 1.0 && m[0].y == 0.0)) {
+                                            why = "DIAG1 marked d0 = 1 with another d0";
+                                            return -1;
+                                        }
+                                        if (on) o[J(r)] = a[J(r)] * C(m[hi ? 1 : 0]);
                                         break;
                                     }
                                     case lqcs_dev::RK_PAULI: {
