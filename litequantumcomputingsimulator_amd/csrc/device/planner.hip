@@ -151,7 +151,8 @@ void step_features(const PlanStep& st, const std::vector<DecodedOp>& ops, uint32
                     case RK_DENSE1: f[free ? 6 : 7] += 1.0; break;
                     case RK_X: f[free ? 8 : 9] += 1.0; break;
                     case RK_SEL1: f[h.la ? 7 : 6] += 1.0; break;  // a 2x2 op; per-lane select like a control
-                    case RK_DENSE2: f[10] += 1.0; break;
+                    case RK_DENSE2:
+                    case RK_DENSE2R: f[10] += 1.0; break;
                     case RK_DIAG:
                     case RK_DIAG1: f[11] += 1.0; break;
                     default: f[12] += 1.0; break;

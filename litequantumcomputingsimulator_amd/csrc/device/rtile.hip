@@ -212,10 +212,18 @@ __device__ __forceinline__ amp_t row4(const amp_t* m, const amp_t (&x)[4]) {
                             m2.y * x[2].x + m3.x * x[3].y + m3.y * x[3].x);
 }
 
+// Row k of a real 4x4 op times x (entries' imaginary parts zero): half the
+// multiply-adds of a complex row.
+__device__ __forceinline__ amp_t row4r(const amp_t* m, const amp_t (&x)[4]) {
+    const double m0 = uni(m).x, m1 = uni(m + 1).x, m2 = uni(m + 2).x, m3 = uni(m + 3).x;
+    return make_double2(m0 * x[0].x + m1 * x[1].x + m2 * x[2].x + m3 * x[3].x,
+                        m0 * x[0].y + m1 * x[1].y + m2 * x[2].y + m3 * x[3].y);
+}
+
 // 4x4 op on register bits B0 (local bit 0) and B1. `masked` keeps the
 // amplitudes of groups whose register bits miss a control, or of lanes whose
 // thread bits miss one; without controls there is nothing to select.
-template <int B0, int B1, bool MASKED>
+template <int B0, int B1, bool MASKED, bool REAL = false>
 __device__ __forceinline__ void d2(const Regs& a, Regs& o, const amp_t* m, uint32_t rc, bool tp) {
 #pragma unroll
     for (int r = 0; r < kRR; ++r) {
@@ -226,7 +234,7 @@ __device__ __forceinline__ void d2(const Regs& a, Regs& o, const amp_t* m, uint3
         for (int k = 0; k < 4; ++k) x[k] = a[idx[k]];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const amp_t n = row4(m + 4 * k, x);
+            const amp_t n = REAL ? row4r(m + 4 * k, x) : row4(m + 4 * k, x);
             if constexpr (MASKED) {
                 const bool on = tp && ((uint32_t(r) & rc) == rc);
                 o[idx[k]] = pick(on, n, x[k]);
@@ -387,6 +395,21 @@ __device__ __forceinline__ void apply_reg_op(const Regs& a, Regs& o, const Hdr& 
                 case 1: x_masked<1>(a, o, h.rctl, tp); pin_out<OUT>(o); return;
                 case 2: x_masked<2>(a, o, h.rctl, tp); pin_out<OUT>(o); return;
                 default: x_masked<3>(a, o, h.rctl, tp); pin_out<OUT>(o); return;
+            }
+        case RK_DENSE2R:  // real entries, no controls
+            switch (h.b0 * 4 + h.b1) {
+                case 1: d2<0, 1, false, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 2: d2<0, 2, false, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 3: d2<0, 3, false, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 4: d2<1, 0, false, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 6: d2<1, 2, false, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 7: d2<1, 3, false, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 8: d2<2, 0, false, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 9: d2<2, 1, false, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 11: d2<2, 3, false, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 12: d2<3, 0, false, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                case 13: d2<3, 1, false, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
+                default: d2<3, 2, false, true>(a, o, m, h.rctl, tp); pin_out<OUT>(o); return;
             }
         case RK_DENSE2:
             if (free) {
@@ -858,6 +881,7 @@ uint32_t matrix_words(uint32_t kind) {
         case RK_PAULI: return 2;
         case RK_SEL1: return 8;
         case RK_DIAG1: return 2;
+        case RK_DENSE2R: return 16;
         default: return 0;
     }
 }
@@ -1100,11 +1124,14 @@ bool build_rtile(const std::vector<TileOp>& tops_in, uint32_t L, uint64_t smask,
                     }
                     kind = is_x(t) ? RK_X : RK_DENSE1;
                     break;
-                case TK_DENSE2:
-                    kind = RK_DENSE2;
+                case TK_DENSE2: {
+                    bool real = rctl == 0 && h.tctl == 0;
+                    for (int e = 0; e < 16 && real; ++e) real = t.m[e].y == 0.0;
+                    kind = real ? RK_DENSE2R : RK_DENSE2;
                     b0 = ridx(t.t0);
                     b1 = ridx(t.t1);
                     break;
+                }
                 case TK_DIAG:
                     if (t.dl1 == 0 && t.dg1 == 0 && popc(t.dl0) + popc(t.dg0) == 1) {  // one qubit
                         kind = RK_DIAG1;

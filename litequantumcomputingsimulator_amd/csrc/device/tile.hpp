@@ -33,7 +33,8 @@ struct TileOp {
 };
 
 // Register-resident tile (rtile.hip): per-phase layout and in-register ops.
-enum RegKind : uint32_t { RK_DENSE1 = 0, RK_DENSE2 = 1, RK_X = 2, RK_DIAG = 3, RK_PAULI = 4, RK_SEL1 = 5, RK_DIAG1 = 6 };
+enum RegKind : uint32_t { RK_DENSE1 = 0, RK_DENSE2 = 1, RK_X = 2, RK_DIAG = 3, RK_PAULI = 4, RK_SEL1 = 5, RK_DIAG1 = 6,
+                          RK_DENSE2R = 7 };
 
 // Header of one in-register op in a pass's op words: two 16-byte words, then
 // the op's matrix entries (4 for DENSE1 and DIAG, 16 for DENSE2, 2 for
@@ -43,7 +44,9 @@ enum RegKind : uint32_t { RK_DENSE1 = 0, RK_DENSE2 = 1, RK_X = 2, RK_DIAG = 3, R
 // is nonzero, else the tile-local thread bit la when nonzero, else the
 // tile-number bit ga. DIAG1 is diag(d0, d1) on one qubit (2 entries): the
 // qubit is register bit xr - 1 when xr is nonzero, else thread bit la when
-// nonzero, else tile-number bit ga; b1 = 1 marks d0 exactly 1. Masks named "tile number" are over the qubits outside
+// nonzero, else tile-number bit ga; b1 = 1 marks d0 exactly 1. DENSE2R is
+// a DENSE2 without controls whose entries are all real (16 entries, imaginary
+// parts zero), such as the superoperator of a real Kraus channel. Masks named "tile number" are over the qubits outside
 // the tile, compressed to the tile's index. The kernel reads them with
 // scalar loads (the same address in every lane).
 struct RegOpHdr {

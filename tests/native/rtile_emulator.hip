@@ -201,6 +201,8 @@ int main(int argc, char** argv) {
                 t.t0 = pick_bit();
                 do t.t1 = pick_bit(); while (t.t1 == t.t0);
                 rand_m(16);
+                if (rng() % 2)  // real entries (a real channel's superoperator)
+                    for (int e = 0; e < 16; ++e) t.m[e].y = 0.0;
                 ctrl((1u << t.t0) | (1u << t.t1));
             } else if (kind <= 7) {
                 t.kind = lqcs_dev::TK_DIAG;
@@ -321,6 +323,17 @@ int main(int argc, char** argv) {
                                         o[J(r)] = C(ms[2 * row]) * a[J(r0)] + C(ms[2 * row + 1]) * a[J(r1)];
                                         break;
                                     }
+                                    case lqcs_dev::RK_DENSE2R:
+                                        if (rctl || h.tctl) {
+                                            why = "DENSE2R with controls";
+                                            return -1;
+                                        }
+                                        for (int e = 0; e < 16; ++e)
+                                            if (m[e].y != 0.0) {
+                                                why = "DENSE2R with a complex entry";
+                                                return -1;
+                                            }
+                                        [[fallthrough]];
                                     case lqcs_dev::RK_DENSE2: {
                                         const int z = r & ~((1 << b0) | (1 << b1));
                                         const bool onz = tp && ((uint32_t(z) & rctl) == rctl);
