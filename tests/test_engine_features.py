@@ -91,3 +91,54 @@ def test_register_tile_on_a_1gib_register_matches_gate_by_gate(lq):
     comp.apply(b, plan="gate_by_gate")
     assert a.max_abs_diff(b) <= 1e-12
     assert abs(a.norm() - 1.0) <= 1e-10
+
+
+def _diag_rich_circuit(lq, n, seed):
+    rng = np.random.default_rng(seed)
+    qc = lq.QuantumCircuit(n)
+    for _ in range(120):
+        k = int(rng.integers(0, 11))
+        a, b, c = (int(v) for v in rng.choice(n, 3, replace=False))
+        th = float(rng.uniform(0, 6.28))
+        if k == 0:
+            qc.rz(th, a)
+        elif k == 1:
+            qc.p(th, a)
+        elif k == 2:
+            qc.t(a).s(b)
+        elif k == 3:
+            qc.cz(a, b)
+        elif k == 4:
+            qc.cp(th, a, b)
+        elif k == 5:
+            qc.crz(th, a, b)
+        elif k == 6:
+            qc.mcp(th, [a, b], c)
+        elif k == 7:
+            qc.cx(a, b)
+        elif k == 8:
+            qc.cy(a, b)
+        else:
+            qc.u(*(float(v) for v in rng.uniform(0, 6.28, 3)), a)
+    return qc
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [11, 12, 13])
+def test_diagonal_and_select_forms_match_gate_by_gate(lq, seed):
+    """Circuits dense in single-qubit diagonals (RZ, P, T, S), controlled
+    diagonals (CZ, CP, CRZ, MCP) and CX/CY next to U3 gates, at 16 qubits
+    (tiles of 12): the register tile's diagonal, select and controlled forms,
+    on register, thread and tile-number bits, with and without controls,
+    give the gate-by-gate kernels' state within 1e-12, with and without the
+    host's single-qubit fusion."""
+    n = 16
+    qc = _diag_rich_circuit(lq, n, seed)
+    for fuse in (False, True):
+        comp = lq.CompiledCircuit(qc, fuse)
+        a = lq.statevector(n)
+        comp.apply(a, plan="greedy", tile_qubits=12, low_qubits=4, tile_kernel="register")
+        b = lq.statevector(n)
+        comp.apply(b, plan="gate_by_gate")
+        assert a.max_abs_diff(b) <= 1e-12, fuse
+        assert abs(a.norm() - 1.0) <= 1e-10
