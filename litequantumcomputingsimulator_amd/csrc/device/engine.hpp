@@ -96,6 +96,7 @@ void last_plan_choice(int* plan, int* low, double* predicted_ms);
 bool rtile_enabled();
 // Installed cost model (mlp.hip).
 bool cost_model_installed();
+uint64_t cost_model_version();
 std::vector<double> cost_model_predict_ms(const std::vector<double>& raw_features, uint64_t rows);
 void set_last_stats(uint64_t passes, uint64_t launches);
 // Runs f under the engine lock, mapping exceptions to an lqcs_status and the

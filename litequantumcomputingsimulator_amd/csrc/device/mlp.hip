@@ -235,7 +235,15 @@ CostModel& cost_model() {
 
 }  // namespace
 
+// Counts installs and removals, so that decisions taken with one model are
+// not reused under another.
+uint64_t& cost_model_counter() {
+    static uint64_t n = 0;
+    return n;
+}
+
 bool cost_model_installed() { return cost_model().on; }
+uint64_t cost_model_version() { return cost_model_counter(); }
 uint32_t cost_model_features() { return cost_model().F; }
 
 // Predicted milliseconds of each feature row (raw features, normalised here),
@@ -381,6 +389,7 @@ lqcs_status lqcs_costmodel_install(uint32_t n_features, const double* params, co
                                    const double* x_scale, double t_mean, double t_scale) {
     return run_guarded([&] {
         CostModel& cm = cost_model();
+        ++cost_model_counter();
         if (!params) {
             cm = CostModel{};
             return;

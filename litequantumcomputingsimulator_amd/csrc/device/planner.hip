@@ -112,6 +112,54 @@ struct Choice {
 };
 thread_local Choice g_last_choice;
 
+// The learned choice is a function of the ops, the width, the batch, the
+// options, the engine switches the plans depend on, and the installed model.
+// Repeated applies of one program (a compiled circuit) reuse it instead of
+// planning six candidates and running the network again: tens of
+// milliseconds of host time per apply at 32 qubits.
+struct CachedChoice {
+    uint64_t key;
+    Choice choice;
+};
+thread_local std::vector<CachedChoice> g_choice_cache;
+
+struct Fnv {
+    uint64_t h = 1469598103934665603ull;
+    void bytes(const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    }
+    template <typename T>
+    void pod(const T& v) { bytes(&v, sizeof v); }
+};
+
+uint64_t choice_key(const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t batch, uint32_t L, uint32_t low_req) {
+    Fnv f;
+    f.pod(nq);
+    f.pod(batch);
+    f.pod(L);
+    f.pod(low_req);
+    f.pod(cost_model_version());
+    f.pod(rtile_enabled());
+    f.pod(g_dense_mfma_min_targets);
+    for (const DecodedOp& d : ops) {
+        f.pod(d.kind);
+        f.pod(d.member);
+        f.pod(d.only);
+        f.pod(d.x);
+        f.pod(d.zy);
+        f.pod(d.n_y);
+        f.pod(d.theta);
+        f.bytes(d.controls.data(), d.controls.size() * sizeof(uint32_t));
+        f.pod(uint32_t(0xffffffffu));
+        f.bytes(d.targets.data(), d.targets.size() * sizeof(uint32_t));
+        f.pod(uint32_t(0xffffffffu));
+        f.bytes(d.m.data(), d.m.size() * sizeof(amp_t));
+        f.bytes(d.perm.data(), d.perm.size() * sizeof(uint64_t));
+    }
+    return f.h;
+}
+
 uint32_t lowest_bits_qubits_at_least(uint64_t smask, uint32_t tmask, uint32_t lanes, uint32_t qmin) {
     // Of the `lanes` lowest tile positions in tmask, how many map to qubits >= qmin.
     uint32_t count = 0, seen = 0;
@@ -195,6 +243,12 @@ Plan choose_plan(const lqcs_exec_options* opts, const std::vector<DecodedOp>& op
         return make_plan(k, ops, nq, L, low);
     }
     ensure_ready();
+    const uint64_t key = choice_key(ops, nq, batch, L, low_req);
+    for (const CachedChoice& c : g_choice_cache) {
+        if (c.key != key) continue;
+        g_last_choice = c.choice;
+        return make_plan(c.choice.plan, ops, nq, L, static_cast<uint32_t>(c.choice.low));
+    }
     std::vector<Plan> cands;
     std::vector<Choice> meta;
     for (int k : {LQCS_PLAN_GREEDY, LQCS_PLAN_GAIN}) {
@@ -233,6 +287,8 @@ Plan choose_plan(const lqcs_exec_options* opts, const std::vector<DecodedOp>& op
     // to be faster by kLearnedMargin: a near tie is within the model's error.
     if (base < cands.size() && meta[best].predicted_ms > (1.0 - kLearnedMargin) * meta[base].predicted_ms) best = base;
     g_last_choice = meta[best];
+    if (g_choice_cache.size() >= 16) g_choice_cache.erase(g_choice_cache.begin());
+    g_choice_cache.push_back(CachedChoice{key, meta[best]});
     return std::move(cands[best]);
 }
 
