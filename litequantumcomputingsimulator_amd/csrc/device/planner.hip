@@ -193,7 +193,7 @@ void step_features(const PlanStep& st, const std::vector<DecodedOp>& ops, uint32
             for (size_t i = 0; i + 1 < rp.words.size();) {
                 RegOpHdr h;
                 std::memcpy(&h, &rp.words[i], sizeof h);
-                const uint32_t kind = h.code & 15u, rctl = (h.code >> 12) & 15u, nm = h.code >> 16;
+                const uint32_t kind = h.code & 15u, rctl = (h.code >> 12) & 15u, nm = (h.code >> 16) & 0xFFFu;
                 const bool free = rctl == 0 && h.tctl == 0;
                 switch (kind) {
                     case RK_DENSE1: f[free ? 6 : 7] += 1.0; break;

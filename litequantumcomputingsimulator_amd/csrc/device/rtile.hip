@@ -64,6 +64,8 @@ using Regs = amp_t[kRR];
 // An op's uniform fields, decoded from its header.
 struct Hdr {
     uint32_t kind, b0, b1, xr, rctl, tctl, la, lb, gctrl, ga, gb, member;
+    bool restricted;  // a tile-number control or a batch-register restriction to check
+    bool free;        // no controls among the register or thread bits
 };
 
 __device__ __forceinline__ uint32_t rfl(uint32_t v) {
@@ -334,12 +336,13 @@ __device__ __forceinline__ void r_pauli_diag(const Regs& a, Regs& o, const Hdr& 
 template <int OUT>
 __device__ __forceinline__ void apply_reg_op(const Regs& a, Regs& o, const Hdr& h, const amp_t* m, uint32_t jt,
                                              const RPhase& ph, uint32_t tile) {
-    if ((tile & h.gctrl) != h.gctrl ||  // tile-uniform control not met
-        (h.member && ((h.member & 0x7fffffffu) - 1 == blockIdx.y) != (h.member >> 31 != 0))) {  // batch register
+    if (h.restricted &&
+        ((tile & h.gctrl) != h.gctrl ||  // tile-uniform control not met
+         (h.member && ((h.member & 0x7fffffffu) - 1 == blockIdx.y) != (h.member >> 31 != 0)))) {  // batch register
         copy_regs(a, o); pin_out<OUT>(o);
         return;
     }
-    const bool free = (h.rctl | h.tctl) == 0;
+    const bool free = h.free;
     const bool tp = (jt & h.tctl) == h.tctl;
     // A select op whose select bit is outside the tile is a plain 2x2 op with
     // one of its two matrices, the same for the whole tile.
@@ -545,6 +548,8 @@ __device__ __forceinline__ uint32_t apply_op(const Regs& a, Regs& o, const amp_t
     h.b1 = (code >> 6) & 3u;
     h.xr = (code >> 8) & 15u;
     h.rctl = (code >> 12) & 15u;
+    h.restricted = (code & kCodeRestricted) != 0;
+    h.free = (code & kCodeFree) != 0;
     h.tctl = rfl(h0.y);
     h.la = rfl(h0.z);
     h.lb = rfl(h0.w);
@@ -553,7 +558,7 @@ __device__ __forceinline__ uint32_t apply_op(const Regs& a, Regs& o, const amp_t
     h.gb = rfl(h1.z);
     h.member = rfl(h1.w);
     apply_reg_op<OUT>(a, o, h, words + off + 2, jt, ph, tile);
-    return off + 2 + (code >> 16);
+    return off + 2 + ((code >> 16) & 0xFFFu);
 }
 
 // s_waitcnt vmcnt(0) (gfx9 encoding: expcnt and lgkmcnt fields at their
@@ -1164,6 +1169,8 @@ bool build_rtile(const std::vector<TileOp>& tops_in, uint32_t L, uint64_t smask,
             }
             const uint32_t nm = matrix_words(kind);
             h.code = kind | (b0 << 4) | (b1 << 6) | (xr << 8) | (rctl << 12) | (nm << 16);
+            if (h.gctrl || h.member) h.code |= kCodeRestricted;
+            if (rctl == 0 && h.tctl == 0) h.code |= kCodeFree;
             put_hdr(h);
             for (uint32_t e = 0; e < nm; ++e) out.words.push_back(t.m[e]);
             ++ph.n_ops;

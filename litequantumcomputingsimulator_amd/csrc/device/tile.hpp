@@ -49,8 +49,14 @@ enum RegKind : uint32_t { RK_DENSE1 = 0, RK_DENSE2 = 1, RK_X = 2, RK_DIAG = 3, R
 // parts zero), such as the superoperator of a real Kraus channel. Masks named "tile number" are over the qubits outside
 // the tile, compressed to the tile's index. The kernel reads them with
 // scalar loads (the same address in every lane).
+// Flags of an op's code word, precomputed by the builder so that the kernel's
+// dispatch tests one bit: the op has a tile-number control or a batch-register
+// restriction to check; the op has no controls among the register or thread bits.
+constexpr uint32_t kCodeRestricted = 1u << 28;
+constexpr uint32_t kCodeFree = 1u << 29;
+
 struct RegOpHdr {
-    uint32_t code;    // kind | b0 << 4 | b1 << 6 | xr << 8 | rctl << 12 | matrix words << 16
+    uint32_t code;    // kind | b0 << 4 | b1 << 6 | xr << 8 | rctl << 12 | matrix words << 16 | flags
     uint32_t tctl;    // controls among the thread's tile bits (tile-local mask)
     uint32_t la, lb;  // DIAG: tile-local diagonal bits; PAULI: la = tile-local Z bits
     uint32_t gctrl;   // controls on the tile number

@@ -294,8 +294,16 @@ int main(int argc, char** argv) {
                     RegOpHdr h;
                     std::memcpy(&h, &prog.words[off], sizeof h);
                     const uint32_t kind = h.code & 15u, b0 = (h.code >> 4) & 3u, b1 = (h.code >> 6) & 3u;
-                    const uint32_t xr = (h.code >> 8) & 15u, rctl = (h.code >> 12) & 15u, nm = h.code >> 16;
+                    const uint32_t xr = (h.code >> 8) & 15u, rctl = (h.code >> 12) & 15u, nm = (h.code >> 16) & 0xFFFu;
                     if (off + 2 + nm > total) { why = "matrix outside the op words"; return -1; }
+                    if (((h.code & lqcs_dev::kCodeFree) != 0) != (rctl == 0 && h.tctl == 0)) {
+                        why = "free flag disagrees with the controls";
+                        return -1;
+                    }
+                    if (((h.code & lqcs_dev::kCodeRestricted) != 0) != (h.gctrl != 0 || h.member != 0)) {
+                        why = "restricted flag disagrees with the header";
+                        return -1;
+                    }
                     const amp_t* m = &prog.words[off + 2];
                     std::vector<cd> o = a;
                     if ((tile & h.gctrl) == h.gctrl) {
