@@ -223,6 +223,78 @@ def test_batched_adjoint_matches_single(lq):
         assert abs(e - O.bound_energy(t, pinst, theta, h)) <= 1e-10
 
 
+def _pauli_only(lq, n, seed):
+    """Pauli rotations only (no other gates), so that a batch of them shares
+    launches, with the cases that shape fused rotation groups: Z-only
+    strings (no X part), one rotation repeated, and three rotations whose
+    third X mask is the XOR of the first two."""
+    rots, theta = W.random_pauli_ansatz(n, 10, seed)
+    rots = list(rots)
+    theta = [float(x) for x in theta]
+    special = [("ZZ", [0, 1]), ("XY", [0, 2]), ("XY", [0, 2]), ("X", [0]), ("X", [1]), ("XX", [0, 1]),
+               ("Z", [n - 1]), ("YZX", [n - 1, 0, 1])]
+    rng = np.random.default_rng(seed + 7)
+    at = 3
+    for p, qs in special:
+        rots.insert(at, (p, qs))
+        theta.insert(at, float(rng.uniform(-math.pi, math.pi)))
+        at += 1
+    pc = lq.ParametricCircuit(n)
+    t = O.Circuit(n)
+    pinst = []
+    for p, qs in rots:
+        pc.rp(p, qs)
+        pinst.append(len(t.insts))
+        t.rp(0.0, p, qs)
+    return pc, t, pinst, theta
+
+
+@pytest.mark.parametrize("widths", [(3, 4, 5, 8), (2, 5), (7,)])
+def test_batched_adjoint_pauli_only_matches_oracle(lq, widths):
+    jobs = []
+    for k, n in enumerate(widths):
+        if n >= 3:
+            pc, t, pinst, theta = _pauli_only(lq, n, seed=40 + k)
+        else:  # two qubits: rotations on both
+            pc, t, pinst = lq.ParametricCircuit(n), O.Circuit(n), []
+            theta = [0.3, -1.1, 0.7, 2.0, -0.4]
+            for p, qs in [("XY", [0, 1]), ("Z", [1]), ("YY", [0, 1]), ("ZX", [0, 1]), ("X", [0])]:
+                pc.rp(p, qs)
+                pinst.append(len(t.insts))
+                t.rp(0.0, p, qs)
+        h = W.random_hamiltonian(n, 9, seed=60 + k)
+        jobs.append((pc, t, pinst, theta, h))
+    out = lq.algorithms.adjoint_gradient_batched([j[4] for j in jobs], [j[0] for j in jobs],
+                                                 [j[3] for j in jobs])
+    for (pc, t, pinst, theta, h), (e, g) in zip(jobs, out):
+        want = O.parameter_shift_gradient(t, pinst, theta, h)
+        assert np.max(np.abs(np.array(g) - want)) <= 1e-10
+        assert abs(e - O.bound_energy(t, pinst, theta, h)) <= 1e-10
+
+
+def test_torch_expectation_backward_matches_oracle(lq):
+    torch = pytest.importorskip("torch")
+    n = 6
+    pc, t, pinst, theta = _ansatz(lq, n, 10, seed=31)
+    h = W.random_hamiltonian(n, 10, seed=32)
+    want = O.parameter_shift_gradient(t, pinst, theta, h)
+    e_want = O.bound_energy(t, pinst, theta, h)
+    # theta stays on the host: the torch wheel carries its own HIP runtime,
+    # and whichever of it and the engine's runtime opens the device first
+    # holds it for the process, so this test never initialises torch's.
+    th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+    e = lq.expectation(th, pc, h)
+    (2.0 * e).backward()
+    assert np.max(np.abs(th.grad.numpy() - 2.0 * want)) <= 1e-10
+    assert abs(e.item() - e_want) <= 1e-10
+    # a float32 theta gets the float32-rounded gradient at its own values
+    th32 = torch.tensor(theta, dtype=torch.float32, requires_grad=True)
+    lq.expectation(th32, pc, h).backward()
+    assert th32.grad.dtype == torch.float32
+    want32 = O.parameter_shift_gradient(t, pinst, [float(x) for x in th32.detach().numpy()], h)
+    assert np.max(np.abs(th32.grad.numpy().astype(np.float64) - want32)) <= 1e-5
+
+
 def test_errors_keep_reference_types(lq):
     with pytest.raises(ValueError):
         lq.Statevector(31)
