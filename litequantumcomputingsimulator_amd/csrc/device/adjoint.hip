@@ -8,6 +8,10 @@
 // amplitude, energy = Re <psi|lam> from the same pass), then one backward
 // sweep in which each Pauli-rotation step reads psi and lam once, accumulates
 // its inner product and un-rotates both registers in the same kernel.
+// Consecutive rotations run fused (k_fwd_group, k_bwd_group): up to four of
+// them whose X masks span at most three dimensions share one pass over the
+// registers, each thread holding one coset of eight amplitudes; registers of
+// fewer than three qubits take the one-step kernels (k_fwd, k_bwd).
 //
 // The batched form runs many circuits of different widths in one launch per
 // op step: a flattened block list maps blocks to circuits, each circuit
