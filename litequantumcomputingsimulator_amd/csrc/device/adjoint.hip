@@ -8,10 +8,11 @@
 // amplitude, energy = Re <psi|lam> from the same pass), then one backward
 // sweep in which each Pauli-rotation step reads psi and lam once, accumulates
 // its inner product and un-rotates both registers in the same kernel.
-// Consecutive rotations run fused (k_fwd_group, k_bwd_group): up to four of
-// them whose X masks span at most three dimensions share one pass over the
-// registers, each thread holding one coset of eight amplitudes; registers of
-// fewer than three qubits take the one-step kernels (k_fwd, k_bwd).
+// Consecutive rotations run fused (k_fwd_group, k_bwd_group): up to D + 1
+// of them whose X masks span at most D dimensions share one pass over the
+// registers, each thread holding one coset of 2^D amplitudes (D = 4 forward,
+// 3 backward); registers of fewer than three qubits take the one-step
+// kernels (k_fwd, k_bwd).
 //
 // The batched form runs many circuits of different widths in one launch per
 // op step: a flattened block list maps blocks to circuits, each circuit
@@ -50,7 +51,7 @@ struct CircuitDesc {
     uint32_t n_ops;
     uint64_t op_off;
     uint64_t term_off, n_terms;
-    uint32_t n_groups;   // fused rotation groups (all-Pauli circuits of width >= 3)
+    uint32_t n_groups;   // fused rotation groups of this sweep (widths >= 3)
     uint32_t group_off;
 };
 
@@ -207,14 +208,14 @@ __global__ void __launch_bounds__(kBlock) k_bwd(amp_t* __restrict__ base, const 
 }
 
 // ---- fused Pauli-rotation steps ----------------------------------------------
-// Up to kGroupSteps consecutive rotations whose X masks span at most three
-// dimensions run as one pass: the register splits into cosets of an
-// 8-element subgroup {off[a]} containing every X mask, each thread loads
-// one coset (8 amplitudes, index i0 ^ off[a], i0 with the pivot bits zero)
+// Up to D + 1 consecutive rotations whose X masks span at most D dimensions
+// (D = 3 or 4) run as one pass: the register splits into cosets of a
+// 2^D-element subgroup {off[a]} containing every X mask, each thread loads
+// one coset (2^D amplitudes, index i0 ^ off[a], i0 with the pivot bits zero)
 // and applies the rotations in registers. Rotation k pairs coset element a
 // with a ^ coef, since X_k = off[coef]; the Z sign of index i0 ^ off[a] is
 // parity(i0 & zy) ^ bit a of sgn.
-constexpr uint32_t kGroupSteps = 4;
+constexpr uint32_t kMaxGroupDim = 4;
 
 struct GroupStep {
     uint64_t zy;
@@ -225,44 +226,44 @@ struct GroupStep {
 };
 
 struct Group {
-    uint64_t off[8];   // XOR combinations of the three basis masks
-    uint32_t piv[3];   // pivot bits, ascending: bit piv[t] of off[a] is bit t of a
+    uint64_t off[1u << kMaxGroupDim];  // XOR combinations of the basis masks
+    uint32_t piv[kMaxGroupDim];        // pivot bits, ascending: bit piv[t] of off[a] is bit t of a
     uint32_t n_steps;
     uint32_t step_off;
-    uint32_t pad[3];
 };
 
 // new[e] = c old[e] + y sign(e ^ C) old[e ^ C], sign bits in sb.
-template <uint32_t C>
-__device__ __forceinline__ void rot8(amp_t (&v)[8], amp_t c, amp_t y, uint32_t sb) {
-    amp_t t[8];
+template <uint32_t NE, uint32_t C>
+__device__ __forceinline__ void rot_c(amp_t (&v)[NE], amp_t c, amp_t y, uint32_t sb) {
+    amp_t t[NE];
 #pragma unroll
-    for (uint32_t e = 0; e < 8; ++e) {
+    for (uint32_t e = 0; e < NE; ++e) {
         const amp_t ys = ((sb >> (e ^ C)) & 1u) ? cscale(y, -1.0) : y;
         t[e] = cmac2(c, v[e], ys, v[e ^ C]);
     }
 #pragma unroll
-    for (uint32_t e = 0; e < 8; ++e) v[e] = t[e];
+    for (uint32_t e = 0; e < NE; ++e) v[e] = t[e];
 }
 
-__device__ __forceinline__ void rot8_any(amp_t (&v)[8], uint32_t coef, amp_t c, amp_t y, uint32_t sb) {
-    switch (coef) {
-        case 0: rot8<0>(v, c, y, sb); break;
-        case 1: rot8<1>(v, c, y, sb); break;
-        case 2: rot8<2>(v, c, y, sb); break;
-        case 3: rot8<3>(v, c, y, sb); break;
-        case 4: rot8<4>(v, c, y, sb); break;
-        case 5: rot8<5>(v, c, y, sb); break;
-        case 6: rot8<6>(v, c, y, sb); break;
-        default: rot8<7>(v, c, y, sb); break;
+// coef is uniform per launch: a chain of scalar compares selects the form
+// with constant register indices.
+template <uint32_t NE, uint32_t C = 0>
+__device__ __forceinline__ void rot_any(amp_t (&v)[NE], uint32_t coef, amp_t c, amp_t y, uint32_t sb) {
+    if constexpr (C + 1 < NE) {
+        if (coef != C) {
+            rot_any<NE, C + 1>(v, coef, c, y, sb);
+            return;
+        }
     }
+    rot_c<NE, C>(v, c, y, sb);
 }
 
 // sum_e conj(l[e]) sign(e ^ C) p[e ^ C]
-template <uint32_t C>
-__device__ __forceinline__ void dot8(const amp_t (&l)[8], const amp_t (&p)[8], uint32_t sb, double& sr, double& si) {
+template <uint32_t NE, uint32_t C>
+__device__ __forceinline__ void dot_c(const amp_t (&l)[NE], const amp_t (&p)[NE], uint32_t sb, double& sr,
+                                      double& si) {
 #pragma unroll
-    for (uint32_t e = 0; e < 8; ++e) {
+    for (uint32_t e = 0; e < NE; ++e) {
         const double sg = ((sb >> (e ^ C)) & 1u) ? -1.0 : 1.0;
         const amp_t a = l[e], b = p[e ^ C];
         sr += sg * (a.x * b.x + a.y * b.y);
@@ -270,100 +271,108 @@ __device__ __forceinline__ void dot8(const amp_t (&l)[8], const amp_t (&p)[8], u
     }
 }
 
-__device__ __forceinline__ void dot8_any(const amp_t (&l)[8], const amp_t (&p)[8], uint32_t coef, uint32_t sb,
-                                         double& sr, double& si) {
-    switch (coef) {
-        case 0: dot8<0>(l, p, sb, sr, si); break;
-        case 1: dot8<1>(l, p, sb, sr, si); break;
-        case 2: dot8<2>(l, p, sb, sr, si); break;
-        case 3: dot8<3>(l, p, sb, sr, si); break;
-        case 4: dot8<4>(l, p, sb, sr, si); break;
-        case 5: dot8<5>(l, p, sb, sr, si); break;
-        case 6: dot8<6>(l, p, sb, sr, si); break;
-        default: dot8<7>(l, p, sb, sr, si); break;
+template <uint32_t NE, uint32_t C = 0>
+__device__ __forceinline__ void dot_any(const amp_t (&l)[NE], const amp_t (&p)[NE], uint32_t coef, uint32_t sb,
+                                        double& sr, double& si) {
+    if constexpr (C + 1 < NE) {
+        if (coef != C) {
+            dot_any<NE, C + 1>(l, p, coef, sb, sr, si);
+            return;
+        }
     }
+    dot_c<NE, C>(l, p, sb, sr, si);
 }
 
+template <uint32_t D>
 __device__ __forceinline__ uint64_t coset_base(uint64_t g, const Group& G) {
-    return insert_zero_bit(insert_zero_bit(insert_zero_bit(g, G.piv[0]), G.piv[1]), G.piv[2]);
+#pragma unroll
+    for (uint32_t t = 0; t < D; ++t) g = insert_zero_bit(g, G.piv[t]);
+    return g;
 }
 
-__device__ __forceinline__ uint32_t sign_bits(uint64_t i0, const GroupStep& st) {
-    return (__popcll(i0 & st.zy) & 1) ? (st.sgn ^ 0xFFu) : st.sgn;
+__device__ __forceinline__ uint32_t sign_bits(uint64_t i0, const GroupStep& st, uint32_t all) {
+    return (__popcll(i0 & st.zy) & 1) ? (st.sgn ^ all) : st.sgn;
 }
 
-// Forward: group `grp` of every circuit.
+// Forward: group `grp` of every circuit, cosets of 2^D.
+template <uint32_t D>
 __global__ void __launch_bounds__(kBlock) k_fwd_group(amp_t* __restrict__ base, const CircuitDesc* __restrict__ cd,
                                                       uint32_t nc, const Group* __restrict__ groups,
                                                       const GroupStep* __restrict__ steps, uint32_t grp) {
+    constexpr uint32_t NE = 1u << D;
     const uint32_t c = find_circuit(cd, nc, blockIdx.x);
     const CircuitDesc d = cd[c];
     if (grp >= d.n_groups) return;
-    const Group G = groups[d.group_off + grp];
+    const Group& G = groups[d.group_off + grp];
     amp_t* s = base + d.reg_off;
     const uint32_t lb = blockIdx.x - d.blk_start;
     const uint64_t stride = uint64_t(d.blocks) * kBlock;
-    const uint64_t n_cosets = uint64_t{1} << (d.nq - 3);
+    const uint64_t n_cosets = uint64_t{1} << (d.nq - D);
+    const uint32_t n_steps = G.n_steps;
     for (uint64_t g = uint64_t(lb) * kBlock + threadIdx.x; g < n_cosets; g += stride) {
-        const uint64_t i0 = coset_base(g, G);
-        amp_t v[8];
+        const uint64_t i0 = coset_base<D>(g, G);
+        amp_t v[NE];
 #pragma unroll
-        for (uint32_t a = 0; a < 8; ++a) v[a] = s[i0 ^ G.off[a]];
+        for (uint32_t a = 0; a < NE; ++a) v[a] = s[i0 ^ G.off[a]];
 #pragma unroll
-        for (uint32_t k = 0; k < kGroupSteps; ++k) {
-            if (k >= G.n_steps) break;
-            const GroupStep st = steps[G.step_off + k];
-            rot8_any(v, st.coef, st.c, st.y, sign_bits(i0, st));
+        for (uint32_t k = 0; k < D + 1; ++k) {
+            if (k >= n_steps) break;
+            const GroupStep& st = steps[G.step_off + k];
+            rot_any<NE>(v, st.coef, st.c, st.y, sign_bits(i0, st, (1u << NE) - 1));
         }
 #pragma unroll
-        for (uint32_t a = 0; a < 8; ++a) s[i0 ^ G.off[a]] = v[a];
+        for (uint32_t a = 0; a < NE; ++a) s[i0 ^ G.off[a]] = v[a];
     }
 }
 
-// Backward: for the group's steps in reverse, S_k += <lam|sign P_k-shift|psi>
-// (before step k is undone), then psi, lam <- U_k^dag psi, U_k^dag lam.
+// Backward: for the group's steps in reverse, S_k += sum conj(lam) sign
+// psi-shifted (before step k is undone), then psi, lam <- U_k^dag.
+template <uint32_t D>
 __global__ void __launch_bounds__(kBlock) k_bwd_group(amp_t* __restrict__ base, const CircuitDesc* __restrict__ cd,
                                                       uint32_t nc, const Group* __restrict__ groups,
                                                       const GroupStep* __restrict__ steps, uint32_t grp,
                                                       double* __restrict__ gpart) {
+    constexpr uint32_t NE = 1u << D;
+    constexpr uint32_t kSteps = D + 1;
     const uint32_t c = find_circuit(cd, nc, blockIdx.x);
     const CircuitDesc d = cd[c];
     if (grp >= d.n_groups) return;
-    const Group G = groups[d.group_off + grp];
+    const Group& G = groups[d.group_off + grp];
     amp_t* psi = base + d.reg_off;
     amp_t* lam = psi + (uint64_t{1} << d.nq);
     const uint32_t lb = blockIdx.x - d.blk_start;
     const uint64_t stride = uint64_t(d.blocks) * kBlock;
-    const uint64_t n_cosets = uint64_t{1} << (d.nq - 3);
-    double sr[kGroupSteps] = {}, si[kGroupSteps] = {};
+    const uint64_t n_cosets = uint64_t{1} << (d.nq - D);
+    const uint32_t n_steps = G.n_steps;
+    double sr[kSteps] = {}, si[kSteps] = {};
     for (uint64_t g = uint64_t(lb) * kBlock + threadIdx.x; g < n_cosets; g += stride) {
-        const uint64_t i0 = coset_base(g, G);
-        amp_t p[8], l[8];
+        const uint64_t i0 = coset_base<D>(g, G);
+        amp_t p[NE], l[NE];
 #pragma unroll
-        for (uint32_t a = 0; a < 8; ++a) {
+        for (uint32_t a = 0; a < NE; ++a) {
             p[a] = psi[i0 ^ G.off[a]];
             l[a] = lam[i0 ^ G.off[a]];
         }
 #pragma unroll
-        for (uint32_t r = 0; r < kGroupSteps; ++r) {
-            const uint32_t k = kGroupSteps - 1 - r;
-            if (k >= G.n_steps) continue;
-            const GroupStep st = steps[G.step_off + k];
-            const uint32_t sb = sign_bits(i0, st);
-            dot8_any(l, p, st.coef, sb, sr[k], si[k]);
+        for (uint32_t r = 0; r < kSteps; ++r) {
+            const uint32_t k = kSteps - 1 - r;
+            if (k >= n_steps) continue;
+            const GroupStep& st = steps[G.step_off + k];
+            const uint32_t sb = sign_bits(i0, st, (1u << NE) - 1);
+            dot_any<NE>(l, p, st.coef, sb, sr[k], si[k]);
             const amp_t yinv = cscale(st.y, -1.0);
-            rot8_any(p, st.coef, st.c, yinv, sb);
-            rot8_any(l, st.coef, st.c, yinv, sb);
+            rot_any<NE>(p, st.coef, st.c, yinv, sb);
+            rot_any<NE>(l, st.coef, st.c, yinv, sb);
         }
 #pragma unroll
-        for (uint32_t a = 0; a < 8; ++a) {
+        for (uint32_t a = 0; a < NE; ++a) {
             psi[i0 ^ G.off[a]] = p[a];
             lam[i0 ^ G.off[a]] = l[a];
         }
     }
 #pragma unroll
-    for (uint32_t k = 0; k < kGroupSteps; ++k) {
-        if (k >= G.n_steps) break;
+    for (uint32_t k = 0; k < kSteps; ++k) {
+        if (k >= n_steps) break;
         const int64_t param = steps[G.step_off + k].param;
         if (param < 0) continue;  // uniform per block
         block_sum2(sr[k], si[k]);
@@ -456,17 +465,18 @@ struct Unit {
     uint32_t index;  // op index, or group index within the circuit
 };
 
-// Splits the Pauli rotations ops[first, last) into groups of at most
-// kGroupSteps whose X masks span at most three dimensions (greedy, in
-// order). The span's basis is kept in reduced row-echelon form, pivot =
-// highest set bit, and padded to three with the highest free qubits so
-// that every group walks cosets of eight.
-void group_rotations(const Job& j, size_t first, size_t last, std::vector<Group>& groups,
+// Splits the Pauli rotations ops[first, last) into groups of at most D + 1
+// whose X masks span at most D dimensions (greedy, in order). The span's
+// basis is kept in reduced row-echelon form, pivot = highest set bit, and
+// padded to D vectors with the highest free qubits so that every group
+// walks cosets of 2^D.
+void group_rotations(const Job& j, uint32_t D, size_t first, size_t last, std::vector<Group>& groups,
                      std::vector<GroupStep>& gsteps, std::vector<Unit>& units, uint32_t& n_groups) {
+    const uint32_t NE = 1u << D;
     size_t k = first;
     while (k < last) {
-        uint64_t basis[3] = {};
-        uint32_t piv[3] = {};
+        uint64_t basis[kMaxGroupDim] = {};
+        uint32_t piv[kMaxGroupDim] = {};
         uint32_t dim = 0;
         auto reduce = [&](uint64_t v) {
             for (uint32_t t = 0; t < dim; ++t)
@@ -481,26 +491,26 @@ void group_rotations(const Job& j, size_t first, size_t last, std::vector<Group>
             piv[dim++] = p;
         };
         size_t end = k;
-        while (end < last && end - k < kGroupSteps) {
+        while (end < last && end - k < D + 1) {
             const uint64_t v = reduce(j.ops[end].x);
             if (v != 0) {
-                if (dim == 3) break;
+                if (dim == D) break;
                 add(v);
             }
             ++end;
         }
-        for (int q = static_cast<int>(j.nq) - 1; q >= 0 && dim < 3; --q) {
+        for (int q = static_cast<int>(j.nq) - 1; q >= 0 && dim < D; --q) {
             bool used = false;
             for (uint32_t t = 0; t < dim; ++t) used = used || piv[t] == static_cast<uint32_t>(q);
             if (!used) add(uint64_t{1} << q);
         }
-        uint32_t order[3] = {0, 1, 2};
-        std::sort(order, order + 3, [&](uint32_t a, uint32_t b) { return piv[a] < piv[b]; });
+        uint32_t order[kMaxGroupDim] = {0, 1, 2, 3};
+        std::sort(order, order + D, [&](uint32_t a, uint32_t b) { return piv[a] < piv[b]; });
         Group G;
         std::memset(&G, 0, sizeof G);
-        for (uint32_t t = 0; t < 3; ++t) G.piv[t] = piv[order[t]];
-        for (uint32_t a = 0; a < 8; ++a)
-            for (uint32_t t = 0; t < 3; ++t)
+        for (uint32_t t = 0; t < D; ++t) G.piv[t] = piv[order[t]];
+        for (uint32_t a = 0; a < NE; ++a)
+            for (uint32_t t = 0; t < D; ++t)
                 if ((a >> t) & 1) G.off[a] ^= basis[order[t]];
         G.n_steps = static_cast<uint32_t>(end - k);
         G.step_off = static_cast<uint32_t>(gsteps.size());
@@ -512,9 +522,9 @@ void group_rotations(const Job& j, size_t first, size_t last, std::vector<Group>
             st.c = ps.c;
             st.y = ps.yphase;
             st.param = ps.param;
-            for (uint32_t t = 0; t < 3; ++t) st.coef |= static_cast<uint32_t>((ps.x >> G.piv[t]) & 1) << t;
+            for (uint32_t t = 0; t < D; ++t) st.coef |= static_cast<uint32_t>((ps.x >> G.piv[t]) & 1) << t;
             if (G.off[st.coef] != ps.x) throw Error(LQCS_ERR_LOGIC, "rotation group basis does not span the X mask");
-            for (uint32_t a = 0; a < 8; ++a)
+            for (uint32_t a = 0; a < NE; ++a)
                 st.sgn |= static_cast<uint32_t>(__builtin_popcountll(G.off[a] & ps.zy) & 1) << a;
             gsteps.push_back(st);
         }
@@ -524,10 +534,11 @@ void group_rotations(const Job& j, size_t first, size_t last, std::vector<Group>
     }
 }
 
-// The fused schedule of one circuit (width >= 3): maximal runs of Pauli
-// rotations become groups, every other op stays a unit of its own.
-void schedule_job(const Job& j, std::vector<Group>& groups, std::vector<GroupStep>& gsteps, std::vector<Unit>& units,
-                  uint32_t& n_groups) {
+// The fused schedule of one circuit (width >= D): maximal runs of Pauli
+// rotations become groups of dimension D, every other op stays a unit of
+// its own.
+void schedule_job(const Job& j, uint32_t D, std::vector<Group>& groups, std::vector<GroupStep>& gsteps,
+                  std::vector<Unit>& units, uint32_t& n_groups) {
     n_groups = 0;
     size_t s = 0;
     while (s < j.ops.size()) {
@@ -538,7 +549,7 @@ void schedule_job(const Job& j, std::vector<Group>& groups, std::vector<GroupSte
         }
         size_t e = s;
         while (e < j.ops.size() && j.ops[e].kind == LQCS_OP_PAULI_ROTATION) ++e;
-        group_rotations(j, s, e, groups, gsteps, units, n_groups);
+        group_rotations(j, D, s, e, groups, gsteps, units, n_groups);
         s = e;
     }
 }
@@ -557,13 +568,14 @@ void run_group(const std::vector<const Job*>& jobs, uint64_t n_params_total, boo
     std::vector<PauliStep> steps;
     std::vector<Term> terms;
     uint64_t reg = 0;
-    uint32_t blk = 0, max_ops = 0, max_groups = 0;
-    // Fused rotation groups walk cosets of eight, so every width must be >= 3.
-    bool fused = true;
-    for (const Job* j : jobs) fused = fused && j->nq >= 3;
-    std::vector<Group> groups;
-    std::vector<GroupStep> gsteps;
-    std::vector<Unit> units;  // the schedule of jobs[0] when `general`
+    uint32_t blk = 0, max_ops = 0;
+    // Fused rotation groups walk cosets of 2^D, so every width must be >= D.
+    // The forward holds psi alone and takes D = 4 where the widths allow it;
+    // the backward holds psi and lam and takes D = 3 (at D = 4 it spills).
+    uint32_t min_nq = 64;
+    for (const Job* j : jobs) min_nq = std::min(min_nq, j->nq);
+    const bool fused = min_nq >= 3;
+    const uint32_t Df = min_nq >= kMaxGroupDim ? kMaxGroupDim : 3, Db = 3;
     for (uint32_t c = 0; c < nc; ++c) {
         const Job& j = *jobs[c];
         cd[c].reg_off = reg;
@@ -575,14 +587,8 @@ void run_group(const std::vector<const Job*>& jobs, uint64_t n_params_total, boo
         cd[c].term_off = terms.size();
         cd[c].n_terms = j.terms.size();
         for (size_t k = 0; k < j.ops.size(); ++k) steps.push_back(make_step(j.ops[k], j.param_of_op[k]));
-        cd[c].group_off = static_cast<uint32_t>(groups.size());
+        cd[c].group_off = 0;
         cd[c].n_groups = 0;
-        if (fused) {
-            std::vector<Unit> u;
-            schedule_job(j, groups, gsteps, u, cd[c].n_groups);
-            if (c == 0) units.swap(u);
-            max_groups = std::max(max_groups, cd[c].n_groups);
-        }
         terms.insert(terms.end(), j.terms.begin(), j.terms.end());
         reg += uint64_t{2} << j.nq;
         blk += cd[c].blocks;
@@ -599,19 +605,52 @@ void run_group(const std::vector<const Job*>& jobs, uint64_t n_params_total, boo
     double* epart = static_cast<double*>(arena(sizeof(double) * 2 * kMaxBlocks * nc));
     double* gpart = static_cast<double*>(arena(sizeof(double) * 2 * kMaxBlocks * std::max<uint64_t>(1, n_params_total)));
 
-    const Group* d_groups = fused ? stage(groups.data(), std::max<size_t>(1, groups.size())) : nullptr;
-    const GroupStep* d_gsteps = fused ? stage(gsteps.data(), std::max<size_t>(1, gsteps.size())) : nullptr;
+    // One fused schedule per sweep: each circuit's groups and, for jobs[0]
+    // when `general`, its units in program order.
+    struct Sched {
+        std::vector<CircuitDesc> cd;
+        std::vector<Group> groups;
+        std::vector<GroupStep> steps;
+        std::vector<Unit> units;
+        uint32_t max_groups = 0;
+        const CircuitDesc* d_cd = nullptr;
+        const Group* d_groups = nullptr;
+        const GroupStep* d_steps = nullptr;
+    } sf, sb;
+    auto build_sched = [&](Sched& S, uint32_t D) {
+        S.cd = cd;
+        for (uint32_t c = 0; c < nc; ++c) {
+            S.cd[c].group_off = static_cast<uint32_t>(S.groups.size());
+            std::vector<Unit> u;
+            schedule_job(*jobs[c], D, S.groups, S.steps, u, S.cd[c].n_groups);
+            if (c == 0) S.units.swap(u);
+            S.max_groups = std::max(S.max_groups, S.cd[c].n_groups);
+        }
+        S.d_cd = stage(S.cd.data(), S.cd.size());
+        S.d_groups = stage(S.groups.data(), std::max<size_t>(1, S.groups.size()));
+        S.d_steps = stage(S.steps.data(), std::max<size_t>(1, S.steps.size()));
+    };
+    if (fused) {
+        build_sched(sf, Df);
+        build_sched(sb, Db);
+    }
+    auto fwd_group = [&](uint32_t g) {
+        if (Df == 4) k_fwd_group<4><<<blk, kBlock, 0, stream()>>>(base, sf.d_cd, nc, sf.d_groups, sf.d_steps, g);
+        else k_fwd_group<3><<<blk, kBlock, 0, stream()>>>(base, sf.d_cd, nc, sf.d_groups, sf.d_steps, g);
+    };
+    auto bwd_group = [&](uint32_t g) {
+        k_bwd_group<3><<<blk, kBlock, 0, stream()>>>(base, sb.d_cd, nc, sb.d_groups, sb.d_steps, g, gpart);
+    };
 
     k_zero2<<<blk, kBlock, 0, stream()>>>(base, d_cd, nc);
     // forward
     if (fused && general) {
-        for (const Unit& u : units) {
-            if (u.pauli) k_fwd_group<<<blk, kBlock, 0, stream()>>>(base, d_cd, nc, d_groups, d_gsteps, u.index);
+        for (const Unit& u : sf.units) {
+            if (u.pauli) fwd_group(u.index);
             else run_single_op(base, jobs[0]->nq, jobs[0]->ops[u.index], 1);
         }
     } else if (fused) {
-        for (uint32_t g = 0; g < max_groups; ++g)
-            k_fwd_group<<<blk, kBlock, 0, stream()>>>(base, d_cd, nc, d_groups, d_gsteps, g);
+        for (uint32_t g = 0; g < sf.max_groups; ++g) fwd_group(g);
     }
     for (uint32_t s = 0; !fused && s < max_ops; ++s) {
         if (general) {
@@ -626,14 +665,13 @@ void run_group(const std::vector<const Job*>& jobs, uint64_t n_params_total, boo
     k_hamiltonian<<<blk, kBlock, 0, stream()>>>(base, d_cd, nc, d_terms, epart);
     // backward
     if (fused && general) {
-        for (size_t k = units.size(); k-- > 0;) {
-            const Unit& u = units[k];
-            if (u.pauli) k_bwd_group<<<blk, kBlock, 0, stream()>>>(base, d_cd, nc, d_groups, d_gsteps, u.index, gpart);
+        for (size_t k = sb.units.size(); k-- > 0;) {
+            const Unit& u = sb.units[k];
+            if (u.pauli) bwd_group(u.index);
             else run_single_op(base, jobs[0]->nq, inverse_op(jobs[0]->ops[u.index]), 2);
         }
     } else if (fused) {
-        for (uint32_t g = max_groups; g-- > 0;)
-            k_bwd_group<<<blk, kBlock, 0, stream()>>>(base, d_cd, nc, d_groups, d_gsteps, g, gpart);
+        for (uint32_t g = sb.max_groups; g-- > 0;) bwd_group(g);
     }
     for (uint32_t s = max_ops; !fused && s-- > 0;) {
         if (general) {
