@@ -19,7 +19,7 @@ next to greedy's, on the headline workload and on random U3/CX/CP layers of
 depth 40 at 32 qubits, with the two final states compared). Further objects
 time fused dense blocks of 5 and 6 qubits (complex GEMMs on FP64 MFMA) at
 30 qubits against the FP64 peak, and the noisy workload's density matrix at
-12 and 14 qubits, each with the reference beside it.
+12, 14 and 16 qubits, each with the reference beside it.
 
 `traffic` (HBM bytes per tiled pass) comes from the rocprofv3 counter summary
 committed under profiles/ for the same kernel and workload, when one exists.
