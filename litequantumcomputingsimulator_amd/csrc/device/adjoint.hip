@@ -15,9 +15,10 @@
 // kernels (k_fwd, k_bwd).
 //
 // The batched form runs many circuits of different widths in one launch per
-// op step: a flattened block list maps blocks to circuits, each circuit
-// taking at most kMaxBlocks blocks (grid-stride inside), so per-parameter
-// partial sums have a fixed, deterministic layout.
+// rotation group (or per op step for the one-step kernels): a flattened
+// block list maps blocks to circuits, each circuit taking at most
+// kMaxBlocks blocks (grid-stride inside), so per-parameter partial sums
+// have a fixed, deterministic layout.
 #include "common.hpp"
 #include "engine.hpp"
 
