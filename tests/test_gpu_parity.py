@@ -272,6 +272,32 @@ def test_batched_adjoint_pauli_only_matches_oracle(lq, widths):
         assert abs(e - O.bound_energy(t, pinst, theta, h)) <= 1e-10
 
 
+def test_adjoint_on_28_qubit_registers_matches_parameter_shift(lq):
+    """Rotation groups on registers past 2^27 amplitudes (byte offsets past
+    4 GiB), where the oracle is too slow: the adjoint against the engine's
+    own parameter shift, which shares no kernel with it."""
+    n = 28
+
+    def pauli(letters):
+        s = ["I"] * n
+        for q, p in letters.items():
+            s[q] = p
+        return "".join(s)
+
+    rots = [("XY", [0, 27]), ("ZX", [3, 26]), ("YYZ", [27, 1, 14]), ("Z", [27]), ("XZ", [25, 27]),
+            ("YX", [0, 26]), ("XXY", [2, 13, 27])]
+    theta = [0.3, -1.2, 0.8, 2.1, -0.5, 1.4, -2.6]
+    h = [(0.7, pauli({27: "X", 0: "Y"})), (-1.1, pauli({26: "Z", 3: "Z"})), (0.4, pauli({14: "X", 27: "Z"})),
+         (0.9, pauli({1: "Y", 25: "Y", 2: "X"}))]
+    pc = lq.ParametricCircuit(n)
+    for p, qs in rots:
+        pc.rp(p, qs)
+    ps = np.array(lq.algorithms.parameter_shift_gradient(h, pc, theta))
+    e, adj = lq.algorithms.adjoint_gradient(h, pc, theta)
+    assert np.max(np.abs(np.array(adj) - ps)) <= 1e-10
+    assert np.max(np.abs(ps)) > 1e-3  # the check is not vacuous
+
+
 def test_torch_expectation_backward_matches_oracle(lq):
     torch = pytest.importorskip("torch")
     n = 6
