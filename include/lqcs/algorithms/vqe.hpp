@@ -1,13 +1,15 @@
-// Expectations and gradients of Pauli Hamiltonians (reference
-// include/lqcs/algorithms/vqe.hpp:15-52), plus the adjoint gradient.
+// Expectations, gradients and variational solvers of Pauli Hamiltonians
+// (reference include/lqcs/algorithms/vqe.hpp:12-64), plus the adjoint gradient.
 #pragma once
 
+#include <functional>
 #include <span>
 #include <string>
 #include <vector>
 
 #include "../backend/statevector.hpp"
 #include "../circuit/parametric_circuit.hpp"
+#include "../circuit/quantum_circuit.hpp"
 
 namespace lqcs::algorithms {
 
@@ -20,10 +22,43 @@ using Hamiltonian = std::vector<PauliTerm>;
 // <psi|H|psi>
 double expectation(const Statevector& sv, const Hamiltonian& hamiltonian);
 
+// Parameter vector -> circuit preparing |psi(theta)>.
+using Ansatz = std::function<QuantumCircuit(std::span<const double>)>;
+
+struct VQEOptions {
+    std::size_t max_iterations = 100;        // sweeps over all parameters
+    double tol = 1e-9;                       // stop when a sweep lowers E by less
+    std::vector<double> initial_parameters;  // empty: all zero
+};
+
+struct VQEResult {
+    double energy;                   // final (variational upper bound) energy
+    std::vector<double> parameters;  // parameters reaching it
+    std::size_t iterations;          // sweeps or steps taken
+    std::vector<double> history;     // energy after each sweep or step
+};
+
+// Rotosolve: every parameter is a rotation angle, so E is a sinusoid in each
+// coordinate and three energies fix its minimum.
+VQEResult vqe(const Hamiltonian& hamiltonian, const Ansatz& ansatz, std::size_t n_params,
+              const VQEOptions& options = {});
+
 // dE/dtheta_k = [E(theta + pi/2 e_k) - E(theta - pi/2 e_k)] / 2, 2P evaluations.
 std::vector<double> parameter_shift_gradient(const Hamiltonian& hamiltonian,
                                              const ParametricCircuit& ansatz,
                                              std::span<const double> params);
+
+struct GradientVQEOptions {
+    std::size_t max_iterations = 200;
+    double learning_rate = 0.1;
+    double tol = 1e-9;                       // stop when the gradient norm is below
+    std::vector<double> initial_parameters;  // empty: all zero
+};
+
+// Gradient descent on the exact gradient (computed here by the adjoint
+// method, equal to the parameter-shift gradient).
+VQEResult vqe_gradient_descent(const Hamiltonian& hamiltonian, const ParametricCircuit& ansatz,
+                               const GradientVQEOptions& options = {});
 
 // Engine extension: the same gradient from one forward and one backward
 // sweep (adjoint method). `energy`, when given, receives E(theta).

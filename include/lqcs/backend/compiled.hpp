@@ -73,7 +73,9 @@ private:
 
 // A compiled circuit's launches on one register, recorded once into a HIP
 // graph and replayed: launch() applies U to the register again (queued on
-// the engine stream; synchronize() waits). The register must outlive it.
+// the engine stream; synchronize() waits). The Statevector object must
+// outlive it; once the object holds another device register (assignment from
+// another Statevector), launch() throws std::logic_error.
 class CircuitGraph {
 public:
     CircuitGraph(const CompiledCircuit& circuit, Statevector& sv, const EngineOptions& engine = {});

@@ -131,14 +131,18 @@ lqcs_status lqcs_state_apply(lqcs_dev_state* s, const lqcs_program* prog,
 /* ---- captured launch sequences (no counterpart in the reference): the
  * launches lqcs_state_apply would issue for (s, prog, opts), recorded once
  * into a HIP graph with the tile programs in device memory owned by the
- * graph, then replayed on s by lqcs_graph_launch (asynchronous; ordered with
- * the engine's other work; lqcs_dev_synchronize waits). Stand-alone
+ * graph, then replayed by lqcs_graph_launch (asynchronous; ordered with
+ * the engine's other work; lqcs_dev_synchronize waits). The graph's kernels
+ * address the register s held at capture: lqcs_graph_launch takes the
+ * register to replay on and returns LOGIC unless it holds that same device
+ * buffer at the same width (a register replaced since, e.g. by assignment,
+ * is refused rather than written through a stale address). Stand-alone
  * controlled-unitary and controlled-permutation steps cannot be captured
  * (their tables are staged per call): INVALID_ARGUMENT. */
 typedef struct lqcs_graph lqcs_graph;
 lqcs_status lqcs_graph_create(lqcs_dev_state* s, const lqcs_program* prog, const lqcs_exec_options* opts,
                               lqcs_graph** out);
-lqcs_status lqcs_graph_launch(lqcs_graph* g);
+lqcs_status lqcs_graph_launch(lqcs_graph* g, lqcs_dev_state* s);
 lqcs_status lqcs_graph_destroy(lqcs_graph* g);
 
 /* Plan summary of the last lqcs_state_apply on this thread: HBM passes and

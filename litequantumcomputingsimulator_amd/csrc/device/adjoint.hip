@@ -454,10 +454,17 @@ DecodedOp inverse_op(const DecodedOp& d) {
 struct Job {
     uint32_t nq;
     std::vector<DecodedOp> ops;
-    std::vector<int64_t> param_of_op;  // global parameter index per op
+    std::vector<int64_t> param_of_op;  // the circuit's own parameter index per op, -1 for none
+    uint64_t n_params;
     std::vector<Term> terms;
     bool all_pauli;
 };
+
+// Slot of op k's parameter within a group whose slots for this job start at
+// `base`; -1 when the op carries no parameter.
+int64_t param_slot(const Job& j, size_t k, uint64_t base) {
+    return j.param_of_op[k] >= 0 ? static_cast<int64_t>(base) + j.param_of_op[k] : -1;
+}
 
 // One entry of a circuit's fused schedule: a non-Pauli op (run gate by gate)
 // or a group of consecutive Pauli rotations.
@@ -471,8 +478,9 @@ struct Unit {
 // basis is kept in reduced row-echelon form, pivot = highest set bit, and
 // padded to D vectors with the highest free qubits so that every group
 // walks cosets of 2^D.
-void group_rotations(const Job& j, uint32_t D, size_t first, size_t last, std::vector<Group>& groups,
-                     std::vector<GroupStep>& gsteps, std::vector<Unit>& units, uint32_t& n_groups) {
+void group_rotations(const Job& j, uint64_t param_base, uint32_t D, size_t first, size_t last,
+                     std::vector<Group>& groups, std::vector<GroupStep>& gsteps, std::vector<Unit>& units,
+                     uint32_t& n_groups) {
     const uint32_t NE = 1u << D;
     size_t k = first;
     while (k < last) {
@@ -516,7 +524,7 @@ void group_rotations(const Job& j, uint32_t D, size_t first, size_t last, std::v
         G.n_steps = static_cast<uint32_t>(end - k);
         G.step_off = static_cast<uint32_t>(gsteps.size());
         for (size_t s = k; s < end; ++s) {
-            const PauliStep ps = make_step(j.ops[s], j.param_of_op[s]);
+            const PauliStep ps = make_step(j.ops[s], param_slot(j, s, param_base));
             GroupStep st;
             std::memset(&st, 0, sizeof st);
             st.zy = ps.zy;
@@ -538,8 +546,8 @@ void group_rotations(const Job& j, uint32_t D, size_t first, size_t last, std::v
 // The fused schedule of one circuit (width >= D): maximal runs of Pauli
 // rotations become groups of dimension D, every other op stays a unit of
 // its own.
-void schedule_job(const Job& j, uint32_t D, std::vector<Group>& groups, std::vector<GroupStep>& gsteps,
-                  std::vector<Unit>& units, uint32_t& n_groups) {
+void schedule_job(const Job& j, uint64_t param_base, uint32_t D, std::vector<Group>& groups,
+                  std::vector<GroupStep>& gsteps, std::vector<Unit>& units, uint32_t& n_groups) {
     n_groups = 0;
     size_t s = 0;
     while (s < j.ops.size()) {
@@ -550,7 +558,7 @@ void schedule_job(const Job& j, uint32_t D, std::vector<Group>& groups, std::vec
         }
         size_t e = s;
         while (e < j.ops.size() && j.ops[e].kind == LQCS_OP_PAULI_ROTATION) ++e;
-        group_rotations(j, D, s, e, groups, gsteps, units, n_groups);
+        group_rotations(j, param_base, D, s, e, groups, gsteps, units, n_groups);
         s = e;
     }
 }
@@ -562,9 +570,19 @@ uint32_t blocks_for(uint32_t nq) {
 
 // Runs a group of jobs that share launches. When `general` is set the group
 // holds one job whose non-Pauli ops run through the gate-by-gate kernels.
-void run_group(const std::vector<const Job*>& jobs, uint64_t n_params_total, bool general,
-               std::vector<double>& energies_out, std::vector<double>& gpart_host) {
+// Parameter partials are indexed within the group: job c's parameter p sits
+// at slot param_base[c] + p, so the partial buffers and the reduction launch
+// scale with the group's parameters, not the whole batch's. `gsum` receives
+// the summed (re, im) of every slot.
+void run_group(const std::vector<const Job*>& jobs, bool general, std::vector<double>& energies_out,
+               std::vector<double>& gsum, std::vector<uint64_t>& param_base) {
     const uint32_t nc = static_cast<uint32_t>(jobs.size());
+    param_base.assign(nc, 0);
+    uint64_t n_slots = 0;
+    for (uint32_t c = 0; c < nc; ++c) {
+        param_base[c] = n_slots;
+        n_slots += jobs[c]->n_params;
+    }
     std::vector<CircuitDesc> cd(nc);
     std::vector<PauliStep> steps;
     std::vector<Term> terms;
@@ -587,7 +605,7 @@ void run_group(const std::vector<const Job*>& jobs, uint64_t n_params_total, boo
         cd[c].op_off = steps.size();
         cd[c].term_off = terms.size();
         cd[c].n_terms = j.terms.size();
-        for (size_t k = 0; k < j.ops.size(); ++k) steps.push_back(make_step(j.ops[k], j.param_of_op[k]));
+        for (size_t k = 0; k < j.ops.size(); ++k) steps.push_back(make_step(j.ops[k], param_slot(j, k, param_base[c])));
         cd[c].group_off = 0;
         cd[c].n_groups = 0;
         terms.insert(terms.end(), j.terms.begin(), j.terms.end());
@@ -604,7 +622,7 @@ void run_group(const std::vector<const Job*>& jobs, uint64_t n_params_total, boo
     const PauliStep* d_steps = stage(steps.data(), std::max<size_t>(1, steps.size()));
     const Term* d_terms = stage(terms.data(), std::max<size_t>(1, terms.size()));
     double* epart = static_cast<double*>(arena(sizeof(double) * 2 * kMaxBlocks * nc));
-    double* gpart = static_cast<double*>(arena(sizeof(double) * 2 * kMaxBlocks * std::max<uint64_t>(1, n_params_total)));
+    double* gpart = static_cast<double*>(arena(sizeof(double) * 2 * kMaxBlocks * std::max<uint64_t>(1, n_slots)));
 
     // One fused schedule per sweep: each circuit's groups and, for jobs[0]
     // when `general`, its units in program order.
@@ -623,7 +641,7 @@ void run_group(const std::vector<const Job*>& jobs, uint64_t n_params_total, boo
         for (uint32_t c = 0; c < nc; ++c) {
             S.cd[c].group_off = static_cast<uint32_t>(S.groups.size());
             std::vector<Unit> u;
-            schedule_job(*jobs[c], D, S.groups, S.steps, u, S.cd[c].n_groups);
+            schedule_job(*jobs[c], param_base[c], D, S.groups, S.steps, u, S.cd[c].n_groups);
             if (c == 0) S.units.swap(u);
             S.max_groups = std::max(S.max_groups, S.cd[c].n_groups);
         }
@@ -689,25 +707,24 @@ void run_group(const std::vector<const Job*>& jobs, uint64_t n_params_total, boo
     for (uint32_t c = 0; c < nc; ++c) ecount[c] = cd[c].blocks;
     double* d_e = static_cast<double*>(arena(sizeof(double) * 2 * nc));
     k_rows<<<nc, kBlock, 0, stream()>>>(epart, stage(ecount.data(), nc), d_e);
-    std::vector<uint32_t> gcount(std::max<uint64_t>(1, n_params_total), 0);
+    std::vector<uint32_t> gcount(std::max<uint64_t>(1, n_slots), 0);
     for (uint32_t c = 0; c < nc; ++c)
         for (size_t k = 0; k < jobs[c]->ops.size(); ++k)
-            if (jobs[c]->param_of_op[k] >= 0) gcount[jobs[c]->param_of_op[k]] = cd[c].blocks;
+            if (jobs[c]->param_of_op[k] >= 0) gcount[param_base[c] + jobs[c]->param_of_op[k]] = cd[c].blocks;
     double* d_g = static_cast<double*>(arena(sizeof(double) * 2 * gcount.size()));
     k_rows<<<static_cast<unsigned>(gcount.size()), kBlock, 0, stream()>>>(gpart, stage(gcount.data(), gcount.size()), d_g);
     std::vector<double> e(2 * nc);
     LQCS_HIP(hipMemcpyAsync(e.data(), d_e, sizeof(double) * 2 * nc, hipMemcpyDeviceToHost, stream()));
-    gpart_host.assign(2 * gcount.size(), 0.0);
-    LQCS_HIP(hipMemcpyAsync(gpart_host.data(), d_g, sizeof(double) * gpart_host.size(), hipMemcpyDeviceToHost,
-                            stream()));
+    gsum.assign(2 * gcount.size(), 0.0);
+    LQCS_HIP(hipMemcpyAsync(gsum.data(), d_g, sizeof(double) * gsum.size(), hipMemcpyDeviceToHost, stream()));
     sync();
     energies_out.resize(nc);
     for (uint32_t c = 0; c < nc; ++c) energies_out[c] = e[2 * c];
 }
 
 Job build_job(uint32_t nq, const lqcs_program& prog, const uint64_t* param_op, uint64_t n_params,
-              uint64_t param_base, const uint64_t* xs, const uint64_t* zys, const uint32_t* nys,
-              const double* coeffs, uint64_t n_terms) {
+              const uint64_t* xs, const uint64_t* zys, const uint32_t* nys, const double* coeffs,
+              uint64_t n_terms) {
     if (nq == 0 || nq > LQCS_DEVICE_MAX_QUBITS) throw Error(LQCS_ERR_INVALID_ARGUMENT, "bad register width");
     Job j;
     j.nq = nq;
@@ -718,8 +735,9 @@ Job build_job(uint32_t nq, const lqcs_program& prog, const uint64_t* param_op, u
         if (j.ops[param_op[p]].kind != LQCS_OP_PAULI_ROTATION)
             throw Error(LQCS_ERR_INVALID_ARGUMENT, "parameter op must be a Pauli rotation");
         if (j.param_of_op[param_op[p]] >= 0) throw Error(LQCS_ERR_INVALID_ARGUMENT, "op bound to two parameters");
-        j.param_of_op[param_op[p]] = static_cast<int64_t>(param_base + p);
+        j.param_of_op[param_op[p]] = static_cast<int64_t>(p);
     }
+    j.n_params = n_params;
     const uint64_t full = (uint64_t{1} << nq) - 1;
     static const amp_t kIPow[4] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
     for (uint64_t t = 0; t < n_terms; ++t) {
@@ -732,9 +750,9 @@ Job build_job(uint32_t nq, const lqcs_program& prog, const uint64_t* param_op, u
     return j;
 }
 
-// grad_p = Im(i^ny_p * S_p) from the accumulated S_p.
-double grad_from(const Job& j, uint64_t global_p, uint64_t op_index, const std::vector<double>& g) {
-    const double sr = g[2 * global_p], si = g[2 * global_p + 1];
+// grad_p = Im(i^ny_p * S_p) from the accumulated S_p at its slot.
+double grad_from(const Job& j, uint64_t slot, uint64_t op_index, const std::vector<double>& g) {
+    const double sr = g[2 * slot], si = g[2 * slot + 1];
     switch (j.ops[op_index].n_y % 4) {
         case 0: return si;
         case 1: return sr;   // Im(i (sr + i si)) = sr
@@ -758,9 +776,10 @@ extern "C" lqcs_status lqcs_adjoint_gradient(uint32_t nq, const lqcs_program* pr
             throw Error(LQCS_ERR_INVALID_ARGUMENT, "null argument");
         ensure_ready();
         arena_reset();
-        const Job j = build_job(nq, *prog, param_op, n_params, 0, x_masks, zy_masks, n_ys, coeffs, n_terms);
+        const Job j = build_job(nq, *prog, param_op, n_params, x_masks, zy_masks, n_ys, coeffs, n_terms);
         std::vector<double> e, g;
-        run_group({&j}, n_params, !j.all_pauli, e, g);
+        std::vector<uint64_t> pb;
+        run_group({&j}, !j.all_pauli, e, g, pb);
         if (energy) *energy = e[0];
         for (uint64_t p = 0; p < n_params; ++p) grad[p] = grad_from(j, p, param_op[p], g);
     });
@@ -775,6 +794,13 @@ extern "C" lqcs_status lqcs_adjoint_gradient_batched(uint32_t n_circuits, const 
     return run_guarded([&] {
         if (n_circuits == 0) return;
         if (!n_qubits || !progs || !param_off || !term_off) throw Error(LQCS_ERR_INVALID_ARGUMENT, "null argument");
+        for (uint32_t c = 0; c < n_circuits; ++c)
+            if (param_off[c + 1] < param_off[c] || term_off[c + 1] < term_off[c])
+                throw Error(LQCS_ERR_INVALID_ARGUMENT, "offsets must be non-decreasing");
+        if (param_off[n_circuits] > param_off[0] && (!param_op || !grads))
+            throw Error(LQCS_ERR_INVALID_ARGUMENT, "null argument");
+        if (term_off[n_circuits] > term_off[0] && (!x_masks || !zy_masks || !n_ys || !coeffs))
+            throw Error(LQCS_ERR_INVALID_ARGUMENT, "null argument");
         ensure_ready();
         arena_reset();
         std::vector<Job> jobs;
@@ -782,11 +808,9 @@ extern "C" lqcs_status lqcs_adjoint_gradient_batched(uint32_t n_circuits, const 
         for (uint32_t c = 0; c < n_circuits; ++c) {
             const uint64_t p0 = param_off[c], p1 = param_off[c + 1];
             const uint64_t t0 = term_off[c], t1 = term_off[c + 1];
-            if (p1 < p0 || t1 < t0) throw Error(LQCS_ERR_INVALID_ARGUMENT, "offsets must be non-decreasing");
-            jobs.push_back(build_job(n_qubits[c], progs[c], param_op + p0, p1 - p0, p0, x_masks + t0, zy_masks + t0,
+            jobs.push_back(build_job(n_qubits[c], progs[c], param_op + p0, p1 - p0, x_masks + t0, zy_masks + t0,
                                      n_ys + t0, coeffs + t0, t1 - t0));
         }
-        const uint64_t n_params = param_off[n_circuits];
         // Pauli-only circuits share launches in groups bounded by HBM use.
         size_t free_b = 0, total_b = 0;
         LQCS_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -797,12 +821,13 @@ extern "C" lqcs_status lqcs_adjoint_gradient_batched(uint32_t n_circuits, const 
         auto flush = [&] {
             if (group.empty()) return;
             std::vector<double> e, g;
-            run_group(group, n_params, false, e, g);
+            std::vector<uint64_t> pb;
+            run_group(group, false, e, g, pb);
             for (size_t i = 0; i < group.size(); ++i) {
                 const uint32_t c = group_ids[i];
                 if (energies) energies[c] = e[i];
                 for (uint64_t p = param_off[c]; p < param_off[c + 1]; ++p)
-                    grads[p] = grad_from(jobs[c], p, param_op[p], g);
+                    grads[p] = grad_from(jobs[c], pb[i] + (p - param_off[c]), param_op[p], g);
             }
             group.clear();
             group_ids.clear();
@@ -814,9 +839,11 @@ extern "C" lqcs_status lqcs_adjoint_gradient_batched(uint32_t n_circuits, const 
             if (!j.all_pauli) {
                 flush();
                 std::vector<double> e, g;
-                run_group({&j}, n_params, true, e, g);
+                std::vector<uint64_t> pb;
+                run_group({&j}, true, e, g, pb);
                 if (energies) energies[c] = e[0];
-                for (uint64_t p = param_off[c]; p < param_off[c + 1]; ++p) grads[p] = grad_from(j, p, param_op[p], g);
+                for (uint64_t p = param_off[c]; p < param_off[c + 1]; ++p)
+                    grads[p] = grad_from(j, p - param_off[c], param_op[p], g);
                 arena_reset();
                 continue;
             }

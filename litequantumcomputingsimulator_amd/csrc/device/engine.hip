@@ -872,6 +872,8 @@ struct lqcs_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void* buf = nullptr;  // the tile programs the captured kernels read
+    const lqcs_dev::amp_t* d = nullptr;  // the register buffer the kernels address
+    uint32_t nq = 0;
 };
 
 lqcs_status lqcs_graph_create(lqcs_dev_state* s, const lqcs_program* prog, const lqcs_exec_options* opts,
@@ -899,6 +901,8 @@ lqcs_status lqcs_graph_create(lqcs_dev_state* s, const lqcs_program* prog, const
         const size_t b_w = pad(sp.rwords.size() * sizeof(amp_t));
         auto* g = new lqcs_graph;
         try {
+            g->d = s->d;
+            g->nq = s->nq;
             g->buf = device_alloc(std::max<size_t>(b_ops + b_ph + b_w, 256));
             char* base = static_cast<char*>(g->buf);
             auto* d_ops = reinterpret_cast<TileOp*>(base);
@@ -931,9 +935,12 @@ lqcs_status lqcs_graph_create(lqcs_dev_state* s, const lqcs_program* prog, const
     });
 }
 
-lqcs_status lqcs_graph_launch(lqcs_graph* g) {
+lqcs_status lqcs_graph_launch(lqcs_graph* g, lqcs_dev_state* s) {
     return guarded([&] {
         need(g, "graph");
+        need(s, "state");
+        if (s->d != g->d || s->nq != g->nq)
+            throw Error(LQCS_ERR_LOGIC, "graph launch: the register does not hold the buffer the graph was captured on");
         ensure_ready();
         LQCS_HIP(hipGraphLaunch(g->exec, stream()));
     });

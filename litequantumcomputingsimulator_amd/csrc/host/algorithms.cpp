@@ -1,5 +1,6 @@
-// Expectation values and gradients (reference src/algorithms/vqe.cpp:11-17,
-// 68-94) and the QFT builder (src/algorithms/qft.cpp:7-27), on the engine.
+// Expectation values, gradients and the two variational solvers (reference
+// src/algorithms/vqe.cpp:11-133) and the QFT builder (src/algorithms/qft.cpp:7-27),
+// on the engine.
 #include <cmath>
 #include <numbers>
 #include <stdexcept>
@@ -138,6 +139,78 @@ std::vector<GradientResult> adjoint_gradient_batched(std::span<const GradientJob
         out[c].grad.assign(grads.begin() + param_off[c], grads.begin() + param_off[c + 1]);
     }
     return out;
+}
+
+// Rotosolve (reference src/algorithms/vqe.cpp:19-63). Along one coordinate
+// E(t) = a + b cos(t - phi); with e0 = E(s), e+ = E(s + pi/2), e- = E(s - pi/2)
+// the minimum sits at s - pi/2 - atan2(2 e0 - e+ - e-, e+ - e-), folded into
+// (-pi, pi]. Each coordinate costs three state evolutions on the device.
+VQEResult vqe(const Hamiltonian& hamiltonian, const Ansatz& ansatz, std::size_t n_params,
+              const VQEOptions& options) {
+    if (n_params == 0) throw std::invalid_argument("vqe: n_params must be >= 1");
+    std::vector<double> theta = options.initial_parameters;
+    if (theta.empty())
+        theta.assign(n_params, 0.0);
+    else if (theta.size() != n_params)
+        throw std::invalid_argument("vqe: initial_parameters size must equal n_params");
+    StatevectorSimulator sim;
+    auto energy = [&](const std::vector<double>& th) {
+        return expectation(sim.run_statevector(ansatz(std::span<const double>(th))), hamiltonian);
+    };
+    constexpr double kQuarter = std::numbers::pi / 2;
+    VQEResult res{};
+    double e = energy(theta);
+    for (std::size_t sweep = 0; sweep < options.max_iterations; ++sweep) {
+        const double start = e;
+        for (double& t : theta) {
+            const double s0 = t;
+            t = s0 + kQuarter;
+            const double ep = energy(theta);
+            t = s0 - kQuarter;
+            const double em = energy(theta);
+            t = std::remainder(s0 - kQuarter - std::atan2(2.0 * e - ep - em, ep - em), 2.0 * std::numbers::pi);
+            e = energy(theta);
+        }
+        res.history.push_back(e);
+        res.iterations = sweep + 1;
+        if (start - e < options.tol) break;
+    }
+    res.energy = e;
+    res.parameters = std::move(theta);
+    return res;
+}
+
+// Gradient descent (reference src/algorithms/vqe.cpp:96-133): theta -= lr * g,
+// the energy after each step recorded, stopping once |g| < tol. The gradient
+// is the adjoint one (one forward and one backward sweep instead of 2P
+// evolutions); the adjoint call at the new point also returns the energy
+// there, which is the step's history entry and the next step's gradient.
+VQEResult vqe_gradient_descent(const Hamiltonian& hamiltonian, const ParametricCircuit& ansatz,
+                               const GradientVQEOptions& options) {
+    const std::size_t n = ansatz.num_parameters();
+    if (n == 0) throw std::invalid_argument("vqe_gradient_descent: ansatz has no parameters");
+    std::vector<double> theta = options.initial_parameters;
+    if (theta.empty())
+        theta.assign(n, 0.0);
+    else if (theta.size() != n)
+        throw std::invalid_argument("vqe_gradient_descent: initial_parameters size must equal num_parameters");
+    VQEResult res{};
+    double e = 0.0;
+    std::vector<double> g = adjoint_gradient(hamiltonian, ansatz, theta, &e);
+    for (std::size_t it = 0; it < options.max_iterations; ++it) {
+        double norm2 = 0.0;
+        for (std::size_t k = 0; k < n; ++k) {
+            theta[k] -= options.learning_rate * g[k];
+            norm2 += g[k] * g[k];
+        }
+        g = adjoint_gradient(hamiltonian, ansatz, theta, &e);
+        res.history.push_back(e);
+        res.iterations = it + 1;
+        if (std::sqrt(norm2) < options.tol) break;
+    }
+    res.energy = e;
+    res.parameters = std::move(theta);
+    return res;
 }
 
 QuantumCircuit qft(std::size_t n, bool do_swaps) {

@@ -360,7 +360,7 @@ CircuitGraph::CircuitGraph(const CompiledCircuit& circuit, Statevector& sv, cons
 CircuitGraph::~CircuitGraph() { (void)lqcs_graph_destroy(graph_); }
 
 void CircuitGraph::launch() {
-    check(lqcs_graph_launch(graph_));
+    check(lqcs_graph_launch(graph_, sv_->device()));
     sv_->mark_device_newer();
 }
 

@@ -3,6 +3,7 @@
 // statevector path, plus engine extensions (adjoint gradients, sample
 // indices, plan statistics). Long-running calls release the GIL.
 #include <pybind11/complex.h>
+#include <pybind11/functional.h>
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -429,6 +430,36 @@ PYBIND11_MODULE(_pylqcs, m) {
             return out;
         },
         "hamiltonians"_a, "ansatze"_a, "params"_a, "list of (energy, gradient)");
+    py::class_<algorithms::VQEResult>(alg, "VQEResult")
+        .def_readonly("energy", &algorithms::VQEResult::energy)
+        .def_readonly("parameters", &algorithms::VQEResult::parameters)
+        .def_readonly("iterations", &algorithms::VQEResult::iterations)
+        .def_readonly("history", &algorithms::VQEResult::history);
+    // The ansatz is a Python callable, so the GIL stays held while vqe runs.
+    alg.def(
+        "vqe",
+        [](const std::vector<std::pair<double, std::string>>& terms,
+           const std::function<QuantumCircuit(std::vector<double>)>& ansatz, std::size_t n_params,
+           std::size_t max_iterations, double tol, std::vector<double> initial_parameters) {
+            const algorithms::Ansatz call = [&ansatz](std::span<const double> th) {
+                return ansatz(std::vector<double>(th.begin(), th.end()));
+            };
+            return algorithms::vqe(to_h(terms), call, n_params,
+                                   {max_iterations, tol, std::move(initial_parameters)});
+        },
+        "hamiltonian"_a, "ansatz"_a, "n_params"_a, "max_iterations"_a = 100, "tol"_a = 1e-9,
+        "initial_parameters"_a = std::vector<double>{});
+    alg.def(
+        "vqe_gradient_descent",
+        [](const std::vector<std::pair<double, std::string>>& terms, const ParametricCircuit& ansatz,
+           std::size_t max_iterations, double learning_rate, double tol, std::vector<double> initial_parameters) {
+            const auto h = to_h(terms);
+            py::gil_scoped_release nogil;
+            return algorithms::vqe_gradient_descent(
+                h, ansatz, {max_iterations, learning_rate, tol, std::move(initial_parameters)});
+        },
+        "hamiltonian"_a, "ansatz"_a, "max_iterations"_a = 200, "learning_rate"_a = 0.1, "tol"_a = 1e-9,
+        "initial_parameters"_a = std::vector<double>{});
 
     py::class_<CompiledCircuit>(m, "CompiledCircuit")
         .def(py::init<const QuantumCircuit&, bool>(), "circuit"_a, "fuse_gates"_a = true)

@@ -357,6 +357,54 @@ def bound_energy(template: Circuit, param_inst: Sequence[int], theta, h) -> floa
     return e.value
 
 
+def vqe_rotosolve(template: Circuit, param_inst: Sequence[int], h, max_iterations: int = 100,
+                  tol: float = 1e-9, init=None):
+    """algorithms::vqe (src/algorithms/vqe.cpp:19-63) over the restated
+    bound_energy: per coordinate, energies at s and s +- pi/2 fix the sinusoid
+    E(t) = a + b cos(t - phi); the coordinate moves to its minimum folded into
+    (-pi, pi] (std::remainder). Returns (energy, params, iterations, history)."""
+    n = len(param_inst)
+    theta = np.zeros(n) if init is None or len(init) == 0 else np.array(init, dtype=np.float64)
+    if len(theta) != n:
+        raise ValueError("initial_parameters size must equal n_params")
+    e = bound_energy(template, param_inst, theta, h)
+    history, it = [], 0
+    for sweep in range(max_iterations):
+        start = e
+        for k in range(n):
+            s0 = theta[k]
+            theta[k] = s0 + math.pi / 2
+            ep = bound_energy(template, param_inst, theta, h)
+            theta[k] = s0 - math.pi / 2
+            em = bound_energy(template, param_inst, theta, h)
+            theta[k] = math.remainder(s0 - math.pi / 2 - math.atan2(2.0 * e - ep - em, ep - em), 2.0 * math.pi)
+            e = bound_energy(template, param_inst, theta, h)
+        history.append(e)
+        it = sweep + 1
+        if start - e < tol:
+            break
+    return e, theta, it, history
+
+
+def vqe_gradient_descent(template: Circuit, param_inst: Sequence[int], h, max_iterations: int = 200,
+                         learning_rate: float = 0.1, tol: float = 1e-9, init=None):
+    """algorithms::vqe_gradient_descent (src/algorithms/vqe.cpp:96-133) over
+    the restated parameter-shift gradient: theta -= lr * g, record E(theta),
+    stop once |g| < tol. Returns (energy, params, iterations, history)."""
+    n = len(param_inst)
+    theta = np.zeros(n) if init is None or len(init) == 0 else np.array(init, dtype=np.float64)
+    history, it = [], 0
+    for i in range(max_iterations):
+        g = parameter_shift_gradient(template, param_inst, theta, h)
+        theta = theta - learning_rate * g
+        history.append(bound_energy(template, param_inst, theta, h))
+        it = i + 1
+        if math.sqrt(float(np.dot(g, g))) < tol:
+            break
+    e = history[-1] if history else bound_energy(template, param_inst, theta, h)
+    return e, theta, it, history
+
+
 def density_probabilities(circ: Circuit, channels=()) -> np.ndarray:
     lib = oracle_lib()
     c, keep = circ.to_orc()
@@ -454,6 +502,63 @@ def ref_density_probabilities(circ: Circuit, channels=()) -> np.ndarray:
     _ref_check(ref_lib().ref_density_probabilities(C.byref(c), C.byref(nz) if channels else None,
                                                    out.ctypes.data_as(C.POINTER(C.c_double))))
     return out
+
+
+def ref_to_qasm(circ: Circuit) -> str:
+    c, keep = circ.to_orc()
+    buf = C.create_string_buffer(1 << 22)
+    _ref_check(ref_lib().ref_to_qasm(C.byref(c), buf, C.c_uint64(len(buf))))
+    return buf.value.decode()
+
+
+def ref_qasm_reparse(text: str) -> tuple[str, int]:
+    """(to_qasm(from_qasm(text)), instruction count) in the reference."""
+    buf = C.create_string_buffer(1 << 22)
+    n = C.c_uint64(0)
+    _ref_check(ref_lib().ref_qasm_reparse(text.encode(), buf, C.c_uint64(len(buf)), C.byref(n)))
+    return buf.value.decode(), n.value
+
+
+def ref_eliminate_qasm(circ: Circuit) -> tuple[str, int]:
+    """RedundantGateElimination::run in the reference, as (QASM, instruction count)."""
+    c, keep = circ.to_orc()
+    buf = C.create_string_buffer(1 << 22)
+    n = C.c_uint64(0)
+    _ref_check(ref_lib().ref_eliminate_qasm(C.byref(c), buf, C.c_uint64(len(buf)), C.byref(n)))
+    return buf.value.decode(), n.value
+
+
+def ref_qft_qasm(n: int, do_swaps: bool = True, inverse: bool = False) -> str:
+    buf = C.create_string_buffer(1 << 22)
+    _ref_check(ref_lib().ref_qft_qasm(C.c_uint32(n), int(do_swaps), int(inverse), buf, C.c_uint64(len(buf))))
+    return buf.value.decode()
+
+
+def _ref_vqe(fn, template: Circuit, param_inst, h, max_iterations, extra, init):
+    c, keep = template.to_orc()
+    pi = np.asarray(param_inst, dtype=np.uint64)
+    arr, keep2 = _terms(h)
+    n = len(pi)
+    ini = None if init is None or len(init) == 0 else np.asarray(init, dtype=np.float64)
+    e = C.c_double(0)
+    params = np.zeros(n)
+    it = C.c_uint64(0)
+    hist = np.zeros(max(1, max_iterations))
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    _ref_check(fn(C.byref(c), pi.ctypes.data_as(C.POINTER(C.c_uint64)), C.c_uint64(n), arr, C.c_uint64(len(h)),
+                  C.c_uint64(max_iterations), *[C.c_double(x) for x in extra], dp(ini) if ini is not None else None,
+                  C.byref(e), dp(params), C.byref(it), dp(hist)))
+    return e.value, params, it.value, list(hist[: it.value])
+
+
+def ref_vqe_rotosolve(template: Circuit, param_inst, h, max_iterations=100, tol=1e-9, init=None):
+    return _ref_vqe(ref_lib().ref_vqe_rotosolve, template, param_inst, h, max_iterations, (tol,), init)
+
+
+def ref_vqe_gradient_descent(template: Circuit, param_inst, h, max_iterations=200, learning_rate=0.1, tol=1e-9,
+                             init=None):
+    return _ref_vqe(ref_lib().ref_vqe_gradient_descent, template, param_inst, h, max_iterations,
+                    (learning_rate, tol), init)
 
 
 def ref_uniform_stream(seed: int, n: int) -> np.ndarray:

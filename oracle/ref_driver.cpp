@@ -239,6 +239,87 @@ int ref_base_matrix(const orc_inst* g, orc_c* out, uint64_t* dim) {
     });
 }
 
+namespace {
+void put_text(const std::string& s, char* buf, uint64_t buf_len) {
+    if (s.size() + 1 > buf_len) throw std::length_error("ref_driver: text buffer too small");
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+}
+}  // namespace
+
+// io::to_qasm of a circuit.
+int ref_to_qasm(const orc_circuit* c, char* buf, uint64_t buf_len) {
+    return guarded([&] { put_text(io::to_qasm(to_circuit(c)), buf, buf_len); });
+}
+
+// io::from_qasm of `text`, written back out by io::to_qasm (the exporter is
+// pinned on its own, so the text is a faithful record of what was parsed).
+int ref_qasm_reparse(const char* text, char* buf, uint64_t buf_len, uint64_t* n_inst) {
+    return guarded([&] {
+        const QuantumCircuit qc = io::from_qasm(text);
+        *n_inst = qc.num_instructions();
+        put_text(io::to_qasm(qc), buf, buf_len);
+    });
+}
+
+// RedundantGateElimination::run, as QASM text.
+int ref_eliminate_qasm(const orc_circuit* c, char* buf, uint64_t buf_len, uint64_t* n_inst) {
+    return guarded([&] {
+        const QuantumCircuit out = RedundantGateElimination().run(to_circuit(c));
+        *n_inst = out.num_instructions();
+        put_text(io::to_qasm(out), buf, buf_len);
+    });
+}
+
+// algorithms::qft / qft_inverse, as QASM text.
+int ref_qft_qasm(uint32_t n, int do_swaps, int inverse, char* buf, uint64_t buf_len) {
+    return guarded([&] {
+        const QuantumCircuit qc = inverse ? algorithms::qft_inverse(n, do_swaps != 0)
+                                          : algorithms::qft(n, do_swaps != 0);
+        put_text(io::to_qasm(qc), buf, buf_len);
+    });
+}
+
+// algorithms::vqe (Rotosolve) with the ansatz theta -> template.bind(theta).
+// history must hold max_iterations doubles.
+int ref_vqe_rotosolve(const orc_circuit* tmpl, const uint64_t* param_inst, uint64_t np,
+                      const orc_term* h, uint64_t nt, uint64_t max_iterations, double tol,
+                      const double* init, double* energy, double* params, uint64_t* iterations,
+                      double* history) {
+    return guarded([&] {
+        const ParametricCircuit pc = to_parametric(tmpl, param_inst, np);
+        algorithms::VQEOptions o;
+        o.max_iterations = max_iterations;
+        o.tol = tol;
+        if (init) o.initial_parameters.assign(init, init + np);
+        const algorithms::Ansatz ansatz = [&pc](std::span<const double> th) { return pc.bind(th); };
+        const auto r = algorithms::vqe(to_hamiltonian(h, nt), ansatz, np, o);
+        *energy = r.energy;
+        std::memcpy(params, r.parameters.data(), sizeof(double) * np);
+        *iterations = r.iterations;
+        std::memcpy(history, r.history.data(), sizeof(double) * r.history.size());
+    });
+}
+
+// algorithms::vqe_gradient_descent. history must hold max_iterations doubles.
+int ref_vqe_gradient_descent(const orc_circuit* tmpl, const uint64_t* param_inst, uint64_t np,
+                             const orc_term* h, uint64_t nt, uint64_t max_iterations,
+                             double learning_rate, double tol, const double* init, double* energy,
+                             double* params, uint64_t* iterations, double* history) {
+    return guarded([&] {
+        const ParametricCircuit pc = to_parametric(tmpl, param_inst, np);
+        algorithms::GradientVQEOptions o;
+        o.max_iterations = max_iterations;
+        o.learning_rate = learning_rate;
+        o.tol = tol;
+        if (init) o.initial_parameters.assign(init, init + np);
+        const auto r = algorithms::vqe_gradient_descent(to_hamiltonian(h, nt), pc, o);
+        *energy = r.energy;
+        std::memcpy(params, r.parameters.data(), sizeof(double) * np);
+        *iterations = r.iterations;
+        std::memcpy(history, r.history.data(), sizeof(double) * r.history.size());
+    });
+}
+
 // Draws of lqcs::Rng (mt19937_64 + uniform_real_distribution) for a seed.
 void ref_uniform_stream(uint64_t seed, uint64_t n, double* out) {
     Rng r(seed);

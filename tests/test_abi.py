@@ -115,3 +115,88 @@ def test_host_gate_matrices_equal_oracle(lq):
         assert np.array_equal(got, want), inst
         seen += 1
     assert seen >= 15
+
+
+class _Program(ctypes.Structure):
+    """lqcs_program (include/lqcs_device.h)."""
+    _fields_ = [("ops", ctypes.c_void_p), ("n_ops", ctypes.c_uint64), ("qubits", ctypes.c_void_p),
+                ("n_qubits", ctypes.c_uint64), ("cplx", ctypes.c_void_p), ("n_cplx", ctypes.c_uint64),
+                ("perm", ctypes.c_void_p), ("n_perm", ctypes.c_uint64)]
+
+
+def test_batched_adjoint_rejects_null_arrays_before_any_device_call():
+    """lqcs_adjoint_gradient_batched checks its arrays (as the single-circuit
+    entry does) and returns INVALID_ARGUMENT instead of dereferencing a null
+    pointer; these checks come before the device is touched."""
+    lib = ctypes.CDLL(os.path.join(LIB_DIR, "liblqcs_device.so"))
+    lib.lqcs_dev_last_error.restype = ctypes.c_char_p
+    u64 = ctypes.c_uint64
+    nq = (ctypes.c_uint32 * 1)(4)
+    progs = (_Program * 1)()
+    pop = (u64 * 2)(0, 0)
+    xs, zs = (u64 * 1)(1), (u64 * 1)(0)
+    ys = (ctypes.c_uint32 * 1)(0)
+    cs = (ctypes.c_double * 1)(1.0)
+    grads, energies = (ctypes.c_double * 2)(), (ctypes.c_double * 1)()
+
+    def call(param_op, param_off, x, z, y, c, term_off, g):
+        return lib.lqcs_adjoint_gradient_batched(ctypes.c_uint32(1), nq, progs, param_op, param_off, x, z, y, c,
+                                                 term_off, energies, g)
+
+    p1, t1, t0 = (u64 * 2)(0, 2), (u64 * 2)(0, 1), (u64 * 2)(0, 0)
+    assert call(None, p1, xs, zs, ys, cs, t1, grads) == 1  # parameters without param_op
+    assert call(pop, p1, xs, zs, ys, cs, t1, None) == 1    # parameters without grads
+    assert call(pop, p1, None, zs, ys, cs, t1, grads) == 1  # terms without x masks
+    assert call(pop, p1, xs, zs, None, cs, t1, grads) == 1  # terms without n_y
+    assert call(pop, p1, xs, zs, ys, None, t1, grads) == 1  # terms without coefficients
+    assert b"null" in lib.lqcs_dev_last_error()
+    assert call(pop, (u64 * 2)(2, 1), xs, zs, ys, cs, t0, grads) == 1  # decreasing offsets
+    assert b"non-decreasing" in lib.lqcs_dev_last_error()
+
+
+class _Op(ctypes.Structure):
+    """lqcs_op (include/lqcs_device.h)."""
+    _fields_ = [("kind", ctypes.c_uint32), ("n_controls", ctypes.c_uint32), ("n_targets", ctypes.c_uint32),
+                ("qubit_offset", ctypes.c_uint32), ("data_offset", ctypes.c_uint64), ("x_mask", ctypes.c_uint64),
+                ("zy_mask", ctypes.c_uint64), ("n_y", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("theta", ctypes.c_double)]
+
+
+@pytest.mark.gpu
+def test_graph_launch_refuses_a_register_it_was_not_captured_on():
+    """A captured graph addresses the buffer of the register it was recorded
+    on: launching it on another register (same or other width) returns LOGIC
+    and changes nothing; on its own register it applies H again."""
+    lib = ctypes.CDLL(os.path.join(LIB_DIR, "liblqcs_device.so"))
+    lib.lqcs_dev_last_error.restype = ctypes.c_char_p
+    assert lib.lqcs_dev_init(0) == 0
+    a, b, c = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    for st, n in ((a, 10), (b, 10), (c, 11)):
+        assert lib.lqcs_state_create(ctypes.c_uint32(n), ctypes.byref(st)) == 0
+    h = 1.0 / np.sqrt(2.0)
+    ops = (_Op * 1)(_Op(0, 0, 1, 0, 0, 0, 0, 0, 0, 0.0))  # LQCS_OP_SINGLE on qubit 3
+    qubits = (ctypes.c_uint32 * 1)(3)
+    cplx = (ctypes.c_double * 8)(h, 0, h, 0, h, 0, -h, 0)
+    prog = _Program(ctypes.cast(ops, ctypes.c_void_p), 1, ctypes.cast(qubits, ctypes.c_void_p), 1,
+                    ctypes.cast(cplx, ctypes.c_void_p), 4, None, 0)
+    g = ctypes.c_void_p()
+    try:
+        assert lib.lqcs_graph_create(a, ctypes.byref(prog), None, ctypes.byref(g)) == 0
+        assert lib.lqcs_graph_launch(g, b) == 3
+        assert b"captured" in lib.lqcs_dev_last_error()
+        assert lib.lqcs_graph_launch(g, c) == 3
+        assert lib.lqcs_graph_launch(g, a) == 0
+        assert lib.lqcs_dev_synchronize() == 0
+        out = np.zeros(2 << 10)
+        assert lib.lqcs_state_download(a, out.ctypes.data_as(ctypes.c_void_p)) == 0
+        amps = out[0::2] + 1j * out[1::2]
+        want = np.zeros(1 << 10, complex)
+        want[0] = want[8] = h
+        assert np.max(np.abs(amps - want)) <= 1e-15
+        assert lib.lqcs_state_download(b, out.ctypes.data_as(ctypes.c_void_p)) == 0
+        assert out[0] == 1.0 and np.count_nonzero(out) == 1  # untouched |0>
+    finally:
+        if g:
+            lib.lqcs_graph_destroy(g)
+        for st in (a, b, c):
+            lib.lqcs_state_destroy(st)
