@@ -248,6 +248,19 @@ lqcs_status lqcs_state_expectation_pauli(const lqcs_dev_state* s, uint64_t x_mas
 lqcs_status lqcs_state_expectation_terms(const lqcs_dev_state* s, const uint64_t* x_masks,
                                          const uint64_t* zy_masks, const uint32_t* n_ys,
                                          const double* coeffs, uint64_t n_terms, double* out);
+/* No counterpart in the reference: lqcs_state_expectation_terms for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Same arguments and result;
+ * it calls no hipMalloc/hipFree: the staged terms and partial sums live in one
+ * static device array of about 8 MiB per GPU, held until the process ends.
+ * At most LQCS_NOALLOC_MAX_TERMS terms (more: INVALID_ARGUMENT). Every call
+ * shares that array; calls hold the engine lock and return after their work
+ * completes, so calls through this library never overlap on one GPU (work
+ * that did overlap would race on the array). */
+#define LQCS_NOALLOC_MAX_TERMS 512
+lqcs_status lqcs_state_expectation_terms_noalloc(const lqcs_dev_state* s, const uint64_t* x_masks,
+                                                 const uint64_t* zy_masks, const uint32_t* n_ys,
+                                                 const double* coeffs, uint64_t n_terms, double* out);
 /* run_sampling (statevector_simulator.cpp:88-119): for each uniform draw u,
  * the index upper_bound(cumulative, u * total). */
 lqcs_status lqcs_state_sample(const lqcs_dev_state* s, const double* draws, uint64_t n_draws,

@@ -19,11 +19,6 @@
 
 #include "../../../include/lqcs_device.h"
 
-struct lqcs_dev_state {
-    lqcs_dev::amp_t* d = nullptr;
-    uint32_t nq = 0;
-};
-
 namespace lqcs_dev {
 
 // ---- context ------------------------------------------------------------------

@@ -9,6 +9,12 @@
 
 #include "../../../include/lqcs_device.h"
 
+// A device register: 2^nq complex128 amplitudes owned by the handle.
+struct lqcs_dev_state {
+    lqcs_dev::amp_t* d = nullptr;
+    uint32_t nq = 0;
+};
+
 namespace lqcs_dev {
 
 void init_device(int device);

@@ -372,8 +372,15 @@ void reduce_max_abs_diff(const amp_t* a, const amp_t* b, uint32_t nq, double* d_
 
 void reduce_pauli_terms(const amp_t* s, uint32_t nq, const uint64_t* d_x, const uint64_t* d_zy,
                         uint32_t n_terms, double* d_out) {
+    double* part = static_cast<double*>(arena(sizeof(double) * pauli_term_partials(nq) * n_terms));
+    reduce_pauli_terms_with(s, nq, d_x, d_zy, n_terms, part, d_out);
+}
+
+uint64_t pauli_term_partials(uint32_t nq) { return 2 * uint64_t{red_blocks(nq, 1)}; }
+
+void reduce_pauli_terms_with(const amp_t* s, uint32_t nq, const uint64_t* d_x, const uint64_t* d_zy,
+                             uint32_t n_terms, double* part, double* d_out) {
     const unsigned g = red_blocks(nq, 1);
-    double* part = static_cast<double*>(arena(sizeof(double) * 2 * g * n_terms));
     k_pauli_expect_partial<<<dim3(g, n_terms), kBlock, 0, stream()>>>(s, nq, d_x, d_zy, part);
     k_finish<2><<<n_terms, kBlock, 0, stream()>>>(part, g, d_out);
 }

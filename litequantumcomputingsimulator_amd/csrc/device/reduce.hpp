@@ -12,6 +12,11 @@ void reduce_max_abs_diff(const amp_t* a, const amp_t* b, uint32_t nq, double* d_
 // d_out[2*t .. 2*t+1] = (re, im) of sum_i sign_t(i) a_i conj(a_{i ^ x_t}).
 void reduce_pauli_terms(const amp_t* s, uint32_t nq, const uint64_t* d_x, const uint64_t* d_zy,
                         uint32_t n_terms, double* d_out);
+// The same with the partial sums in a caller-given buffer of
+// pauli_term_partials(nq) * n_terms doubles (no arena use).
+uint64_t pauli_term_partials(uint32_t nq);
+void reduce_pauli_terms_with(const amp_t* s, uint32_t nq, const uint64_t* d_x, const uint64_t* d_zy,
+                             uint32_t n_terms, double* part, double* d_out);
 void reduce_p1(const amp_t* s, uint32_t nq, uint32_t q, uint32_t batch, double* d_out);
 // d_out[4*r ..] = S00, S11, Re S01, Im S01 of qubit q in register r.
 void reduce_qstats(const amp_t* s, uint32_t nq, uint32_t q, uint32_t batch, double* d_out);
