@@ -573,25 +573,27 @@ __device__ __forceinline__ void run_phases(Regs& v, Regs& w, amp_t* lds, const R
     uint32_t jt_prev = 0;
     for (uint32_t p = 0; p < n_phases; ++p) {
         const RPhase& ph = phases[p];
-        const uint32_t jt = pdep32(threadIdx.x, ph.tmask);
+        const uint32_t jt = rtile_thread_bits(threadIdx.x, ph);
         if (p > 0 && ph.transpose) {
             // Half a tile per round: the amplitudes whose bit h is `half`.
             // Bit h is a thread bit of both layouts, so a thread writes all
             // or none of its registers in a round, and reads likewise; reads
             // land in w so that nothing still to be written is overwritten.
+            // Addresses are byte offsets: the thread's part XOR the register's.
             const uint32_t h = ph.h;
             const uint32_t hp = (jt_prev >> h) & 1u, hc = (jt >> h) & 1u;
-            const uint32_t sp = rtile_slot(jt_prev, h), sc = rtile_slot(jt, h);
+            const uint32_t sp = rtile_slot(jt_prev, h) << 4, sc = rtile_slot(jt, h) << 4;
+            char* lb = reinterpret_cast<char*>(lds);
             for (uint32_t half = 0; half < 2; ++half) {
                 if (lds_used) __syncthreads();  // every thread has read the previous round
                 if (hp == half) {
 #pragma unroll
-                    for (int r = 0; r < kRR; ++r) lds[sp ^ ph.cs_prev[r]] = v[r];
+                    for (int r = 0; r < kRR; ++r) *reinterpret_cast<amp_t*>(lb + (sp ^ ph.cs_prev[r])) = v[r];
                 }
                 __syncthreads();
                 if (hc == half) {
 #pragma unroll
-                    for (int r = 0; r < kRR; ++r) w[r] = lds[sc ^ ph.cs_cur[r]];
+                    for (int r = 0; r < kRR; ++r) w[r] = *reinterpret_cast<const amp_t*>(lb + (sc ^ ph.cs_cur[r]));
                 }
                 lds_used = true;
             }
@@ -656,21 +658,23 @@ __global__ void __launch_bounds__(kRT) k_rtile(amp_t* __restrict__ s, uint32_t n
     __shared__ amp_t lds[1 << (kRL - 1)];
     amp_t* b = s + (uint64_t(blockIdx.y) << nq);
     const uint32_t tile = blockIdx.x;
-    const uint64_t base = pdep64(tile, ((uint64_t{1} << nq) - 1) & ~smask);
+    const uint64_t base = rtile_tile_base(tile, phases[0], ((uint64_t{1} << nq) - 1) & ~smask);
     Regs v, w;
     {
+        // base, the thread's bits and the register's bits are disjoint: the
+        // register part is added to a per-thread pointer.
         const RPhase& p0 = phases[0];
-        const uint64_t gt = base | pdep64(threadIdx.x, p0.tmask_global);
+        const amp_t* bt = b + (base | rtile_thread_global(threadIdx.x, p0));
 #pragma unroll
-        for (int r = 0; r < kRR; ++r) v[r] = ld_amp<NT>(b + (gt | p0.gofs[r]));
+        for (int r = 0; r < kRR; ++r) v[r] = ld_amp<NT>(bt + p0.gofs[r]);
     }
     bool lds_used = false;
     run_phases(v, w, lds, phases, n_phases, words, tile, lds_used);
     {
         const RPhase& pl = phases[n_phases - 1];
-        const uint64_t gt = base | pdep64(threadIdx.x, pl.tmask_global);
+        amp_t* bt = b + (base | rtile_thread_global(threadIdx.x, pl));
 #pragma unroll
-        for (int r = 0; r < kRR; ++r) st_amp<NT>(b + (gt | pl.gofs[r]), v[r]);
+        for (int r = 0; r < kRR; ++r) st_amp<NT>(bt + pl.gofs[r], v[r]);
     }
 }
 
@@ -685,12 +689,12 @@ __global__ void __launch_bounds__(kRT) k_rtile_stream(amp_t* __restrict__ s, uin
     const uint64_t outside = ((uint64_t{1} << nq) - 1) & ~smask;
     const RPhase& p0 = phases[0];
     const RPhase& pl = phases[n_phases - 1];
-    const uint64_t gt0 = pdep64(threadIdx.x, p0.tmask_global);
-    const uint64_t gtl = pdep64(threadIdx.x, pl.tmask_global);
+    const uint64_t gt0 = rtile_thread_global(threadIdx.x, p0);
+    const uint64_t gtl = rtile_thread_global(threadIdx.x, pl);
     uint32_t tile = blockIdx.x;
     Regs v, w, nx;
     {
-        const uint64_t gt = pdep64(tile, outside) | gt0;
+        const uint64_t gt = rtile_tile_base(tile, p0, outside) | gt0;
 #pragma unroll
         for (int r = 0; r < kRR; ++r) v[r] = b[gt | p0.gofs[r]];
     }
@@ -701,10 +705,10 @@ __global__ void __launch_bounds__(kRT) k_rtile_stream(amp_t* __restrict__ s, uin
     wait_vmem();
     bool lds_used = false;
     for (;;) {
-        const uint64_t base = pdep64(tile, outside);
+        const uint64_t base = rtile_tile_base(tile, p0, outside);
         const uint32_t next = tile + gridDim.x;
         if (next < tiles) {
-            const uint64_t gt = pdep64(next, outside) | gt0;
+            const uint64_t gt = rtile_tile_base(next, p0, outside) | gt0;
 #pragma unroll
             for (int r = 0; r < kRR; ++r) nx[r] = b[gt | p0.gofs[r]];
         }
@@ -760,14 +764,14 @@ __global__ void __launch_bounds__(kRT) k_rtile_land(amp_t* __restrict__ s, uint3
     const uint32_t wave = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6)));
     const uint32_t land_base = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(land)) + wave * 64 * 16;
     auto issue = [&](uint32_t t) {
-        const amp_t* g = b + (pdep64(t, outside) | lane_off);
+        const amp_t* g = b + (rtile_tile_base(t, phases[0], outside) | lane_off);
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) dma16(g + pdep64(k, hi_mask), land_base + k * 256 * 16);
     };
     const RPhase& p0 = phases[0];
     const RPhase& pl = phases[n_phases - 1];
-    const uint32_t jt0 = pdep32(threadIdx.x, p0.tmask);
-    const uint64_t gtl = pdep64(threadIdx.x, pl.tmask_global);
+    const uint32_t jt0 = rtile_thread_bits(threadIdx.x, p0);
+    const uint64_t gtl = rtile_thread_global(threadIdx.x, pl);
     uint32_t tile = blockIdx.x;
     issue(tile);
     wait_vmem();
@@ -782,7 +786,7 @@ __global__ void __launch_bounds__(kRT) k_rtile_land(amp_t* __restrict__ s, uint3
         if (next < tiles) issue(next);
         run_phases(v, w, lds, phases, n_phases, words, tile, lds_used);
         {
-            const uint64_t gt = pdep64(tile, outside) | gtl;
+            const uint64_t gt = rtile_tile_base(tile, p0, outside) | gtl;
 #pragma unroll
             for (int r = 0; r < kRR; ++r) b[gt | pl.gofs[r]] = v[r];
         }
@@ -1071,6 +1075,15 @@ bool build_rtile(const std::vector<TileOp>& tops_in, uint32_t L, uint64_t smask,
         int rb[4], k = 0;
         for (int p = 0; p < kRL; ++p)
             if (rmask & (1u << p)) rb[k++] = p;
+        for (int i = 0; i < 4; ++i) ph.hole_low[i] = (1u << rb[i]) - 1u;
+        {
+            int t = 0;
+            for (int q = 0; q < 64; ++q)
+                if (ph.tmask_global & (uint64_t{1} << q)) ph.tpos[t++] = static_cast<uint8_t>(q);
+            t = 0;
+            for (int q = 0; q < 64; ++q)
+                if (smask & (uint64_t{1} << q)) ph.tile_low[t++] = (uint64_t{1} << q) - 1;
+        }
         for (int r = 0; r < kRR; ++r) {
             uint32_t j = 0;
             for (int i = 0; i < 4; ++i)
@@ -1095,8 +1108,8 @@ bool build_rtile(const std::vector<TileOp>& tops_in, uint32_t L, uint64_t smask,
             }
             ph.h = static_cast<uint32_t>(best);
             for (int r = 0; r < kRR; ++r) {
-                ph.cs_prev[r] = rtile_slot(pp.jr[r], ph.h);
-                ph.cs_cur[r] = rtile_slot(ph.jr[r], ph.h);
+                ph.cs_prev[r] = rtile_slot(pp.jr[r], ph.h) << 4;
+                ph.cs_cur[r] = rtile_slot(ph.jr[r], ph.h) << 4;
             }
         }
         auto ridx = [&](uint32_t pos) {

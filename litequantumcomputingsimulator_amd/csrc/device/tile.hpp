@@ -77,12 +77,49 @@ struct RPhase {
     // The transpose into this layout moves the tile through LDS in two
     // halves, split by tile bit h (a thread bit of both layouts): the LDS
     // slot of tile index j is slot(j without bit h), XOR-linear, so slot(jt |
-    // jr) = slot(jt) ^ slot(jr); these are the register parts.
+    // jr) = slot(jt) ^ slot(jr); these are the register parts, as LDS byte
+    // offsets (slot * 16).
     uint32_t h;
     uint32_t pad2;
     uint32_t cs_prev[16];    // previous layout's register r
     uint32_t cs_cur[16];     // this layout's register r
+    // Index arithmetic without bit loops: (1 << p) - 1 for the 4 register
+    // bits p, ascending (the thread's tile bits are the thread id with zeros
+    // inserted there); the global qubit of each of the 8 thread bits; and
+    // (1 << q) - 1 for the 12 tile qubits q, ascending (the tile number with
+    // zeros inserted there is the tile's base index).
+    uint32_t hole_low[4];
+    uint8_t tpos[8];
+    uint64_t tile_low[12];
 };
+
+// Tile bits of thread `tid` in layout ph (pdep of tid over ph.tmask): each
+// insertion keeps the bits below the register bit and shifts the rest up;
+// the bit left at the register position is cleared by the final mask.
+__host__ __device__ __forceinline__ uint32_t rtile_thread_bits(uint32_t tid, const RPhase& ph) {
+    uint32_t x = tid;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) x = (x & ph.hole_low[i]) | ((x << 1) & ~ph.hole_low[i]);
+    return x & ph.tmask;
+}
+
+// Global index bits of thread `tid` in layout ph (pdep of tid over
+// ph.tmask_global).
+__host__ __device__ __forceinline__ uint64_t rtile_thread_global(uint32_t tid, const RPhase& ph) {
+    uint64_t g = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) g |= uint64_t((tid >> k) & 1u) << ph.tpos[k];
+    return g;
+}
+
+// Base index of tile number `tile` (pdep over the qubits outside the tile
+// set; `outside` = every qubit of the register not in the tile set).
+__host__ __device__ __forceinline__ uint64_t rtile_tile_base(uint64_t tile, const RPhase& ph, uint64_t outside) {
+    uint64_t x = tile;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) x = (x & ph.tile_low[i]) | ((x << 1) & ~ph.tile_low[i]);
+    return x & outside;
+}
 
 // LDS slot of tile index j in a half-tile transpose split by bit h: j with
 // bit h removed (11 bits), low 3 bits XOR-folded with higher ones so that

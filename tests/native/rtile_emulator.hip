@@ -254,8 +254,15 @@ int main(int argc, char** argv) {
                 if ((~ph->tmask & 0xfffu) & avoid) { why = "edge layout keeps low qubits in registers"; return -1; }
             }
             size_t seen_ops = 0;
+            for (uint64_t t : {uint64_t{0}, uint64_t{1}, uint64_t{tile}, (outside >> 12) ^ tile, uint64_t{0x2aaaa}})
+                if (lqcs_dev::rtile_tile_base(t & ((uint64_t{1} << (nq - kL)) - 1), prog.phases[0], outside) !=
+                    pdep(t & ((uint64_t{1} << (nq - kL)) - 1), outside)) { why = "tile base"; return -1; }
             for (size_t p = 0; p < prog.phases.size(); ++p) {
                 const RPhase& ph = prog.phases[p];
+                for (uint32_t t = 0; t < 256; ++t) {
+                    if (lqcs_dev::rtile_thread_bits(t, ph) != pdep(t, ph.tmask)) { why = "thread bits"; return -1; }
+                    if (lqcs_dev::rtile_thread_global(t, ph) != pdep(t, ph.tmask_global)) { why = "thread global"; return -1; }
+                }
                 if (__builtin_popcount(ph.tmask) != 8 || (ph.tmask & ~0xfffu)) { why = "tmask"; return -1; }
                 if (p > 0 && !ph.transpose && ph.tmask != prog.phases[p - 1].tmask) { why = "layout change without transpose"; return -1; }
                 if (p > 0 && ph.transpose) {
@@ -272,7 +279,8 @@ int main(int argc, char** argv) {
                             const uint32_t jp = static_cast<uint32_t>(pdep(t, pp.tmask));
                             if (((jp >> h) & 1u) != half) continue;
                             for (int r = 0; r < 16; ++r) {
-                                const uint32_t slot = lqcs_dev::rtile_slot(jp, h) ^ ph.cs_prev[r];
+                                const uint32_t slot = lqcs_dev::rtile_slot(jp, h) ^ (ph.cs_prev[r] >> 4);
+                                if (ph.cs_prev[r] & 15u) { why = "LDS register offset not in bytes"; return -1; }
                                 if (slot >= kN / 2 || owner[slot] != -1) { why = "LDS slot collision"; return -1; }
                                 owner[slot] = jp | pp.jr[r];
                             }
@@ -281,7 +289,7 @@ int main(int argc, char** argv) {
                             const uint32_t jc = static_cast<uint32_t>(pdep(t, ph.tmask));
                             if (((jc >> h) & 1u) != half) continue;
                             for (int r = 0; r < 16; ++r) {
-                                const uint32_t slot = lqcs_dev::rtile_slot(jc, h) ^ ph.cs_cur[r];
+                                const uint32_t slot = lqcs_dev::rtile_slot(jc, h) ^ (ph.cs_cur[r] >> 4);
                                 if (slot >= kN / 2 || owner[slot] != int64_t(jc | ph.jr[r])) { why = "LDS read of the wrong amplitude"; return -1; }
                             }
                         }
