@@ -513,7 +513,7 @@ void launch_staged(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan, const st
         }
         const StagedPlan::Where& w = sp.where[i];
         if (w.kind == 2) {
-            launch_rtile(s, nq, st.smask, d_ph + w.first, static_cast<uint32_t>(w.count),
+            launch_rtile(s, nq, L, st.smask, d_ph + w.first, static_cast<uint32_t>(w.count),
                          d_rwords ? d_rwords + w.words : nullptr, batch);
         } else if (w.kind == 1) {
             launch_tile_pass(s, nq, L, st.smask, d_ops + w.first, static_cast<uint32_t>(w.count), batch);

@@ -25,8 +25,11 @@ using cd = std::complex<double>;
 
 namespace {
 
-constexpr int kL = 12;
-constexpr uint32_t kN = 1u << kL;
+// Tile width of the current case: 12 (four-wave tiles) or 10 (wave tiles);
+// both transpose in two half-tile rounds.
+int kL = 12;
+uint32_t kN = 1u << 12;
+uint32_t kT = 256;  // threads per tile
 
 uint64_t pdep(uint64_t v, uint64_t mask) {
     uint64_t r = 0;
@@ -98,7 +101,10 @@ int main(int argc, char** argv) {
     int total_phases = 0, total_transposes = 0;
     size_t merged_total = 0;
     for (int c = 0; c < cases; ++c) {
-        const uint32_t nq = 12 + rng() % 9;  // 12..20
+        kL = (c % 2) ? 10 : 12;
+        kN = 1u << kL;
+        kT = kN >> 4;
+        const uint32_t nq = kL + rng() % 9;
         // Tile set: `low` contiguous low qubits plus random others.
         const uint32_t low = rng() % 7;
         uint64_t smask = (uint64_t{1} << low) - 1;
@@ -244,14 +250,14 @@ int main(int argc, char** argv) {
             std::vector<cd> want = init, a = init;
             for (const TileOp& t : ops) apply_direct(want, t, base);
             for (const RPhase* ph : {&prog.phases.front(), &prog.phases.back()}) {
-                for (uint32_t t = 0; t < 256; ++t)
+                for (uint32_t t = 0; t < kT; ++t)
                     for (int r = 0; r < 16; ++r) {
                         const uint32_t j = static_cast<uint32_t>(pdep(t, ph->tmask)) | ph->jr[r];
                         if ((pdep(t, ph->tmask_global) | ph->gofs[r]) != pdep(j, smask)) { why = "gofs"; return -1; }
                     }
                 const uint32_t run = static_cast<uint32_t>(__builtin_ctzll(~smask));
                 const uint32_t avoid = (1u << std::min<uint32_t>(run, 6)) - 1;
-                if ((~ph->tmask & 0xfffu) & avoid) { why = "edge layout keeps low qubits in registers"; return -1; }
+                if ((~ph->tmask & (kN - 1)) & avoid) { why = "edge layout keeps low qubits in registers"; return -1; }
             }
             size_t seen_ops = 0;
             for (uint64_t t : {uint64_t{0}, uint64_t{1}, uint64_t{tile}, (outside >> 12) ^ tile, uint64_t{0x2aaaa}})
@@ -259,11 +265,11 @@ int main(int argc, char** argv) {
                     pdep(t & ((uint64_t{1} << (nq - kL)) - 1), outside)) { why = "tile base"; return -1; }
             for (size_t p = 0; p < prog.phases.size(); ++p) {
                 const RPhase& ph = prog.phases[p];
-                for (uint32_t t = 0; t < 256; ++t) {
+                for (uint32_t t = 0; t < kT; ++t) {
                     if (lqcs_dev::rtile_thread_bits(t, ph) != pdep(t, ph.tmask)) { why = "thread bits"; return -1; }
                     if (lqcs_dev::rtile_thread_global(t, ph) != pdep(t, ph.tmask_global)) { why = "thread global"; return -1; }
                 }
-                if (__builtin_popcount(ph.tmask) != 8 || (ph.tmask & ~0xfffu)) { why = "tmask"; return -1; }
+                if (__builtin_popcount(ph.tmask) != kL - 4 || (ph.tmask & ~(kN - 1))) { why = "tmask"; return -1; }
                 if (p > 0 && !ph.transpose && ph.tmask != prog.phases[p - 1].tmask) { why = "layout change without transpose"; return -1; }
                 if (p > 0 && ph.transpose) {
                     ++n_tr;
@@ -275,7 +281,7 @@ int main(int argc, char** argv) {
                     if (h >= 12 || !(pp.tmask & ph.tmask & (1u << h))) { why = "split bit is not a thread bit of both layouts"; return -1; }
                     for (uint32_t half = 0; half < 2; ++half) {
                         std::vector<int64_t> owner(kN / 2, -1);
-                        for (uint32_t t = 0; t < 256; ++t) {
+                        for (uint32_t t = 0; t < kT; ++t) {
                             const uint32_t jp = static_cast<uint32_t>(pdep(t, pp.tmask));
                             if (((jp >> h) & 1u) != half) continue;
                             for (int r = 0; r < 16; ++r) {
@@ -285,7 +291,7 @@ int main(int argc, char** argv) {
                                 owner[slot] = jp | pp.jr[r];
                             }
                         }
-                        for (uint32_t t = 0; t < 256; ++t) {
+                        for (uint32_t t = 0; t < kT; ++t) {
                             const uint32_t jc = static_cast<uint32_t>(pdep(t, ph.tmask));
                             if (((jc >> h) & 1u) != half) continue;
                             for (int r = 0; r < 16; ++r) {
@@ -315,7 +321,7 @@ int main(int argc, char** argv) {
                     const amp_t* m = &prog.words[off + 2];
                     std::vector<cd> o = a;
                     if ((tile & h.gctrl) == h.gctrl) {
-                        for (uint32_t t = 0; t < 256; ++t) {
+                        for (uint32_t t = 0; t < kT; ++t) {
                             const uint32_t jt = static_cast<uint32_t>(pdep(t, ph.tmask));
                             const bool tp = (jt & h.tctl) == h.tctl;
                             auto J = [&](int r) { return jt | ph.jr[r]; };

@@ -211,6 +211,8 @@ def test_diagonal_and_select_forms_match_oracle(lq, seed):
         for opts in (dict(plan="greedy", tile_qubits=12, low_qubits=4, tile_kernel="register"),
                      dict(plan="gain", tile_qubits=12, low_qubits=2, tile_kernel="register"),
                      dict(plan="greedy", tile_qubits=10, low_qubits=3, tile_kernel="lds"),
+                     dict(plan="greedy", tile_qubits=10, low_qubits=3, tile_kernel="register"),
+                     dict(plan="gain", tile_qubits=10, low_qubits=4, tile_kernel="register"),
                      dict(plan="gate_by_gate")):
             sv = lq.statevector(n)
             comp.apply(sv, **opts)

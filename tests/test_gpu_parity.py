@@ -112,6 +112,36 @@ def test_random_layers_greedy_vs_oracle(lq, n, layers):
         assert np.max(np.abs(got - want)) <= TOL, eng
 
 
+WAVE_PLANS = [dict(plan="greedy", tile_qubits=10, low_qubits=3), dict(plan="gain", tile_qubits=10, low_qubits=2),
+              dict(plan="gain", tile_qubits=10, low_qubits=4), dict(plan="greedy", tile_qubits=10, low_qubits=6)]
+
+
+@pytest.mark.parametrize("n,seed", [(13, 11), (16, 22)])
+@pytest.mark.parametrize("eng", WAVE_PLANS)
+def test_wave_tiles_match_oracle(lq, n, seed, eng):
+    """Wave tiles (10-qubit tiles, one wave each, wave-private transposes):
+    every gate kind, Pauli rotations with Z parts on tile and outside
+    qubits, multi-controlled and diagonal gates, against the oracle."""
+    c = random_circuit(n, 160, seed)
+    c.rp(0.3, "XZY", [0, n // 2, n - 1]).rp(0.9, "ZIZ", [2, n - 3, 4]).rp(-0.5, "YY", [1, n - 2])
+    c.crz(0.4, n - 1, 3).mcp(0.7, [0, n - 2], 5).cy(2, n - 4).cz(n - 1, 0).ccx(1, n - 1, 6)
+    want = O.run_statevector(c, fuse=True)
+    got = product_state(lq, c, True, **eng)
+    assert np.max(np.abs(got - want)) <= TOL
+
+
+def test_wave_tiles_noisy_trajectories_match_oracle_counts(lq):
+    """Batched trajectories on wave tiles (the batch member is the
+    workgroup's second grid index): counts equal the oracle's."""
+    from tests import cases
+
+    c, chans = cases.noisy_ghz(12)
+    want = O.run_counts(c, seed=13, shots=256, channels=chans)
+    got = lq.StatevectorSimulator(seed=13, noise=_noise_model(lq), plan="greedy", tile_qubits=10,
+                                  low_qubits=3).run(c.to_product(lq), 256).counts()
+    assert dict(got) == want
+
+
 def _ambiguous(cum, draws, margin):
     total = cum[-1]
     r = draws * total
