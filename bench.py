@@ -26,11 +26,20 @@ committed under profiles/ for the same kernel and workload, when one exists.
 """
 from __future__ import annotations
 
-import argparse
-import json
 import os
-import sys
-import time
+
+# The CPU legs run the reference's OpenMP kernels in this process: thread
+# binding as the reference's own benchmark plan asks (one thread per core,
+# spread), set before any OpenMP runtime is loaded.
+os.environ.setdefault("OMP_PROC_BIND", "spread")
+os.environ.setdefault("OMP_PLACES", "cores")
+
+import argparse  # noqa: E402
+import functools  # noqa: E402
+import json  # noqa: E402
+import subprocess  # noqa: E402
+import sys  # noqa: E402
+import time  # noqa: E402
 
 PEAK_HBM_GBPS = 8000.0
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -38,6 +47,44 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
+
+
+@functools.lru_cache(maxsize=1)
+def host_cpu() -> dict:
+    """The host the CPU legs run on: physical cores and model from lscpu, and
+    the OpenMP threads used (one per physical core available to this
+    process, capped by OMP_NUM_THREADS when the environment sets it)."""
+    model, cores = "unknown", 0
+    try:
+        info = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in info.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+        table = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, timeout=20).stdout
+        allowed = os.sched_getaffinity(0)
+        phys, phys_allowed = set(), set()
+        for line in table.splitlines():
+            if line.startswith("#"):
+                continue
+            cpu, core, sock = (int(x) for x in line.split(",")[:3])
+            phys.add((sock, core))
+            if cpu in allowed:
+                phys_allowed.add((sock, core))
+        cores = len(phys)
+        threads = len(phys_allowed) or len(allowed)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        threads = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        threads = min(threads, int(env))
+    return {"threads": threads, "physical_cores": cores, "cpu_model": model,
+            "omp": f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} OMP_PLACES={os.environ.get('OMP_PLACES')}"}
+
+
+def cpu_fields(cores: int) -> dict:
+    """Common fields of every cpu_baseline object."""
+    h = host_cpu()
+    return {"cores": cores, "physical_cores": h["physical_cores"], "cpu_model": h["cpu_model"], "omp": h["omp"]}
 
 
 def cpu_baseline(n: int, layers: int, seed: int, budget_s: float) -> dict | None:
@@ -51,7 +98,7 @@ def cpu_baseline(n: int, layers: int, seed: int, budget_s: float) -> dict | None
     except Exception as e:  # noqa: BLE001
         log(f"cpu baseline unavailable: {e}")
         return None
-    cores = len(os.sched_getaffinity(0))
+    cores = host_cpu()["threads"]
     kind = "reference" if O.ref_available() else "port"
     n_s = min(n, 26)
     c = O.Circuit(n_s)
@@ -74,7 +121,7 @@ def cpu_baseline(n: int, layers: int, seed: int, budget_s: float) -> dict | None
     return {
         "value": 1.0 / per_gate_full,
         "unit": "gates/s",
-        "cores": cores,
+        **cpu_fields(cores),
         "kind": kind,
         "sample": (f"{reps} x run_statevector of 1 layer ({gates} gates) of the same generator at "
                    f"{n_s} qubits, fusion on, OMP threads={cores}; seconds per gate scaled by "
@@ -180,7 +227,7 @@ def _ps_cpu_baseline(W, sizes, seed: int, budget_s: float = 30.0) -> dict | None
         t.rp(float(th), p, qs)
     ref = O.ref_available()
     kind = "reference" if ref else "port"
-    cores = len(os.sched_getaffinity(0))
+    cores = host_cpu()["threads"]
     if ref:
         O.ref_set_threads(cores)
     run = O.ref_run_statevector if ref else O.run_statevector
@@ -204,7 +251,33 @@ def _ps_cpu_baseline(W, sizes, seed: int, budget_s: float = 30.0) -> dict | None
                   f"rotations): {t_two:.3f} s, scaled by 400 to the 800 evaluations of one gradient")
     log(f"parameter-shift cpu baseline: {per_vector:.2f} s per gradient vector at {n} qubits ({kind})")
     return {"value": round(1.0 / per_vector, 5), "unit": "gradient vectors/s (400 parameters, one circuit)",
-            "cores": cores, "kind": kind, "sample": sample}
+            **cpu_fields(cores), "kind": kind, "sample": sample}
+
+
+def tile_roofline(prof: dict, traffic_json: str | None = None) -> dict | None:
+    """Roofline of the tiled passes in a launch profile: every pass reads and
+    writes each amplitude of the registers it covers once (32 bytes per
+    amplitude), timed by HIP events around each launch on the engine stream."""
+    if not prof.get("tile_launches") or not prof.get("tile_ms"):
+        return None
+    by = 32.0 * prof["tile_amps"]
+    achieved = by / (prof["tile_ms"] * 1e6)
+    traffic, source = None, None
+    if traffic_json:
+        path = os.path.join(ROOT, "profiles", traffic_json)
+        try:
+            with open(path) as f:
+                pm = json.load(f)
+            kib = pm["fetch_kib_per_dispatch"] * pm["fetch_scale"] + pm["write_kib_per_dispatch"] * pm["write_scale"]
+            traffic, source = kib * 1024.0, f"profiles/{traffic_json}"
+        except (OSError, ValueError, KeyError):
+            pass
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": traffic, "traffic_source": source,
+            "kernel": "k_rtile", "bytes_per_launch": by / prof["tile_launches"],
+            "avg_launch_ms": round(prof["tile_ms"] / prof["tile_launches"], 4),
+            "launches": prof["tile_launches"], "share_of_kernel_ms":
+                round(prof["tile_ms"] / (prof["tile_ms"] + prof["other_ms"]), 4)}
 
 
 def trajectories_bench(lq, W, seed: int, n: int, shots: int, cpu: bool = True, cpu_shots: int = 1) -> dict:
@@ -224,12 +297,19 @@ def trajectories_bench(lq, W, seed: int, n: int, shots: int, cpu: bool = True, c
     counts = sim.run(qc, shots).counts()
     lq.synchronize()
     dt = time.perf_counter() - t0
+    # the same run again with every launch bracketed by events (not the timed one)
+    lq.profile_enable(True)
+    lq.profile_read()
+    sim.run(qc, shots)
+    prof = lq.profile_read()
+    lq.profile_enable(False)
     gates = qc.num_instructions() - n
     log(f"trajectories: {shots} shots of {n} qubits in {dt:.3f} s")
     out = {"workload": (f"GHZ prep + 10 random U3/CZ layers, {n} qubits, {gates} gates, depolarizing(1e-3) + "
                         f"amplitude_damping(1e-3) after every gate, measure_all"),
            "shots": shots, "seconds": round(dt, 3), "trajectories_per_s": round(shots / dt, 2),
-           "trajectory_gates_per_s": round(shots * gates / dt, 1), "distinct_outcomes": len(dict(counts))}
+           "trajectory_gates_per_s": round(shots * gates / dt, 1), "distinct_outcomes": len(dict(counts)),
+           "roofline": tile_roofline(prof, f"k_rtile_trajectories_{n}q.json")}
     if cpu:
         try:
             from oracle import oracle as O
@@ -238,7 +318,7 @@ def trajectories_bench(lq, W, seed: int, n: int, shots: int, cpu: bool = True, c
             c.measure_all()
             chans = [O.depolarizing(1e-3), O.amplitude_damping(1e-3)]
             kind = "reference" if O.ref_available() else "port"
-            cores = len(os.sched_getaffinity(0))
+            cores = host_cpu()["threads"]
             t0 = time.perf_counter()
             if kind == "reference":
                 O.ref_set_threads(cores)
@@ -246,7 +326,7 @@ def trajectories_bench(lq, W, seed: int, n: int, shots: int, cpu: bool = True, c
             else:
                 O.run_counts(c, seed, cpu_shots, channels=chans)
             dtc = time.perf_counter() - t0
-            out["cpu_baseline"] = {"value": round(cpu_shots / dtc, 4), "unit": "trajectories/s", "cores": cores,
+            out["cpu_baseline"] = {"value": round(cpu_shots / dtc, 4), "unit": "trajectories/s", **cpu_fields(cores),
                                    "kind": kind,
                                    "sample": f"run() of {cpu_shots} shot(s) of the same circuit and noise, {dtc:.2f} s"}
             log(f"trajectories cpu baseline: {cpu_shots} shot(s) in {dtc:.2f} s ({kind})")
@@ -286,9 +366,11 @@ def density_matrix_bench(lq, W, seed: int, widths=(12, 14, 16), cpu: bool = True
         kms = prof["tile_ms"] + prof["other_ms"]
         out[f"n{n}"] = {"seconds": round(dt, 4), "gates": qc.num_instructions(), "launches": passes,
                         "kernel_ms": round(kms, 3), "trace": rho.trace(),
-                        "GB/s": round(passes * 32 * 4**n / (kms * 1e6), 1) if kms else None}
+                        "GB/s": round(passes * 32 * 4**n / (kms * 1e6), 1) if kms else None,
+                        "roofline": tile_roofline(prof, f"k_rtile_density_matrix_{n}q.json")}
         log(f"density matrix {n} qubits: {dt:.3f} s, {passes} launches, {kms:.1f} ms of kernels")
         del rho
+    out["roofline"] = out[f"n{widths[-1]}"]["roofline"]
     if cpu:
         try:
             from oracle import oracle as O
@@ -296,7 +378,7 @@ def density_matrix_bench(lq, W, seed: int, widths=(12, 14, 16), cpu: bool = True
             W.noisy_ghz_layers(c, cpu_width, 10, seed)
             chans = [O.depolarizing(1e-3), O.amplitude_damping(1e-3)]
             kind = "reference" if O.ref_available() else "port"
-            cores = len(os.sched_getaffinity(0))
+            cores = host_cpu()["threads"]
             t0 = time.perf_counter()
             if kind == "reference":
                 O.ref_set_threads(cores)
@@ -305,7 +387,7 @@ def density_matrix_bench(lq, W, seed: int, widths=(12, 14, 16), cpu: bool = True
                 O.density_probabilities(c, chans)
             dt = time.perf_counter() - t0
             scaled = dt * 4 ** (12 - cpu_width)
-            out["cpu_baseline"] = {"value": round(scaled, 3), "unit": "seconds for rho at 12 qubits", "cores": cores,
+            out["cpu_baseline"] = {"value": round(scaled, 3), "unit": "seconds for rho at 12 qubits", **cpu_fields(cores),
                                    "kind": kind,
                                    "sample": f"run_density_matrix of the same generator at {cpu_width} qubits "
                                              f"({dt:.2f} s), scaled by 4^{12 - cpu_width}"}
@@ -423,7 +505,7 @@ def dense_blocks_bench(lq, seed: int, n: int = 30, reps: int = 4, cpu_seconds: f
             c = O.Circuit(n_s)
             for m in mats[6]:
                 c.unitary(m, list(range(8, 14)))
-            cores = len(os.sched_getaffinity(0))
+            cores = host_cpu()["threads"]
             kind = "reference" if O.ref_available() else "port"
             if kind == "reference":
                 O.ref_set_threads(cores)
@@ -438,7 +520,7 @@ def dense_blocks_bench(lq, seed: int, n: int = 30, reps: int = 4, cpu_seconds: f
                 reps_done += 1
             per_gate = t_total / (reps_done * reps) * (1 << (n - n_s))
             out["cpu_baseline"] = {"value": round(1e3 * per_gate, 3), "unit": "ms per k=6 block at 30 qubits",
-                                   "cores": cores, "kind": kind,
+                                   **cpu_fields(cores), "kind": kind,
                                    "sample": f"{reps_done} x run_statevector of the {reps} k=6 blocks at {n_s} "
                                              f"qubits; seconds per block scaled by 2^{n - n_s}"}
             log(f"dense cpu baseline: {1e3 * per_gate:.1f} ms per k=6 block at {n} qubits ({kind})")

@@ -97,6 +97,7 @@ struct LaunchProfile {
     double tile_ms = 0.0;
     std::uint64_t other_launches = 0;
     double other_ms = 0.0;
+    std::uint64_t tile_amps = 0;  // amplitudes passed over by the tiled passes
 };
 void profile_enable(bool on);
 LaunchProfile profile_read();

@@ -171,12 +171,15 @@ lqcs_status lqcs_set_tile_kernel(int kernel);
 lqcs_status lqcs_set_dense_mfma_min_targets(uint32_t min_targets);
 
 /* Launch timing (no counterpart in the reference): while enabled, every
- * tiled pass and every gate-by-gate kernel of lqcs_state_apply is bracketed
- * by HIP events on the engine stream. lqcs_profile_read synchronises and
- * returns launch counts and summed kernel milliseconds, then clears them. */
+ * tiled pass and every gate-by-gate kernel of lqcs_state_apply (and of the
+ * batched trajectory and density-matrix runs built on it) is bracketed by
+ * HIP events on the engine stream. lqcs_profile_read synchronises and
+ * returns launch counts, summed kernel milliseconds and the amplitudes the
+ * tiled passes passed over (registers x 2^n per pass; each pass reads and
+ * writes 16 bytes per amplitude), then clears them. Any pointer may be NULL. */
 lqcs_status lqcs_profile_enable(int enable);
 lqcs_status lqcs_profile_read(uint64_t* tile_launches, double* tile_ms, uint64_t* other_launches,
-                              double* other_ms);
+                              double* other_ms, uint64_t* tile_amps);
 
 /* Launch times of the same recording, one entry per launch in order
  * (tiled[i] = 1 for a tiled pass), then cleared; n_launches receives the

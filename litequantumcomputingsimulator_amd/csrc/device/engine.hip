@@ -53,6 +53,7 @@ struct Profile {
     struct Launch {
         bool tiled;
         hipEvent_t a, b;
+        uint64_t amps;  // amplitudes the launch passes over (batch * 2^n)
     };
     bool on = false;
     std::vector<Launch> launches;  // in launch order
@@ -76,14 +77,20 @@ hipEvent_t take_event() {
 thread_local uint64_t g_last_passes = 0, g_last_launches = 0;
 
 // Elapsed milliseconds of every recorded launch, in order; recycles events.
-std::vector<std::pair<bool, double>> drain_launches() {
+struct Drained {
+    bool tiled;
+    double ms;
+    uint64_t amps;
+};
+
+std::vector<Drained> drain_launches() {
     Profile& p = prof();
-    std::vector<std::pair<bool, double>> out;
+    std::vector<Drained> out;
     out.reserve(p.launches.size());
     for (const auto& l : p.launches) {
         float t = 0.0f;
         LQCS_HIP(hipEventElapsedTime(&t, l.a, l.b));
-        out.emplace_back(l.tiled, double(t));
+        out.push_back({l.tiled, double(t), l.amps});
         p.pool.push_back(l.a);
         p.pool.push_back(l.b);
     }
@@ -522,7 +529,7 @@ void launch_staged(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan, const st
         }
         if (timed) {
             LQCS_HIP(hipEventRecord(e1, stream()));
-            prof().launches.push_back({st.tiled, e0, e1});
+            prof().launches.push_back({st.tiled, e0, e1, uint64_t(batch) << nq});
         }
     }
 }
@@ -564,7 +571,7 @@ void apply_ops(amp_t* s, uint32_t nq, const std::vector<DecodedOp>& ops, int pla
             run_single_op(s, nq, d, batch);
             if (timed) {
                 LQCS_HIP(hipEventRecord(e1, stream()));
-                prof().launches.push_back({false, e0, e1});
+                prof().launches.push_back({false, e0, e1, uint64_t(batch) << nq});
             }
         }
         g_last_passes = ops.size();
@@ -727,20 +734,22 @@ lqcs_status lqcs_profile_enable(int enable) {
 }
 
 lqcs_status lqcs_profile_read(uint64_t* tile_launches, double* tile_ms, uint64_t* other_launches,
-                              double* other_ms) {
+                              double* other_ms, uint64_t* tile_amps) {
     return guarded([&] {
         ensure_ready();
         sync();
-        uint64_t n[2] = {0, 0};
+        uint64_t n[2] = {0, 0}, amps = 0;
         double ms[2] = {0.0, 0.0};
-        for (const auto& [tiled, t] : drain_launches()) {
-            ++n[tiled ? 1 : 0];
-            ms[tiled ? 1 : 0] += t;
+        for (const Drained& d : drain_launches()) {
+            ++n[d.tiled ? 1 : 0];
+            ms[d.tiled ? 1 : 0] += d.ms;
+            if (d.tiled) amps += d.amps;
         }
         if (tile_launches) *tile_launches = n[1];
         if (tile_ms) *tile_ms = ms[1];
         if (other_launches) *other_launches = n[0];
         if (other_ms) *other_ms = ms[0];
+        if (tile_amps) *tile_amps = amps;
     });
 }
 
@@ -752,8 +761,8 @@ lqcs_status lqcs_profile_read_launches(double* ms, uint8_t* tiled, uint64_t max_
         const auto all = drain_launches();
         *n_launches = all.size();
         for (uint64_t i = 0; i < all.size() && i < max_launches; ++i) {
-            if (ms) ms[i] = all[i].second;
-            if (tiled) tiled[i] = all[i].first ? 1 : 0;
+            if (ms) ms[i] = all[i].ms;
+            if (tiled) tiled[i] = all[i].tiled ? 1 : 0;
         }
     });
 }

@@ -169,8 +169,13 @@ double DensityMatrix::fidelity(const Statevector& psi) const {
 // ---- DensityMatrixSimulator ----------------------------------------------------------------
 // density_matrix_simulator.cpp:13-28: Reset rejected, Measure/Barrier skipped,
 // every channel on every touched qubit after each unitary, channel-major. The
-// whole circuit becomes one op stream, planned into tiled passes; with noise,
-// a single-qubit gate and its channels fold into one superoperator.
+// whole circuit becomes one op stream, planned into tiled passes. With noise,
+// everything that acts on one (ket q, bra q) pair alone -- a single-qubit gate
+// with its channels, S * (U (x) conj(U)), or the channels after a multi-qubit
+// gate -- is collected into one pending 4x4 per qubit and multiplied by the
+// next such block on that pair; a pending block is emitted just before the
+// next op that touches q, or at the end. The pair's blocks commute with every
+// op on other qubits, so the order of operations on rho is unchanged.
 DensityMatrix DensityMatrixSimulator::run_density_matrix(const QuantumCircuit& circuit) const {
     const std::size_t n = circuit.num_qubits();
     DensityMatrix rho(n, options_.max_qubits);
@@ -181,28 +186,38 @@ DensityMatrix DensityMatrixSimulator::run_density_matrix(const QuantumCircuit& c
     std::vector<complex_t> chain;
     for (const KrausChannel& ch : channels) chain = chain.empty() ? superoperator(ch) : mul4(superoperator(ch), chain);
     detail::ProgramBuilder b;
+    std::vector<std::vector<complex_t>> pending(n);  // empty: nothing pending on that pair
+    auto push = [&](qubit_t q, const std::vector<complex_t>& m) {
+        pending[q] = pending[q].empty() ? m : mul4(m, pending[q]);
+    };
+    auto flush = [&](qubit_t q) {
+        if (pending[q].empty()) return;
+        b.add_two_qubit(q, static_cast<qubit_t>(q + n), pending[q]);
+        pending[q].clear();
+    };
     for (const Instruction& inst : circuit.instructions()) {
         if (inst.gate.type == GateType::Reset)
             throw std::invalid_argument("DensityMatrixSimulator: Reset is not supported yet");
         const Gate& g = inst.gate;
         if (!g.is_unitary()) continue;
         if (!chain.empty() && g.n_controls == 0 && g.base_qubits() == 1 && g.type != GateType::Permutation) {
-            // A noisy single-qubit gate is one 4x4 op on (q, q + n):
-            // S * (U (x) conj(U)), the superoperator of U followed by the channels.
+            // A noisy single-qubit gate: S * (U (x) conj(U)) on (q, q + n).
             check_qubits(inst, n);
             const std::vector<complex_t> u = g.base_matrix();
             std::vector<complex_t> m(16);
             for (int l = 0; l < 4; ++l)
                 for (int lp = 0; lp < 4; ++lp)
                     m[4 * l + lp] = u[2 * (l & 1) + (lp & 1)] * std::conj(u[2 * (l >> 1) + (lp >> 1)]);
-            const qubit_t q = inst.qubits[0];
-            b.add_two_qubit(q, static_cast<qubit_t>(q + n), mul4(chain, m));
+            push(inst.qubits[0], mul4(chain, m));
             continue;
         }
+        check_qubits(inst, n);
+        for (qubit_t q : inst.qubits) flush(q);
         add_unitary(b, inst, n);
         if (!chain.empty())
-            for (qubit_t q : inst.qubits) b.add_two_qubit(q, static_cast<qubit_t>(q + n), chain);
+            for (qubit_t q : inst.qubits) push(q, chain);
     }
+    for (qubit_t q = 0; q < n; ++q) flush(q);
     apply_program(rho.device(), b, options_.engine);
     return rho;
 }

@@ -461,7 +461,7 @@ void profile_enable(bool on) { check(lqcs_profile_enable(on ? 1 : 0)); }
 
 LaunchProfile profile_read() {
     LaunchProfile p;
-    check(lqcs_profile_read(&p.tile_launches, &p.tile_ms, &p.other_launches, &p.other_ms));
+    check(lqcs_profile_read(&p.tile_launches, &p.tile_ms, &p.other_launches, &p.other_ms, &p.tile_amps));
     return p;
 }
 

@@ -557,7 +557,8 @@ PYBIND11_MODULE(_pylqcs, m) {
     m.def("profile_read", [] {
         const LaunchProfile p = profile_read();
         return py::dict("tile_launches"_a = p.tile_launches, "tile_ms"_a = p.tile_ms,
-                        "other_launches"_a = p.other_launches, "other_ms"_a = p.other_ms);
+                        "other_launches"_a = p.other_launches, "other_ms"_a = p.other_ms,
+                        "tile_amps"_a = p.tile_amps);
     });
 
     m.def("profile_read_launches", [] {

@@ -101,5 +101,25 @@ def ExpectationFunction():
 
 def expectation(theta, ansatz, hamiltonian):
     """Differentiable <psi(theta)|H|psi(theta)> for a ParametricCircuit;
-    hamiltonian = [(coeff, pauli), ...]."""
+    hamiltonian = [(coeff, pauli), ...].
+
+    When no gradient can flow (theta does not require one, or autograd is
+    off), only the forward state is evolved and H read once; otherwise the
+    adjoint method returns E and dE/dtheta from one forward and one backward
+    sweep, and backward() uses that gradient.
+
+    theta must be a host (CPU) tensor: the engine drives the GPU through the
+    system HIP runtime, and the torch wheel carries a HIP runtime of its own;
+    whichever of the two opens the device first holds it for the process. A
+    theta on another device raises ValueError instead of touching the GPU
+    through torch. The result comes back on the CPU in theta's dtype."""
+    torch = _torch()
+    if theta.device.type != "cpu":
+        raise ValueError(
+            "expectation: theta must be a CPU tensor (the engine and torch's bundled HIP runtime cannot share "
+            "the GPU in one process); pass theta.detach().cpu() from a process where torch has not opened the GPU")
+    if not (torch.is_grad_enabled() and theta.requires_grad):
+        params = theta.detach().to(torch.float64).tolist()
+        sv = StatevectorSimulator().run_statevector(ansatz.bind(params))
+        return torch.tensor(algorithms.expectation(sv, hamiltonian), dtype=theta.dtype)
     return _Expectation.get().apply(theta, ansatz, hamiltonian)

@@ -351,6 +351,27 @@ def test_torch_expectation_backward_matches_oracle(lq):
     assert np.max(np.abs(th32.grad.numpy().astype(np.float64) - want32)) <= 1e-5
 
 
+def test_torch_expectation_forward_only_without_grad(lq):
+    """Under torch.no_grad(), or for a theta that needs no gradient, the
+    bridge evolves the state once (no adjoint sweep) and returns the same
+    energy, with nothing to differentiate; a theta off the CPU is refused
+    before any device work (a meta tensor stands for one here, so torch never
+    opens the GPU in this process)."""
+    torch = pytest.importorskip("torch")
+    n = 6
+    pc, t, pinst, theta = _ansatz(lq, n, 10, seed=33)
+    h = W.random_hamiltonian(n, 10, seed=34)
+    e_want = O.bound_energy(t, pinst, theta, h)
+    th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+    with torch.no_grad():
+        e = lq.expectation(th, pc, h)
+    assert not e.requires_grad and abs(e.item() - e_want) <= 1e-10
+    e2 = lq.expectation(torch.tensor(theta, dtype=torch.float64), pc, h)
+    assert not e2.requires_grad and abs(e2.item() - e_want) <= 1e-10
+    with pytest.raises(ValueError, match="CPU tensor"):
+        lq.expectation(torch.empty(len(theta), dtype=torch.float64, device="meta"), pc, h)
+
+
 def test_errors_keep_reference_types(lq):
     with pytest.raises(ValueError):
         lq.Statevector(31)
