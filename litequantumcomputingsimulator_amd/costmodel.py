@@ -216,7 +216,9 @@ def install_default(lq) -> bool:
 
 
 def default_specs(seed: int = 0, widths=(24, 26, 28, 30)):
-    """The training set: every family at several widths and depths."""
+    """The training set: every family at several widths and depths, plus deep
+    random U3/CX and U3/CX/CP circuits (depth 20-40) from 30 qubits up, the
+    shape of the planner workload."""
     rng = np.random.default_rng(seed)
     specs = []
     for n in widths:
@@ -224,6 +226,10 @@ def default_specs(seed: int = 0, widths=(24, 26, 28, 30)):
             for _ in range(2 if fam != "qft" else 1):
                 depth = int(rng.integers(3, 13)) if fam != "pauli" else int(rng.integers(2, 7))
                 specs.append((n, fam, depth, int(rng.integers(1 << 30))))
+        if n >= 30:
+            for _ in range(3 if n >= 32 else 1):
+                for fam in ("u3cx", "u3cxcp"):
+                    specs.append((n, fam, int(rng.integers(20, 41)), int(rng.integers(1 << 30))))
     return specs
 
 

@@ -239,3 +239,26 @@ def test_learned_plan_32_qubits_matches_greedy_state(lq):
         assert lq.last_plan_choice()["predicted_ms"] > 0
     assert abs(a.norm() - 1.0) <= 1e-10
     assert a.max_abs_diff(b) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_learned_plan_matches_oracle_on_planner_workload(lq):
+    """The learned-planner workload's shape below the reference's 30-qubit
+    cap: random U3/CX/CP layers of depth 40 at 20 qubits, run with the plan
+    the installed model picks, against the oracle's statevector (the
+    reference's fused path restated) at 1e-12."""
+    from litequantumcomputingsimulator_amd import costmodel as CM
+    from litequantumcomputingsimulator_amd import workloads as W
+    from oracle import oracle as O
+
+    n = 20
+    ref = O.Circuit(n)
+    W.random_u3_cx_cp(ref, n, 40, seed=20)
+    want = O.run_statevector(ref, fuse=True)
+    CM.install_default(lq)
+    comp = lq.CompiledCircuit(ref.to_product(lq), True)
+    sv = lq.statevector(n)
+    comp.apply(sv, plan="learned")
+    choice = lq.last_plan_choice()
+    assert choice["plan"] in ("greedy", "gain")
+    assert np.max(np.abs(np.asarray(sv.to_numpy()) - want)) <= 1e-12
