@@ -208,6 +208,58 @@ def adjoint_bench(lq, W, seed: int, n_circuits: int, ps_seconds: float, cpu: boo
     return out
 
 
+def adjoint_plan_bench(lq, W, seed: int, reps: int = 20) -> dict:
+    """Captured adjoint plans against the uncaptured calls at widths where
+    launches and host preparation matter: per-call wall time (host work, the
+    launches and the copies back) of one 16-qubit circuit (400 rotations of
+    weight 2-8, 200 terms: an optimiser loop's step) and of a batch of 16
+    circuits of 12-16 qubits (100 rotations, 50 terms each), at new angles
+    every call, results compared bit for bit."""
+    import numpy as np
+
+    def batch(widths, n_rot, n_terms, s0):
+        pcs, hams = [], []
+        for i, n in enumerate(widths):
+            rots, _ = W.random_pauli_ansatz(n, n_rot, s0 + i)
+            pc = lq.ParametricCircuit(n)
+            for p, qs in rots:
+                pc.rp(p, qs)
+            pcs.append(pc)
+            hams.append(W.random_hamiltonian(n, n_terms, s0 + 1000 + i))
+        return pcs, hams
+
+    rng = np.random.default_rng(seed)
+    cases = {"one_16q_400rot": batch([16], 400, 200, seed + 7),
+             "batch16_12to16q_100rot": batch([int(x) for x in rng.integers(12, 17, size=16)], 100, 50, seed + 70)}
+    out = {}
+    for name, (pcs, hams) in cases.items():
+        angles = [[[float(v) for v in rng.uniform(-np.pi, np.pi, pc.num_parameters())] for pc in pcs]
+                  for _ in range(reps + 1)]
+        t0 = time.perf_counter()
+        plan = lq.algorithms.AdjointPlan(hams, pcs)
+        lq.synchronize()
+        t_create = time.perf_counter() - t0
+        plan.run(angles[0])
+        lq.algorithms.adjoint_gradient_batched(hams, pcs, angles[0])
+        lq.synchronize()
+        t0 = time.perf_counter()
+        eager = [lq.algorithms.adjoint_gradient_batched(hams, pcs, a) for a in angles[1:]]
+        t_eager = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        captured = [plan.run(a) for a in angles[1:]]
+        t_plan = (time.perf_counter() - t0) / reps
+        same = all(eg == ew and np.array_equal(np.array(gg), np.array(gw))
+                   for rg, rw in zip(captured, eager) for (eg, gg), (ew, gw) in zip(rg, rw))
+        out[name] = {"widths": [pc.num_qubits() for pc in pcs], "calls": reps,
+                     "uncaptured_ms_per_call": round(t_eager * 1e3, 3),
+                     "captured_ms_per_call": round(t_plan * 1e3, 3),
+                     "speedup": round(t_eager / t_plan, 2), "plan_create_ms": round(t_create * 1e3, 2),
+                     "bitwise_equal": bool(same)}
+        log(f"adjoint plan {name}: {t_eager * 1e3:.2f} -> {t_plan * 1e3:.2f} ms/call, equal={same}")
+        del plan
+    return out
+
+
 def _ps_cpu_baseline(W, sizes, seed: int, budget_s: float = 30.0) -> dict | None:
     """The reference's parameter_shift_gradient (vqe.cpp:68-94, OpenMP on the
     host cores) for the batch's first 16-qubit circuit: 2 x 400 evaluations.
@@ -667,6 +719,7 @@ def main() -> int:
     del sv
     if not args.no_adjoint:
         adjoint = adjoint_bench(lq, W, args.seed, args.adjoint_circuits, args.ps_seconds, cpu=not args.no_cpu_baseline)
+        adjoint["captured_plans"] = adjoint_plan_bench(lq, W, args.seed)
     if not args.no_trajectories:
         trajectories = trajectories_bench(lq, W, args.seed, args.traj_qubits, args.traj_shots,
                                           cpu=not args.no_cpu_baseline)

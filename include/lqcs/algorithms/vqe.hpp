@@ -11,6 +11,8 @@
 #include "../circuit/parametric_circuit.hpp"
 #include "../circuit/quantum_circuit.hpp"
 
+struct lqcs_adjoint_plan;  // the engine's captured plan (lqcs_device.h)
+
 namespace lqcs::algorithms {
 
 struct PauliTerm {
@@ -77,5 +79,34 @@ struct GradientResult {
     std::vector<double> grad;
 };
 std::vector<GradientResult> adjoint_gradient_batched(std::span<const GradientJob> jobs);
+
+// Engine extension: the jobs of adjoint_gradient_batched with their circuits
+// and Hamiltonians fixed and their parameter values free, for optimiser
+// loops. Construction lowers every job once and captures the device launch
+// sequence in a graph (the jobs' params only fix the sizes); run() evaluates
+// every job at new values by replaying it, with the same results as
+// adjoint_gradient_batched. The ansatz and Hamiltonian objects are not used
+// after construction. Ansatze with multi-controlled or permutation gates are
+// refused (std::invalid_argument).
+class AdjointPlan {
+public:
+    explicit AdjointPlan(std::span<const GradientJob> jobs);
+    AdjointPlan(const Hamiltonian& hamiltonian, const ParametricCircuit& ansatz);
+    ~AdjointPlan();
+    AdjointPlan(AdjointPlan&& o) noexcept;
+    AdjointPlan& operator=(AdjointPlan&& o) noexcept;
+    AdjointPlan(const AdjointPlan&) = delete;
+    AdjointPlan& operator=(const AdjointPlan&) = delete;
+
+    std::size_t num_jobs() const { return param_off_.empty() ? 0 : param_off_.size() - 1; }
+    // params[c] holds job c's values.
+    std::vector<GradientResult> run(std::span<const std::vector<double>> params);
+    // One-job plans: the gradient at `params`, the energy into `energy`.
+    std::vector<double> gradient(std::span<const double> params, double* energy = nullptr);
+
+private:
+    ::lqcs_adjoint_plan* plan_ = nullptr;
+    std::vector<std::uint64_t> param_off_;
+};
 
 }  // namespace lqcs::algorithms

@@ -330,6 +330,30 @@ lqcs_status lqcs_adjoint_gradient_batched(uint32_t n_circuits, const uint32_t* n
                                           const uint64_t* term_off, double* energies,
                                           double* grads);
 
+/* Captured adjoint plan: the batch of lqcs_adjoint_gradient_batched (same
+ * argument layout) with its structure fixed and its angles free. Create
+ * builds and places every launch group once and captures the whole launch
+ * sequence in one HIP graph; each run writes theta[p] (indexed like
+ * param_op, n_theta = param_off[n_circuits]) into parameter p's rotation,
+ * replays the graph and writes energies and grads as the batched call does.
+ * The angles the programs hold for parameter ops are replaced at every run;
+ * other ops keep theirs. Ops of kind CONTROLLED_UNITARY or
+ * CONTROLLED_PERMUTATION are refused (INVALID_ARGUMENT). The plan holds its
+ * registers (the largest launch group's) until destroyed.
+ * No counterpart in the reference: it serves optimiser loops that evaluate
+ * the same ansatz batch at many angles (reference loop: vqe_gradient_descent,
+ * src/algorithms/vqe.cpp:96-133). */
+typedef struct lqcs_adjoint_plan lqcs_adjoint_plan;
+lqcs_status lqcs_adjoint_plan_create(uint32_t n_circuits, const uint32_t* n_qubits,
+                                     const lqcs_program* progs, const uint64_t* param_op,
+                                     const uint64_t* param_off, const uint64_t* x_masks,
+                                     const uint64_t* zy_masks, const uint32_t* n_ys,
+                                     const double* coeffs, const uint64_t* term_off,
+                                     lqcs_adjoint_plan** out);
+lqcs_status lqcs_adjoint_plan_run(lqcs_adjoint_plan* plan, const double* theta, uint64_t n_theta,
+                                  double* energies, double* grads);
+lqcs_status lqcs_adjoint_plan_destroy(lqcs_adjoint_plan* plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,6 +24,9 @@
 
 #include <cmath>
 #include <cstring>
+#include <memory>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace lqcs_dev {
@@ -410,6 +413,16 @@ __global__ void k_zero2(amp_t* __restrict__ base, const CircuitDesc* __restrict_
         psi[i] = make_double2(i == 0 ? 1.0 : 0.0, 0.0);
 }
 
+// Pair coefficients of exp(-i theta/2 P): c = cos(theta/2) and
+// yphase = -i sin(theta/2) i^ny.
+void rotation_coeffs(const DecodedOp& d, amp_t& c, amp_t& yphase) {
+    const double h = d.theta / 2, sn = std::sin(h);
+    static const amp_t kIPow[4] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
+    const amp_t ip = kIPow[d.n_y % 4];
+    c = make_double2(std::cos(h), 0.0);
+    yphase = make_double2(sn * ip.y, -sn * ip.x);  // (0, -sin h) * i^ny
+}
+
 PauliStep make_step(const DecodedOp& d, int64_t param) {
     PauliStep s;
     std::memset(&s, 0, sizeof s);
@@ -418,11 +431,7 @@ PauliStep make_step(const DecodedOp& d, int64_t param) {
     s.is_pauli = 1;
     s.x = d.x;
     s.zy = d.zy;
-    const double h = d.theta / 2, sn = std::sin(h);
-    static const amp_t kIPow[4] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
-    const amp_t ip = kIPow[d.n_y % 4];
-    s.c = make_double2(std::cos(h), 0.0);
-    s.yphase = make_double2(sn * ip.y, -sn * ip.x);  // (0, -sin h) * i^ny
+    rotation_coeffs(d, s.c, s.yphase);
     return s;
 }
 
@@ -478,9 +487,10 @@ struct Unit {
 // basis is kept in reduced row-echelon form, pivot = highest set bit, and
 // padded to D vectors with the highest free qubits so that every group
 // walks cosets of 2^D.
+// src receives the op index of every step it appends.
 void group_rotations(const Job& j, uint64_t param_base, uint32_t D, size_t first, size_t last,
-                     std::vector<Group>& groups, std::vector<GroupStep>& gsteps, std::vector<Unit>& units,
-                     uint32_t& n_groups) {
+                     std::vector<Group>& groups, std::vector<GroupStep>& gsteps, std::vector<uint32_t>& src,
+                     std::vector<Unit>& units, uint32_t& n_groups) {
     const uint32_t NE = 1u << D;
     size_t k = first;
     while (k < last) {
@@ -536,6 +546,7 @@ void group_rotations(const Job& j, uint64_t param_base, uint32_t D, size_t first
             for (uint32_t a = 0; a < NE; ++a)
                 st.sgn |= static_cast<uint32_t>(__builtin_popcountll(G.off[a] & ps.zy) & 1) << a;
             gsteps.push_back(st);
+            src.push_back(static_cast<uint32_t>(s));
         }
         units.push_back(Unit{true, n_groups++});
         groups.push_back(G);
@@ -547,7 +558,8 @@ void group_rotations(const Job& j, uint64_t param_base, uint32_t D, size_t first
 // rotations become groups of dimension D, every other op stays a unit of
 // its own.
 void schedule_job(const Job& j, uint64_t param_base, uint32_t D, std::vector<Group>& groups,
-                  std::vector<GroupStep>& gsteps, std::vector<Unit>& units, uint32_t& n_groups) {
+                  std::vector<GroupStep>& gsteps, std::vector<uint32_t>& src, std::vector<Unit>& units,
+                  uint32_t& n_groups) {
     n_groups = 0;
     size_t s = 0;
     while (s < j.ops.size()) {
@@ -558,7 +570,7 @@ void schedule_job(const Job& j, uint64_t param_base, uint32_t D, std::vector<Gro
         }
         size_t e = s;
         while (e < j.ops.size() && j.ops[e].kind == LQCS_OP_PAULI_ROTATION) ++e;
-        group_rotations(j, param_base, D, s, e, groups, gsteps, units, n_groups);
+        group_rotations(j, param_base, D, s, e, groups, gsteps, src, units, n_groups);
         s = e;
     }
 }
@@ -568,158 +580,227 @@ uint32_t blocks_for(uint32_t nq) {
     return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(kMaxBlocks, want)));
 }
 
-// Runs a group of jobs that share launches. When `general` is set the group
-// holds one job whose non-Pauli ops run through the gate-by-gate kernels.
-// Parameter partials are indexed within the group: job c's parameter p sits
-// at slot param_base[c] + p, so the partial buffers and the reduction launch
-// scale with the group's parameters, not the whole batch's. `gsum` receives
-// the summed (re, im) of every slot.
-void run_group(const std::vector<const Job*>& jobs, bool general, std::vector<double>& energies_out,
-               std::vector<double>& gsum, std::vector<uint64_t>& param_base) {
-    const uint32_t nc = static_cast<uint32_t>(jobs.size());
-    param_base.assign(nc, 0);
+// One sweep's fused schedule: each circuit's rotation groups and, for a
+// general job, its units in program order. src[i] names the (circuit, op)
+// step i was built from, so a new angle can be written into it.
+struct Sched {
+    std::vector<CircuitDesc> cd;
+    std::vector<Group> groups;
+    std::vector<GroupStep> steps;
+    std::vector<std::pair<uint32_t, uint32_t>> src;
+    std::vector<Unit> units;
+    uint32_t max_groups = 0;
+    const CircuitDesc* d_cd = nullptr;
+    const Group* d_groups = nullptr;
+    GroupStep* d_steps = nullptr;
+};
+
+// Everything one group of jobs that share launches needs. When `general` is
+// set the group holds one job whose non-Pauli ops run through the
+// gate-by-gate kernels. Parameter partials are indexed within the group: job
+// c's parameter p sits at slot param_base[c] + p, so the partial buffers and
+// the reduction launch scale with the group's parameters, not the whole
+// batch's. Built on the host (build_work), placed in device memory
+// (place_work), then launched (launch_work) with no allocation or copy in
+// between, so a launch sequence can be captured in a graph.
+struct Work {
+    std::vector<const Job*> jobs;
+    bool general = false;
+    std::vector<uint64_t> param_base;
     uint64_t n_slots = 0;
-    for (uint32_t c = 0; c < nc; ++c) {
-        param_base[c] = n_slots;
-        n_slots += jobs[c]->n_params;
-    }
-    std::vector<CircuitDesc> cd(nc);
-    std::vector<PauliStep> steps;
+    std::vector<CircuitDesc> cd;
+    std::vector<PauliStep> steps;  // the one-step kernels' table (widths below 3)
     std::vector<Term> terms;
-    uint64_t reg = 0;
+    uint64_t reg = 0;              // amplitudes: psi and lam of every circuit
     uint32_t blk = 0, max_ops = 0;
+    bool fused = false;
+    uint32_t Df = 3;
+    Sched sf, sb;
+    std::vector<uint32_t> ecount, gcount;
+    const CircuitDesc* d_cd = nullptr;
+    PauliStep* d_steps = nullptr;
+    const Term* d_terms = nullptr;
+    double *epart = nullptr, *gpart = nullptr, *d_e = nullptr, *d_g = nullptr;
+    const uint32_t *d_ecount = nullptr, *d_gcount = nullptr;
+};
+
+Work build_work(const std::vector<const Job*>& jobs, bool general) {
+    Work w;
+    w.jobs = jobs;
+    w.general = general;
+    const uint32_t nc = static_cast<uint32_t>(jobs.size());
+    w.param_base.assign(nc, 0);
+    for (uint32_t c = 0; c < nc; ++c) {
+        w.param_base[c] = w.n_slots;
+        w.n_slots += jobs[c]->n_params;
+    }
+    w.cd.resize(nc);
     // Fused rotation groups walk cosets of 2^D, so every width must be >= D.
     // The forward holds psi alone and takes D = 4 where the widths allow it;
     // the backward holds psi and lam and takes D = 3 (at D = 4 it spills).
     uint32_t min_nq = 64;
     for (const Job* j : jobs) min_nq = std::min(min_nq, j->nq);
-    const bool fused = min_nq >= 3;
-    const uint32_t Df = min_nq >= kMaxGroupDim ? kMaxGroupDim : 3, Db = 3;
+    w.fused = min_nq >= 3;
+    w.Df = min_nq >= kMaxGroupDim ? kMaxGroupDim : 3;
     for (uint32_t c = 0; c < nc; ++c) {
         const Job& j = *jobs[c];
-        cd[c].reg_off = reg;
-        cd[c].nq = j.nq;
-        cd[c].blocks = blocks_for(j.nq);
-        cd[c].blk_start = blk;
-        cd[c].n_ops = static_cast<uint32_t>(j.ops.size());
-        cd[c].op_off = steps.size();
-        cd[c].term_off = terms.size();
-        cd[c].n_terms = j.terms.size();
-        for (size_t k = 0; k < j.ops.size(); ++k) steps.push_back(make_step(j.ops[k], param_slot(j, k, param_base[c])));
-        cd[c].group_off = 0;
-        cd[c].n_groups = 0;
-        terms.insert(terms.end(), j.terms.begin(), j.terms.end());
-        reg += uint64_t{2} << j.nq;
-        blk += cd[c].blocks;
-        max_ops = std::max<uint32_t>(max_ops, cd[c].n_ops);
+        CircuitDesc& d = w.cd[c];
+        d.reg_off = w.reg;
+        d.nq = j.nq;
+        d.blocks = blocks_for(j.nq);
+        d.blk_start = w.blk;
+        d.n_ops = static_cast<uint32_t>(j.ops.size());
+        d.op_off = w.steps.size();
+        d.term_off = w.terms.size();
+        d.n_terms = j.terms.size();
+        for (size_t k = 0; k < j.ops.size(); ++k)
+            w.steps.push_back(make_step(j.ops[k], param_slot(j, k, w.param_base[c])));
+        d.group_off = 0;
+        d.n_groups = 0;
+        w.terms.insert(w.terms.end(), j.terms.begin(), j.terms.end());
+        w.reg += uint64_t{2} << j.nq;
+        w.blk += d.blocks;
+        w.max_ops = std::max<uint32_t>(w.max_ops, d.n_ops);
     }
-    amp_t* base = static_cast<amp_t*>(device_alloc(reg * sizeof(amp_t)));
+    auto build_sched = [&](Sched& S, uint32_t D) {
+        S.cd = w.cd;
+        for (uint32_t c = 0; c < nc; ++c) {
+            S.cd[c].group_off = static_cast<uint32_t>(S.groups.size());
+            std::vector<Unit> u;
+            std::vector<uint32_t> src;
+            schedule_job(*jobs[c], w.param_base[c], D, S.groups, S.steps, src, u, S.cd[c].n_groups);
+            for (uint32_t k : src) S.src.emplace_back(c, k);
+            if (c == 0) S.units.swap(u);
+            S.max_groups = std::max(S.max_groups, S.cd[c].n_groups);
+        }
+    };
+    if (w.fused) {
+        build_sched(w.sf, w.Df);
+        build_sched(w.sb, 3);
+    }
+    w.ecount.resize(nc);
+    for (uint32_t c = 0; c < nc; ++c) w.ecount[c] = w.cd[c].blocks;
+    w.gcount.assign(std::max<uint64_t>(1, w.n_slots), 0);
+    for (uint32_t c = 0; c < nc; ++c)
+        for (size_t k = 0; k < jobs[c]->ops.size(); ++k)
+            if (jobs[c]->param_of_op[k] >= 0) w.gcount[w.param_base[c] + jobs[c]->param_of_op[k]] = w.cd[c].blocks;
+    return w;
+}
+
+// Each put*(host, bytes) returns device memory of `bytes` holding a copy of
+// `host` (uninitialised when host is null): put_dyn receives the tables that
+// depend on the rotation angles, put_out the energies and parameter sums,
+// put everything else.
+template <class Put, class PutDyn, class PutOut>
+void place_work(Work& w, Put&& put, PutDyn&& put_dyn, PutOut&& put_out) {
+    auto tab = [](auto&& to, const auto& v) {
+        using T = typename std::decay_t<decltype(v)>::value_type;
+        return static_cast<T*>(to(static_cast<const void*>(v.data()), std::max<size_t>(1, v.size()) * sizeof(T)));
+    };
+    const uint32_t nc = static_cast<uint32_t>(w.jobs.size());
+    w.d_cd = tab(put, w.cd);
+    w.d_steps = tab(put_dyn, w.steps);
+    w.d_terms = tab(put, w.terms);
+    w.epart = static_cast<double*>(put(nullptr, sizeof(double) * 2 * kMaxBlocks * nc));
+    w.gpart = static_cast<double*>(put(nullptr, sizeof(double) * 2 * kMaxBlocks * std::max<uint64_t>(1, w.n_slots)));
+    if (w.fused) {
+        for (Sched* S : {&w.sf, &w.sb}) {
+            S->d_cd = tab(put, S->cd);
+            S->d_groups = tab(put, S->groups);
+            S->d_steps = tab(put_dyn, S->steps);
+        }
+    }
+    w.d_ecount = tab(put, w.ecount);
+    w.d_gcount = tab(put, w.gcount);
+    w.d_e = static_cast<double*>(put_out(nullptr, sizeof(double) * 2 * nc));
+    w.d_g = static_cast<double*>(put_out(nullptr, sizeof(double) * 2 * w.gcount.size()));
+}
+
+// The group's whole launch sequence on the engine stream: kernel launches
+// only, registers at `base` (w.reg amplitudes).
+void launch_work(const Work& w, amp_t* base) {
+    const uint32_t nc = static_cast<uint32_t>(w.jobs.size()), blk = w.blk;
+    const Job& j0 = *w.jobs[0];
+    auto fwd_group = [&](uint32_t g) {
+        const Sched& S = w.sf;
+        if (w.Df == 4) k_fwd_group<4><<<blk, kBlock, 0, stream()>>>(base, S.d_cd, nc, S.d_groups, S.d_steps, g);
+        else k_fwd_group<3><<<blk, kBlock, 0, stream()>>>(base, S.d_cd, nc, S.d_groups, S.d_steps, g);
+        LQCS_HIP(hipGetLastError());
+    };
+    auto bwd_group = [&](uint32_t g) {
+        const Sched& S = w.sb;
+        k_bwd_group<3><<<blk, kBlock, 0, stream()>>>(base, S.d_cd, nc, S.d_groups, S.d_steps, g, w.gpart);
+        LQCS_HIP(hipGetLastError());
+    };
+
+    k_zero2<<<blk, kBlock, 0, stream()>>>(base, w.d_cd, nc);
+    // forward
+    if (w.fused && w.general) {
+        for (const Unit& u : w.sf.units) {
+            if (u.pauli) fwd_group(u.index);
+            else run_single_op(base, j0.nq, j0.ops[u.index], 1);
+        }
+    } else if (w.fused) {
+        for (uint32_t g = 0; g < w.sf.max_groups; ++g) fwd_group(g);
+    }
+    for (uint32_t s = 0; !w.fused && s < w.max_ops; ++s) {
+        if (w.general && j0.ops[s].kind != LQCS_OP_PAULI_ROTATION) {
+            run_single_op(base, j0.nq, j0.ops[s], 1);
+            continue;
+        }
+        k_fwd<<<blk, kBlock, 0, stream()>>>(base, w.d_cd, nc, w.d_steps, s);
+    }
+    k_hamiltonian<<<blk, kBlock, 0, stream()>>>(base, w.d_cd, nc, w.d_terms, w.epart);
+    // backward
+    if (w.fused && w.general) {
+        for (size_t k = w.sb.units.size(); k-- > 0;) {
+            const Unit& u = w.sb.units[k];
+            if (u.pauli) bwd_group(u.index);
+            else run_single_op(base, j0.nq, inverse_op(j0.ops[u.index]), 2);
+        }
+    } else if (w.fused) {
+        for (uint32_t g = w.sb.max_groups; g-- > 0;) bwd_group(g);
+    }
+    for (uint32_t s = w.max_ops; !w.fused && s-- > 0;) {
+        if (w.general && j0.ops[s].kind != LQCS_OP_PAULI_ROTATION) {
+            run_single_op(base, j0.nq, inverse_op(j0.ops[s]), 2);
+            continue;
+        }
+        k_bwd<<<blk, kBlock, 0, stream()>>>(base, w.d_cd, nc, w.d_steps, s, w.gpart);
+    }
+    // per-circuit energies and per-parameter sums
+    k_rows<<<nc, kBlock, 0, stream()>>>(w.epart, w.d_ecount, w.d_e);
+    k_rows<<<static_cast<unsigned>(w.gcount.size()), kBlock, 0, stream()>>>(w.gpart, w.d_gcount, w.d_g);
+    LQCS_HIP(hipGetLastError());
+}
+
+// Runs one group now: arena-staged tables, a register buffer for this call.
+// energies_out[c] = <H> of job c; gsum holds the summed (re, im) of every
+// parameter slot; param_base[c] is job c's first slot.
+void run_group(const std::vector<const Job*>& jobs, bool general, std::vector<double>& energies_out,
+               std::vector<double>& gsum, std::vector<uint64_t>& param_base) {
+    Work w = build_work(jobs, general);
+    auto put = [](const void* host, size_t bytes) {
+        void* d = arena(bytes);
+        if (host) LQCS_HIP(hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, stream()));
+        return d;
+    };
+    place_work(w, put, put, put);
+    amp_t* base = static_cast<amp_t*>(device_alloc(w.reg * sizeof(amp_t)));
     struct Free {
         void* p;
         ~Free() { device_free(p); }
     } guard{base};
-    const CircuitDesc* d_cd = stage(cd.data(), cd.size());
-    const PauliStep* d_steps = stage(steps.data(), std::max<size_t>(1, steps.size()));
-    const Term* d_terms = stage(terms.data(), std::max<size_t>(1, terms.size()));
-    double* epart = static_cast<double*>(arena(sizeof(double) * 2 * kMaxBlocks * nc));
-    double* gpart = static_cast<double*>(arena(sizeof(double) * 2 * kMaxBlocks * std::max<uint64_t>(1, n_slots)));
-
-    // One fused schedule per sweep: each circuit's groups and, for jobs[0]
-    // when `general`, its units in program order.
-    struct Sched {
-        std::vector<CircuitDesc> cd;
-        std::vector<Group> groups;
-        std::vector<GroupStep> steps;
-        std::vector<Unit> units;
-        uint32_t max_groups = 0;
-        const CircuitDesc* d_cd = nullptr;
-        const Group* d_groups = nullptr;
-        const GroupStep* d_steps = nullptr;
-    } sf, sb;
-    auto build_sched = [&](Sched& S, uint32_t D) {
-        S.cd = cd;
-        for (uint32_t c = 0; c < nc; ++c) {
-            S.cd[c].group_off = static_cast<uint32_t>(S.groups.size());
-            std::vector<Unit> u;
-            schedule_job(*jobs[c], param_base[c], D, S.groups, S.steps, u, S.cd[c].n_groups);
-            if (c == 0) S.units.swap(u);
-            S.max_groups = std::max(S.max_groups, S.cd[c].n_groups);
-        }
-        S.d_cd = stage(S.cd.data(), S.cd.size());
-        S.d_groups = stage(S.groups.data(), std::max<size_t>(1, S.groups.size()));
-        S.d_steps = stage(S.steps.data(), std::max<size_t>(1, S.steps.size()));
-    };
-    if (fused) {
-        build_sched(sf, Df);
-        build_sched(sb, Db);
-    }
-    auto fwd_group = [&](uint32_t g) {
-        if (Df == 4) k_fwd_group<4><<<blk, kBlock, 0, stream()>>>(base, sf.d_cd, nc, sf.d_groups, sf.d_steps, g);
-        else k_fwd_group<3><<<blk, kBlock, 0, stream()>>>(base, sf.d_cd, nc, sf.d_groups, sf.d_steps, g);
-    };
-    auto bwd_group = [&](uint32_t g) {
-        k_bwd_group<3><<<blk, kBlock, 0, stream()>>>(base, sb.d_cd, nc, sb.d_groups, sb.d_steps, g, gpart);
-    };
-
-    k_zero2<<<blk, kBlock, 0, stream()>>>(base, d_cd, nc);
-    // forward
-    if (fused && general) {
-        for (const Unit& u : sf.units) {
-            if (u.pauli) fwd_group(u.index);
-            else run_single_op(base, jobs[0]->nq, jobs[0]->ops[u.index], 1);
-        }
-    } else if (fused) {
-        for (uint32_t g = 0; g < sf.max_groups; ++g) fwd_group(g);
-    }
-    for (uint32_t s = 0; !fused && s < max_ops; ++s) {
-        if (general) {
-            const DecodedOp& op = jobs[0]->ops[s];
-            if (op.kind != LQCS_OP_PAULI_ROTATION) {
-                run_single_op(base, jobs[0]->nq, op, 1);
-                continue;
-            }
-        }
-        k_fwd<<<blk, kBlock, 0, stream()>>>(base, d_cd, nc, d_steps, s);
-    }
-    k_hamiltonian<<<blk, kBlock, 0, stream()>>>(base, d_cd, nc, d_terms, epart);
-    // backward
-    if (fused && general) {
-        for (size_t k = sb.units.size(); k-- > 0;) {
-            const Unit& u = sb.units[k];
-            if (u.pauli) bwd_group(u.index);
-            else run_single_op(base, jobs[0]->nq, inverse_op(jobs[0]->ops[u.index]), 2);
-        }
-    } else if (fused) {
-        for (uint32_t g = sb.max_groups; g-- > 0;) bwd_group(g);
-    }
-    for (uint32_t s = max_ops; !fused && s-- > 0;) {
-        if (general) {
-            const DecodedOp& op = jobs[0]->ops[s];
-            if (op.kind != LQCS_OP_PAULI_ROTATION) {
-                run_single_op(base, jobs[0]->nq, inverse_op(op), 2);
-                continue;
-            }
-        }
-        k_bwd<<<blk, kBlock, 0, stream()>>>(base, d_cd, nc, d_steps, s, gpart);
-    }
-    // per-circuit energies and per-parameter sums
-    std::vector<uint32_t> ecount(nc);
-    for (uint32_t c = 0; c < nc; ++c) ecount[c] = cd[c].blocks;
-    double* d_e = static_cast<double*>(arena(sizeof(double) * 2 * nc));
-    k_rows<<<nc, kBlock, 0, stream()>>>(epart, stage(ecount.data(), nc), d_e);
-    std::vector<uint32_t> gcount(std::max<uint64_t>(1, n_slots), 0);
-    for (uint32_t c = 0; c < nc; ++c)
-        for (size_t k = 0; k < jobs[c]->ops.size(); ++k)
-            if (jobs[c]->param_of_op[k] >= 0) gcount[param_base[c] + jobs[c]->param_of_op[k]] = cd[c].blocks;
-    double* d_g = static_cast<double*>(arena(sizeof(double) * 2 * gcount.size()));
-    k_rows<<<static_cast<unsigned>(gcount.size()), kBlock, 0, stream()>>>(gpart, stage(gcount.data(), gcount.size()), d_g);
+    launch_work(w, base);
+    const uint32_t nc = static_cast<uint32_t>(jobs.size());
     std::vector<double> e(2 * nc);
-    LQCS_HIP(hipMemcpyAsync(e.data(), d_e, sizeof(double) * 2 * nc, hipMemcpyDeviceToHost, stream()));
-    gsum.assign(2 * gcount.size(), 0.0);
-    LQCS_HIP(hipMemcpyAsync(gsum.data(), d_g, sizeof(double) * gsum.size(), hipMemcpyDeviceToHost, stream()));
+    LQCS_HIP(hipMemcpyAsync(e.data(), w.d_e, sizeof(double) * 2 * nc, hipMemcpyDeviceToHost, stream()));
+    gsum.assign(2 * w.gcount.size(), 0.0);
+    LQCS_HIP(hipMemcpyAsync(gsum.data(), w.d_g, sizeof(double) * gsum.size(), hipMemcpyDeviceToHost, stream()));
     sync();
     energies_out.resize(nc);
     for (uint32_t c = 0; c < nc; ++c) energies_out[c] = e[2 * c];
+    param_base = w.param_base;
 }
 
 Job build_job(uint32_t nq, const lqcs_program& prog, const uint64_t* param_op, uint64_t n_params,
@@ -751,7 +832,7 @@ Job build_job(uint32_t nq, const lqcs_program& prog, const uint64_t* param_op, u
 }
 
 // grad_p = Im(i^ny_p * S_p) from the accumulated S_p at its slot.
-double grad_from(const Job& j, uint64_t slot, uint64_t op_index, const std::vector<double>& g) {
+double grad_from(const Job& j, uint64_t slot, uint64_t op_index, const double* g) {
     const double sr = g[2 * slot], si = g[2 * slot + 1];
     switch (j.ops[op_index].n_y % 4) {
         case 0: return si;
@@ -759,6 +840,72 @@ double grad_from(const Job& j, uint64_t slot, uint64_t op_index, const std::vect
         case 2: return -si;
         default: return -sr;
     }
+}
+
+// Argument checks shared by the batched entry points.
+void check_batch(uint32_t n_circuits, const uint32_t* n_qubits, const lqcs_program* progs, const uint64_t* param_op,
+                 const uint64_t* param_off, const uint64_t* x_masks, const uint64_t* zy_masks, const uint32_t* n_ys,
+                 const double* coeffs, const uint64_t* term_off, bool need_grads) {
+    if (!n_qubits || !progs || !param_off || !term_off) throw Error(LQCS_ERR_INVALID_ARGUMENT, "null argument");
+    for (uint32_t c = 0; c < n_circuits; ++c)
+        if (param_off[c + 1] < param_off[c] || term_off[c + 1] < term_off[c])
+            throw Error(LQCS_ERR_INVALID_ARGUMENT, "offsets must be non-decreasing");
+    if (param_off[n_circuits] > param_off[0] && (!param_op || !need_grads))
+        throw Error(LQCS_ERR_INVALID_ARGUMENT, "null argument");
+    if (term_off[n_circuits] > term_off[0] && (!x_masks || !zy_masks || !n_ys || !coeffs))
+        throw Error(LQCS_ERR_INVALID_ARGUMENT, "null argument");
+}
+
+std::vector<Job> build_jobs(uint32_t n_circuits, const uint32_t* n_qubits, const lqcs_program* progs,
+                            const uint64_t* param_op, const uint64_t* param_off, const uint64_t* x_masks,
+                            const uint64_t* zy_masks, const uint32_t* n_ys, const double* coeffs,
+                            const uint64_t* term_off) {
+    std::vector<Job> jobs;
+    jobs.reserve(n_circuits);
+    for (uint32_t c = 0; c < n_circuits; ++c) {
+        const uint64_t p0 = param_off[c], p1 = param_off[c + 1];
+        const uint64_t t0 = term_off[c], t1 = term_off[c + 1];
+        jobs.push_back(build_job(n_qubits[c], progs[c], param_op + p0, p1 - p0, x_masks + t0, zy_masks + t0,
+                                 n_ys + t0, coeffs + t0, t1 - t0));
+    }
+    return jobs;
+}
+
+// The batch's launch groups, in circuit order: Pauli-only circuits share
+// launches in groups whose registers fit half the free HBM (at most 4096
+// circuits each); any other circuit is a general group of its own.
+struct LaunchGroup {
+    std::vector<uint32_t> ids;
+    bool general;
+};
+
+std::vector<LaunchGroup> launch_groups(const std::vector<Job>& jobs) {
+    size_t free_b = 0, total_b = 0;
+    LQCS_HIP(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t budget = free_b / 2;
+    std::vector<LaunchGroup> out;
+    LaunchGroup cur{{}, false};
+    uint64_t bytes = 0;
+    auto flush = [&] {
+        if (cur.ids.empty()) return;
+        out.push_back(std::move(cur));
+        cur = LaunchGroup{{}, false};
+        bytes = 0;
+    };
+    for (uint32_t c = 0; c < jobs.size(); ++c) {
+        const Job& j = jobs[c];
+        if (!j.all_pauli) {
+            flush();
+            out.push_back(LaunchGroup{{c}, true});
+            continue;
+        }
+        const uint64_t need = (uint64_t{2} << j.nq) * sizeof(amp_t);
+        if (!cur.ids.empty() && (bytes + need > budget || cur.ids.size() >= 4096)) flush();
+        cur.ids.push_back(c);
+        bytes += need;
+    }
+    flush();
+    return out;
 }
 
 }  // namespace
@@ -781,7 +928,7 @@ extern "C" lqcs_status lqcs_adjoint_gradient(uint32_t nq, const lqcs_program* pr
         std::vector<uint64_t> pb;
         run_group({&j}, !j.all_pauli, e, g, pb);
         if (energy) *energy = e[0];
-        for (uint64_t p = 0; p < n_params; ++p) grad[p] = grad_from(j, p, param_op[p], g);
+        for (uint64_t p = 0; p < n_params; ++p) grad[p] = grad_from(j, p, param_op[p], g.data());
     });
 }
 
@@ -793,66 +940,218 @@ extern "C" lqcs_status lqcs_adjoint_gradient_batched(uint32_t n_circuits, const 
                                                      double* energies, double* grads) {
     return run_guarded([&] {
         if (n_circuits == 0) return;
-        if (!n_qubits || !progs || !param_off || !term_off) throw Error(LQCS_ERR_INVALID_ARGUMENT, "null argument");
-        for (uint32_t c = 0; c < n_circuits; ++c)
-            if (param_off[c + 1] < param_off[c] || term_off[c + 1] < term_off[c])
-                throw Error(LQCS_ERR_INVALID_ARGUMENT, "offsets must be non-decreasing");
-        if (param_off[n_circuits] > param_off[0] && (!param_op || !grads))
-            throw Error(LQCS_ERR_INVALID_ARGUMENT, "null argument");
-        if (term_off[n_circuits] > term_off[0] && (!x_masks || !zy_masks || !n_ys || !coeffs))
-            throw Error(LQCS_ERR_INVALID_ARGUMENT, "null argument");
+        check_batch(n_circuits, n_qubits, progs, param_op, param_off, x_masks, zy_masks, n_ys, coeffs, term_off,
+                    grads != nullptr);
         ensure_ready();
         arena_reset();
-        std::vector<Job> jobs;
-        jobs.reserve(n_circuits);
-        for (uint32_t c = 0; c < n_circuits; ++c) {
-            const uint64_t p0 = param_off[c], p1 = param_off[c + 1];
-            const uint64_t t0 = term_off[c], t1 = term_off[c + 1];
-            jobs.push_back(build_job(n_qubits[c], progs[c], param_op + p0, p1 - p0, x_masks + t0, zy_masks + t0,
-                                     n_ys + t0, coeffs + t0, t1 - t0));
-        }
-        // Pauli-only circuits share launches in groups bounded by HBM use.
-        size_t free_b = 0, total_b = 0;
-        LQCS_HIP(hipMemGetInfo(&free_b, &total_b));
-        const uint64_t budget = free_b / 2;
-        std::vector<const Job*> group;
-        std::vector<uint32_t> group_ids;
-        uint64_t bytes = 0;
-        auto flush = [&] {
-            if (group.empty()) return;
+        const std::vector<Job> jobs =
+            build_jobs(n_circuits, n_qubits, progs, param_op, param_off, x_masks, zy_masks, n_ys, coeffs, term_off);
+        for (const LaunchGroup& lg : launch_groups(jobs)) {
+            std::vector<const Job*> group;
+            for (uint32_t c : lg.ids) group.push_back(&jobs[c]);
             std::vector<double> e, g;
             std::vector<uint64_t> pb;
-            run_group(group, false, e, g, pb);
-            for (size_t i = 0; i < group.size(); ++i) {
-                const uint32_t c = group_ids[i];
+            run_group(group, lg.general, e, g, pb);
+            for (size_t i = 0; i < lg.ids.size(); ++i) {
+                const uint32_t c = lg.ids[i];
                 if (energies) energies[c] = e[i];
                 for (uint64_t p = param_off[c]; p < param_off[c + 1]; ++p)
-                    grads[p] = grad_from(jobs[c], pb[i] + (p - param_off[c]), param_op[p], g);
+                    grads[p] = grad_from(jobs[c], pb[i] + (p - param_off[c]), param_op[p], g.data());
             }
-            group.clear();
-            group_ids.clear();
-            bytes = 0;
             arena_reset();
-        };
-        for (uint32_t c = 0; c < n_circuits; ++c) {
-            const Job& j = jobs[c];
-            if (!j.all_pauli) {
-                flush();
-                std::vector<double> e, g;
-                std::vector<uint64_t> pb;
-                run_group({&j}, true, e, g, pb);
-                if (energies) energies[c] = e[0];
-                for (uint64_t p = param_off[c]; p < param_off[c + 1]; ++p)
-                    grads[p] = grad_from(j, p - param_off[c], param_op[p], g);
-                arena_reset();
-                continue;
-            }
-            const uint64_t need = (uint64_t{2} << j.nq) * sizeof(amp_t);
-            if (!group.empty() && (bytes + need > budget || group.size() >= 4096)) flush();
-            group.push_back(&j);
-            group_ids.push_back(c);
-            bytes += need;
         }
-        flush();
+    });
+}
+
+// ---- captured plans ----------------------------------------------------------
+// A batch whose circuits, parameter positions and Hamiltonians stay fixed
+// while the angles change (an optimiser loop): the launch groups are built
+// and placed in device memory once, their whole launch sequence is captured
+// in one HIP graph, and each run writes the new angles into the rotation
+// tables (one host-to-device copy), replays the graph and reads every
+// group's energies and parameter sums back (one copy).
+struct lqcs_adjoint_plan {
+    std::vector<Job> jobs;
+    std::vector<uint64_t> param_op, param_off;
+    std::vector<LaunchGroup> groups;
+    std::vector<Work> works;
+    void* buf = nullptr;        // tables and partial sums of every group
+    amp_t* regs = nullptr;      // registers, used by the groups in turn
+    char* dyn_host = nullptr;   // pinned mirror of the angle-dependent tables
+    char* dyn_dev = nullptr;
+    size_t dyn_bytes = 0;
+    double* out_host = nullptr; // pinned copy of every group's energies and sums
+    double* out_dev = nullptr;
+    size_t out_bytes = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    ~lqcs_adjoint_plan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (dyn_host) (void)hipHostFree(dyn_host);
+        if (out_host) (void)hipHostFree(out_host);
+        device_free(buf);
+        device_free(dyn_dev);
+        device_free(out_dev);
+        device_free(regs);
+    }
+};
+
+namespace lqcs_dev {
+namespace {
+
+// Writes job angles into the mirror of every angle-dependent table.
+void refresh_angles(lqcs_adjoint_plan& P) {
+    for (Work& w : P.works) {
+        for (uint32_t c = 0; c < w.jobs.size(); ++c) {
+            const Job& j = *w.jobs[c];
+            auto* st = reinterpret_cast<PauliStep*>(P.dyn_host + (reinterpret_cast<char*>(w.d_steps) - P.dyn_dev));
+            for (size_t k = 0; k < j.ops.size(); ++k)
+                if (j.param_of_op[k] >= 0) rotation_coeffs(j.ops[k], st[w.cd[c].op_off + k].c, st[w.cd[c].op_off + k].yphase);
+        }
+        if (!w.fused) continue;
+        for (Sched* S : {&w.sf, &w.sb}) {
+            auto* st = reinterpret_cast<GroupStep*>(P.dyn_host + (reinterpret_cast<char*>(S->d_steps) - P.dyn_dev));
+            for (size_t i = 0; i < S->src.size(); ++i) {
+                const Job& j = *w.jobs[S->src[i].first];
+                const uint32_t k = S->src[i].second;
+                if (j.param_of_op[k] >= 0) rotation_coeffs(j.ops[k], st[i].c, st[i].y);
+            }
+        }
+    }
+}
+
+}  // namespace
+}  // namespace lqcs_dev
+
+extern "C" lqcs_status lqcs_adjoint_plan_create(uint32_t n_circuits, const uint32_t* n_qubits,
+                                                const lqcs_program* progs, const uint64_t* param_op,
+                                                const uint64_t* param_off, const uint64_t* x_masks,
+                                                const uint64_t* zy_masks, const uint32_t* n_ys,
+                                                const double* coeffs, const uint64_t* term_off,
+                                                lqcs_adjoint_plan** out) {
+    return run_guarded([&] {
+        if (!out) throw Error(LQCS_ERR_INVALID_ARGUMENT, "out is null");
+        *out = nullptr;
+        if (n_circuits == 0) throw Error(LQCS_ERR_INVALID_ARGUMENT, "a plan needs at least one circuit");
+        check_batch(n_circuits, n_qubits, progs, param_op, param_off, x_masks, zy_masks, n_ys, coeffs, term_off, true);
+        ensure_ready();
+        arena_reset();
+        auto P = std::make_unique<lqcs_adjoint_plan>();
+        P->jobs = build_jobs(n_circuits, n_qubits, progs, param_op, param_off, x_masks, zy_masks, n_ys, coeffs,
+                             term_off);
+        for (const Job& j : P->jobs)
+            for (const DecodedOp& o : j.ops)
+                if (o.kind == LQCS_OP_CONTROLLED_UNITARY || o.kind == LQCS_OP_CONTROLLED_PERMUTATION)
+                    throw Error(LQCS_ERR_INVALID_ARGUMENT,
+                                "adjoint plan: controlled-unitary and permutation ops stage tables per launch");
+        if (param_off[n_circuits]) P->param_op.assign(param_op, param_op + param_off[n_circuits]);
+        P->param_off.assign(param_off, param_off + n_circuits + 1);
+        P->groups = launch_groups(P->jobs);
+        uint64_t max_reg = 0;
+        for (const LaunchGroup& lg : P->groups) {
+            std::vector<const Job*> group;
+            for (uint32_t c : lg.ids) group.push_back(&P->jobs[c]);
+            P->works.push_back(build_work(group, lg.general));
+            max_reg = std::max(max_reg, P->works.back().reg);
+        }
+        // Sizes first (offsets from null), then the real placement.
+        auto align = [](size_t b) { return (b + 255) & ~size_t{255}; };
+        size_t n_buf = 0, n_dyn = 0, n_out = 0;
+        auto count = [&](size_t& n) {
+            return [&n, &align](const void*, size_t bytes) {
+                n += align(bytes);
+                return static_cast<void*>(nullptr);
+            };
+        };
+        for (Work& w : P->works) place_work(w, count(n_buf), count(n_dyn), count(n_out));
+        P->buf = device_alloc(std::max<size_t>(n_buf, 256));
+        P->dyn_dev = static_cast<char*>(device_alloc(std::max<size_t>(n_dyn, 256)));
+        P->out_dev = static_cast<double*>(device_alloc(std::max<size_t>(n_out, 256)));
+        P->regs = static_cast<amp_t*>(device_alloc(max_reg * sizeof(amp_t)));
+        LQCS_HIP(hipHostMalloc(reinterpret_cast<void**>(&P->dyn_host), std::max<size_t>(n_dyn, 256)));
+        LQCS_HIP(hipHostMalloc(reinterpret_cast<void**>(&P->out_host), std::max<size_t>(n_out, 256)));
+        P->dyn_bytes = n_dyn;
+        P->out_bytes = n_out;
+        std::vector<char> tables(n_buf);
+        size_t o_buf = 0, o_dyn = 0, o_out = 0;
+        auto put = [&](const void* host, size_t bytes) {
+            if (host) std::memcpy(tables.data() + o_buf, host, bytes);
+            void* d = static_cast<char*>(P->buf) + o_buf;
+            o_buf += align(bytes);
+            return d;
+        };
+        auto put_dyn = [&](const void* host, size_t bytes) {
+            if (host) std::memcpy(P->dyn_host + o_dyn, host, bytes);
+            void* d = P->dyn_dev + o_dyn;
+            o_dyn += align(bytes);
+            return d;
+        };
+        auto put_out = [&](const void*, size_t bytes) {
+            void* d = reinterpret_cast<char*>(P->out_dev) + o_out;
+            o_out += align(bytes);
+            return d;
+        };
+        for (Work& w : P->works) place_work(w, put, put_dyn, put_out);
+        lqcs_dev::sync();
+        if (n_buf) LQCS_HIP(hipMemcpy(P->buf, tables.data(), n_buf, hipMemcpyHostToDevice));
+        if (n_dyn) LQCS_HIP(hipMemcpy(P->dyn_dev, P->dyn_host, n_dyn, hipMemcpyHostToDevice));
+        LQCS_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+        try {
+            for (const Work& w : P->works) launch_work(w, P->regs);
+        } catch (...) {
+            hipGraph_t dead = nullptr;
+            (void)hipStreamEndCapture(stream(), &dead);
+            if (dead) (void)hipGraphDestroy(dead);
+            throw;
+        }
+        LQCS_HIP(hipStreamEndCapture(stream(), &P->graph));
+        LQCS_HIP(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+        *out = P.release();
+    });
+}
+
+extern "C" lqcs_status lqcs_adjoint_plan_run(lqcs_adjoint_plan* plan, const double* theta, uint64_t n_theta,
+                                             double* energies, double* grads) {
+    return run_guarded([&] {
+        if (!plan) throw Error(LQCS_ERR_INVALID_ARGUMENT, "plan is null");
+        lqcs_adjoint_plan& P = *plan;
+        const uint32_t nc = static_cast<uint32_t>(P.jobs.size());
+        const uint64_t n_p = P.param_off[nc];
+        if (n_theta != n_p)
+            throw Error(LQCS_ERR_INVALID_ARGUMENT, "adjoint plan: expected " + std::to_string(n_p) + " angles");
+        if (n_p > P.param_off[0] && (!theta || !grads)) throw Error(LQCS_ERR_INVALID_ARGUMENT, "null argument");
+        ensure_ready();
+        for (uint32_t c = 0; c < nc; ++c)
+            for (uint64_t p = P.param_off[c]; p < P.param_off[c + 1]; ++p)
+                P.jobs[c].ops[P.param_op[p]].theta = theta[p];
+        refresh_angles(P);
+        if (P.dyn_bytes)
+            LQCS_HIP(hipMemcpyAsync(P.dyn_dev, P.dyn_host, P.dyn_bytes, hipMemcpyHostToDevice, stream()));
+        LQCS_HIP(hipGraphLaunch(P.exec, stream()));
+        LQCS_HIP(hipMemcpyAsync(P.out_host, P.out_dev, P.out_bytes, hipMemcpyDeviceToHost, stream()));
+        lqcs_dev::sync();
+        for (size_t gi = 0; gi < P.works.size(); ++gi) {
+            const Work& w = P.works[gi];
+            const auto off = [&](const double* d) { return P.out_host + (d - P.out_dev); };
+            const double* e = off(w.d_e);
+            const double* g = off(w.d_g);
+            const std::vector<uint32_t>& ids = P.groups[gi].ids;
+            for (size_t i = 0; i < ids.size(); ++i) {
+                const uint32_t c = ids[i];
+                if (energies) energies[c] = e[2 * i];
+                for (uint64_t p = P.param_off[c]; p < P.param_off[c + 1]; ++p)
+                    grads[p] = grad_from(P.jobs[c], w.param_base[i] + (p - P.param_off[c]), P.param_op[p], g);
+            }
+        }
+    });
+}
+
+extern "C" lqcs_status lqcs_adjoint_plan_destroy(lqcs_adjoint_plan* plan) {
+    return run_guarded([&] {
+        if (!plan) return;
+        lqcs_dev::sync();
+        delete plan;
     });
 }

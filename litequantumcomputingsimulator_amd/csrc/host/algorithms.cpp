@@ -3,6 +3,7 @@
 // on the engine.
 #include <cmath>
 #include <numbers>
+#include <optional>
 #include <stdexcept>
 
 #include "lowering.hpp"
@@ -141,6 +142,86 @@ std::vector<GradientResult> adjoint_gradient_batched(std::span<const GradientJob
     return out;
 }
 
+AdjointPlan::AdjointPlan(std::span<const GradientJob> jobs) {
+    if (jobs.empty()) throw std::invalid_argument("AdjointPlan: no jobs");
+    std::vector<Lowered> L(jobs.size());
+    std::vector<uint32_t> nq;
+    std::vector<lqcs_program> progs;
+    std::vector<uint64_t> param_op, xs, zs, term_off{0};
+    std::vector<uint32_t> ys;
+    std::vector<double> cs;
+    param_off_.assign(1, 0);
+    for (std::size_t c = 0; c < jobs.size(); ++c) {
+        lower(*jobs[c].ansatz, *jobs[c].hamiltonian, jobs[c].params, L[c]);
+        nq.push_back(static_cast<uint32_t>(jobs[c].ansatz->num_qubits()));
+        param_op.insert(param_op.end(), L[c].param_op.begin(), L[c].param_op.end());
+        param_off_.push_back(param_op.size());
+        xs.insert(xs.end(), L[c].xs.begin(), L[c].xs.end());
+        zs.insert(zs.end(), L[c].zs.begin(), L[c].zs.end());
+        ys.insert(ys.end(), L[c].ys.begin(), L[c].ys.end());
+        cs.insert(cs.end(), L[c].cs.begin(), L[c].cs.end());
+        term_off.push_back(cs.size());
+    }
+    for (auto& l : L) progs.push_back(l.prog.program());
+    check(lqcs_adjoint_plan_create(static_cast<uint32_t>(jobs.size()), nq.data(), progs.data(), param_op.data(),
+                                   param_off_.data(), xs.data(), zs.data(), ys.data(), cs.data(), term_off.data(),
+                                   &plan_));
+}
+
+AdjointPlan::AdjointPlan(const Hamiltonian& hamiltonian, const ParametricCircuit& ansatz)
+    : AdjointPlan(std::span<const GradientJob>(
+          std::vector<GradientJob>{{&ansatz, &hamiltonian, std::vector<double>(ansatz.num_parameters(), 0.0)}})) {}
+
+AdjointPlan::~AdjointPlan() {
+    if (plan_) (void)lqcs_adjoint_plan_destroy(plan_);
+}
+
+AdjointPlan::AdjointPlan(AdjointPlan&& o) noexcept : plan_(o.plan_), param_off_(std::move(o.param_off_)) {
+    o.plan_ = nullptr;
+}
+
+AdjointPlan& AdjointPlan::operator=(AdjointPlan&& o) noexcept {
+    if (this != &o) {
+        if (plan_) (void)lqcs_adjoint_plan_destroy(plan_);
+        plan_ = o.plan_;
+        param_off_ = std::move(o.param_off_);
+        o.plan_ = nullptr;
+    }
+    return *this;
+}
+
+std::vector<GradientResult> AdjointPlan::run(std::span<const std::vector<double>> params) {
+    if (!plan_) throw std::logic_error("AdjointPlan: moved-from plan");
+    const std::size_t n = num_jobs();
+    if (params.size() != n) throw std::invalid_argument("AdjointPlan::run: one parameter vector per job");
+    std::vector<double> theta;
+    theta.reserve(param_off_.back());
+    for (std::size_t c = 0; c < n; ++c) {
+        if (params[c].size() != param_off_[c + 1] - param_off_[c])
+            throw std::invalid_argument("AdjointPlan::run: params size must equal num_parameters");
+        theta.insert(theta.end(), params[c].begin(), params[c].end());
+    }
+    std::vector<double> energies(n), grads(theta.size());
+    check(lqcs_adjoint_plan_run(plan_, theta.data(), theta.size(), energies.data(), grads.data()));
+    std::vector<GradientResult> out(n);
+    for (std::size_t c = 0; c < n; ++c) {
+        out[c].energy = energies[c];
+        out[c].grad.assign(grads.begin() + param_off_[c], grads.begin() + param_off_[c + 1]);
+    }
+    return out;
+}
+
+std::vector<double> AdjointPlan::gradient(std::span<const double> params, double* energy) {
+    if (!plan_) throw std::logic_error("AdjointPlan: moved-from plan");
+    if (num_jobs() != 1) throw std::invalid_argument("AdjointPlan::gradient: the plan holds several jobs");
+    if (params.size() != param_off_[1]) throw std::invalid_argument("AdjointPlan: params size must equal num_parameters");
+    double e = 0.0;
+    std::vector<double> g(params.size());
+    check(lqcs_adjoint_plan_run(plan_, params.data(), params.size(), &e, g.data()));
+    if (energy) *energy = e;
+    return g;
+}
+
 // Rotosolve (reference src/algorithms/vqe.cpp:19-63). Along one coordinate
 // E(t) = a + b cos(t - phi); with e0 = E(s), e+ = E(s + pi/2), e- = E(s - pi/2)
 // the minimum sits at s - pi/2 - atan2(2 e0 - e+ - e-, e+ - e-), folded into
@@ -185,6 +266,8 @@ VQEResult vqe(const Hamiltonian& hamiltonian, const Ansatz& ansatz, std::size_t 
 // is the adjoint one (one forward and one backward sweep instead of 2P
 // evolutions); the adjoint call at the new point also returns the energy
 // there, which is the step's history entry and the next step's gradient.
+// The ansatz is lowered and its launch sequence captured once (AdjointPlan);
+// an ansatz the plan refuses takes the per-call adjoint_gradient.
 VQEResult vqe_gradient_descent(const Hamiltonian& hamiltonian, const ParametricCircuit& ansatz,
                                const GradientVQEOptions& options) {
     const std::size_t n = ansatz.num_parameters();
@@ -195,15 +278,23 @@ VQEResult vqe_gradient_descent(const Hamiltonian& hamiltonian, const ParametricC
     else if (theta.size() != n)
         throw std::invalid_argument("vqe_gradient_descent: initial_parameters size must equal num_parameters");
     VQEResult res{};
+    std::optional<AdjointPlan> plan;
+    try {
+        plan.emplace(hamiltonian, ansatz);
+    } catch (const std::invalid_argument&) {
+    }
     double e = 0.0;
-    std::vector<double> g = adjoint_gradient(hamiltonian, ansatz, theta, &e);
+    auto grad_at = [&](const std::vector<double>& th) {
+        return plan ? plan->gradient(th, &e) : adjoint_gradient(hamiltonian, ansatz, th, &e);
+    };
+    std::vector<double> g = grad_at(theta);
     for (std::size_t it = 0; it < options.max_iterations; ++it) {
         double norm2 = 0.0;
         for (std::size_t k = 0; k < n; ++k) {
             theta[k] -= options.learning_rate * g[k];
             norm2 += g[k] * g[k];
         }
-        g = adjoint_gradient(hamiltonian, ansatz, theta, &e);
+        g = grad_at(theta);
         res.history.push_back(e);
         res.iterations = it + 1;
         if (std::sqrt(norm2) < options.tol) break;

@@ -430,6 +430,36 @@ PYBIND11_MODULE(_pylqcs, m) {
             return out;
         },
         "hamiltonians"_a, "ansatze"_a, "params"_a, "list of (energy, gradient)");
+    py::class_<algorithms::AdjointPlan>(alg, "AdjointPlan",
+                                        "Adjoint gradients of fixed ansatz/Hamiltonian pairs at changing "
+                                        "parameter values: the device launch sequence is captured once in a "
+                                        "graph and replayed by run().")
+        .def(py::init([](const std::vector<std::vector<std::pair<double, std::string>>>& hams,
+                         const std::vector<const ParametricCircuit*>& ansatze) {
+                 if (hams.size() != ansatze.size()) throw std::invalid_argument("AdjointPlan: list lengths differ");
+                 std::vector<algorithms::Hamiltonian> hs;
+                 for (const auto& t : hams) hs.push_back(to_h(t));
+                 std::vector<algorithms::GradientJob> jobs;
+                 for (std::size_t i = 0; i < ansatze.size(); ++i)
+                     jobs.push_back({ansatze[i], &hs[i], std::vector<double>(ansatze[i]->num_parameters(), 0.0)});
+                 py::gil_scoped_release nogil;
+                 return std::make_unique<algorithms::AdjointPlan>(jobs);
+             }),
+             "hamiltonians"_a, "ansatze"_a)
+        .def_property_readonly("num_jobs", &algorithms::AdjointPlan::num_jobs)
+        .def(
+            "run",
+            [](algorithms::AdjointPlan& plan, const std::vector<std::vector<double>>& params) {
+                std::vector<algorithms::GradientResult> r;
+                {
+                    py::gil_scoped_release nogil;
+                    r = plan.run(params);
+                }
+                py::list out;
+                for (const auto& x : r) out.append(py::make_tuple(x.energy, x.grad));
+                return out;
+            },
+            "params"_a, "list of (energy, gradient), one per job");
     py::class_<algorithms::VQEResult>(alg, "VQEResult")
         .def_readonly("energy", &algorithms::VQEResult::energy)
         .def_readonly("parameters", &algorithms::VQEResult::parameters)

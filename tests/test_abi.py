@@ -154,6 +154,38 @@ def test_batched_adjoint_rejects_null_arrays_before_any_device_call():
     assert b"non-decreasing" in lib.lqcs_dev_last_error()
 
 
+def test_adjoint_plan_checks_arguments_before_any_device_call():
+    """lqcs_adjoint_plan_create refuses an empty batch, a null out pointer and
+    the batched entry's bad arrays with INVALID_ARGUMENT before the device
+    is touched; run refuses a null plan; destroy accepts one."""
+    lib = ctypes.CDLL(os.path.join(LIB_DIR, "liblqcs_device.so"))
+    lib.lqcs_dev_last_error.restype = ctypes.c_char_p
+    u64 = ctypes.c_uint64
+    nq = (ctypes.c_uint32 * 1)(4)
+    progs = (_Program * 1)()
+    pop = (u64 * 2)(0, 0)
+    xs, zs = (u64 * 1)(1), (u64 * 1)(0)
+    ys = (ctypes.c_uint32 * 1)(0)
+    cs = (ctypes.c_double * 1)(1.0)
+    p1, t1 = (u64 * 2)(0, 2), (u64 * 2)(0, 1)
+    plan = ctypes.c_void_p()
+
+    def create(n, param_op, param_off, term_off, out):
+        return lib.lqcs_adjoint_plan_create(ctypes.c_uint32(n), nq, progs, param_op, param_off, xs, zs, ys, cs,
+                                            term_off, out)
+
+    assert create(1, pop, p1, t1, None) == 1
+    assert create(0, pop, p1, t1, ctypes.byref(plan)) == 1
+    assert b"at least one circuit" in lib.lqcs_dev_last_error()
+    assert create(1, None, p1, t1, ctypes.byref(plan)) == 1  # parameters without param_op
+    assert create(1, pop, (u64 * 2)(2, 1), t1, ctypes.byref(plan)) == 1
+    assert b"non-decreasing" in lib.lqcs_dev_last_error()
+    assert plan.value is None
+    th = (ctypes.c_double * 2)()
+    assert lib.lqcs_adjoint_plan_run(None, th, u64(2), None, th) == 1
+    assert lib.lqcs_adjoint_plan_destroy(None) == 0
+
+
 class _Op(ctypes.Structure):
     """lqcs_op (include/lqcs_device.h)."""
     _fields_ = [("kind", ctypes.c_uint32), ("n_controls", ctypes.c_uint32), ("n_targets", ctypes.c_uint32),
