@@ -387,6 +387,37 @@ def trajectories_bench(lq, W, seed: int, n: int, shots: int, cpu: bool = True, c
     return out
 
 
+def trajectories_small_bench(lq, W, seed: int, shots: int = 2048, widths=(8, 12)) -> dict:
+    """The noisy-trajectory workload at widths whose register fits one
+    workgroup's LDS (the 12-qubit variant is the one cross-checked against the
+    density matrix): the shared tiled runs over a batch in HBM against the
+    persistent kernel (one launch, a workgroup per shot), wall time of run()
+    for `shots` trajectories, counts compared."""
+    out = {}
+    noise = lq.NoiseModel()
+    noise.add_all_qubit_channel(lq.KrausChannel.depolarizing(1e-3))
+    noise.add_all_qubit_channel(lq.KrausChannel.amplitude_damping(1e-3))
+    for n in widths:
+        qc = lq.QuantumCircuit(n, n)
+        W.noisy_ghz_layers(qc, n, 10, seed)
+        qc.measure_all()
+        row, got = {}, {}
+        for kernel in ("shared", "persistent"):
+            sim = lq.StatevectorSimulator(seed=seed, noise=noise, trajectory_kernel=kernel)
+            sim.run(qc, 8)  # warm-up
+            lq.synchronize()
+            t0 = time.perf_counter()
+            got[kernel] = dict(sim.run(qc, shots).counts())
+            lq.synchronize()
+            dt = time.perf_counter() - t0
+            row[f"{kernel}_trajectories_per_s"] = round(shots / dt, 1)
+            log(f"trajectories {n}q {kernel}: {shots} shots in {dt * 1e3:.1f} ms")
+        row["speedup"] = round(row["persistent_trajectories_per_s"] / row["shared_trajectories_per_s"], 2)
+        row["counts_equal"] = got["shared"] == got["persistent"]
+        out[f"{n}q"] = row
+    return out
+
+
 def density_matrix_bench(lq, W, seed: int, widths=(12, 14, 16), cpu: bool = True, cpu_width: int = 10) -> dict:
     """The noisy workload's exact form: rho of GHZ prep + 10 random U3/CZ
     layers with depolarizing(1e-3) then amplitude_damping(1e-3) after every
@@ -723,6 +754,7 @@ def main() -> int:
     if not args.no_trajectories:
         trajectories = trajectories_bench(lq, W, args.seed, args.traj_qubits, args.traj_shots,
                                           cpu=not args.no_cpu_baseline)
+        trajectories["small_widths"] = trajectories_small_bench(lq, W, args.seed, args.traj_shots)
     if planner is not None and not args.no_planner_32q:
         planner["at_32_qubits"] = planner_32q(lq, W, args.seed)
     dm = None

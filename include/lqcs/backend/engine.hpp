@@ -17,6 +17,10 @@ struct EngineOptions {
     int low_qubits = 0;             // contiguous low qubits per tile (0 = default)
     std::size_t max_qubits = 30;    // register cap; raise up to 34 on the device
     std::uint32_t max_batch = 0;    // trajectories resident at once (0 = by free HBM)
+    // Noisy-trajectory kernel: 0 auto (persistent where the register fits one
+    // workgroup's LDS), 1 shared tiled runs over a batch of registers in HBM,
+    // 2 persistent (one workgroup per shot, at most 12 qubits).
+    int trajectory_kernel = 0;
     // Tiled-pass kernel: 0 engine default, 1 LDS tile, 2 register tile,
     // 3 register tile streaming, 4 register tile with an LDS landing buffer.
     int tile_kernel = 0;

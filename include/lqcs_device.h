@@ -299,12 +299,25 @@ typedef struct lqcs_channel {
     const double* ops;     /* n_ops * 4 complex, row-major 2x2, (re,im) */
 } lqcs_channel;
 
+/* Trajectory kernels (the `kernel` argument). PERSISTENT runs every shot in
+ * one launch, a workgroup per shot with its register in LDS, all branch
+ * decisions on the device; it needs n_qubits <= 12 and ops of at most two
+ * targets (else INVALID_ARGUMENT). SHARED_RUNS keeps max_batch registers in
+ * HBM and runs the ops between state-dependent events as tiled passes over
+ * the batch. AUTO takes PERSISTENT where it applies. */
+enum {
+    LQCS_TRAJ_AUTO = 0,
+    LQCS_TRAJ_SHARED_RUNS = 1,
+    LQCS_TRAJ_PERSISTENT = 2
+};
+
 lqcs_status lqcs_trajectories_run(uint32_t n_qubits, uint32_t n_clbits,
                                   const lqcs_program* prog, const lqcs_traj_step* steps,
                                   uint64_t n_steps, const uint32_t* noise_qubits,
                                   const lqcs_channel* channels, uint32_t n_channels,
                                   const double* draws, uint64_t draws_per_shot,
-                                  uint64_t n_shots, uint32_t max_batch, uint8_t* clbits_out);
+                                  uint64_t n_shots, uint32_t max_batch, uint32_t kernel,
+                                  uint8_t* clbits_out);
 
 /* ---- adjoint differentiation of <H> for Pauli-rotation parameters
  * (replaces algorithms::parameter_shift_gradient, src/algorithms/vqe.cpp:68-94,

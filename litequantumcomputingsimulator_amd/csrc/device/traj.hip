@@ -32,6 +32,7 @@
 #include "common.hpp"
 #include "engine.hpp"
 #include "reduce.hpp"
+#include "tile_apply.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -392,6 +393,290 @@ void fuse_run(std::vector<DecodedOp>& ops) {
     ops.swap(out);
 }
 
+// ---- persistent trajectories -------------------------------------------------
+// A register of up to 12 qubits fits one workgroup's LDS (64 KiB), so a whole
+// trajectory runs inside one workgroup: one launch covers every shot (one
+// workgroup each), the register never leaves LDS, every branch decision is
+// taken in the kernel (the draw against a mixture's weights or a general
+// channel's default bound, the Kraus rule on the register's 2x2 statistics
+// when the bound does not settle it, measurement and reset), and the only
+// HBM traffic is the shot's draws and outcomes. Registers narrower than 6
+// qubits run in a 6-qubit tile whose extra qubits stay |0>.
+constexpr uint32_t kPersistMaxQubits = 12;
+
+enum PStepKind : uint32_t { PS_GATE = 0, PS_NOISE = 1, PS_MEASURE = 2, PS_RESET = 3 };
+
+struct PStep {
+    uint32_t kind;
+    uint32_t q;     // noise, measure and reset: the qubit
+    uint32_t arg;   // gate: index in the gate table; noise: channel; measure: clbit
+    uint32_t pad;
+    uint64_t draw;  // noise, measure and reset: draw offset within the shot
+};
+
+struct ChannelDev {
+    amp_t k[kMaxKraus][4];     // Kraus operators (row-major 2x2)
+    amp_t unit[kMaxKraus][4];  // mixture: K_i / sqrt(c_i)
+    double c[kMaxKraus];       // mixture: K_i^dag K_i = c_i I
+    double bound;              // general: draws below it take branch `def`
+    uint32_t n;
+    uint32_t mixture;
+    int32_t def;               // default branch, -1 when every operator is zero
+    uint32_t def_identity;     // mixture whose default branch is the identity
+};
+
+// m (row-major 2x2) on qubit q of the LDS register.
+template <int L, int T>
+__device__ __forceinline__ void lds_2x2(amp_t* tile, uint32_t q, amp_t m0, amp_t m1, amp_t m2, amp_t m3) {
+    constexpr int N = 1 << L;
+#pragma unroll
+    for (int k = 0; k < kIters<N / 2, T>; ++k) {
+        const uint32_t p = threadIdx.x + k * T;
+        if (N / 2 < T && p >= N / 2) break;
+        const uint32_t i0 = static_cast<uint32_t>(insert_zero_bit(p, q)), i1 = i0 | (1u << q);
+        const amp_t a0 = tile[i0], a1 = tile[i1];
+        tile[i0] = cmac2(m0, a0, m1, a1);
+        tile[i1] = cmac2(m2, a0, m3, a1);
+    }
+}
+
+// 2x2 statistics of qubit q over the block (S00, S11, Re and Im of
+// sum conj(a0) a1), summed in a fixed order; every thread receives them.
+template <int L, int T>
+__device__ __forceinline__ void lds_stats(const amp_t* tile, uint32_t q, double* red, double (&st)[4]) {
+    constexpr int N = 1 << L;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < kIters<N / 2, T>; ++k) {
+        const uint32_t p = threadIdx.x + k * T;
+        if (N / 2 < T && p >= N / 2) break;
+        const uint32_t i0 = static_cast<uint32_t>(insert_zero_bit(p, q)), i1 = i0 | (1u << q);
+        const amp_t a0 = tile[i0], a1 = tile[i1];
+        a[0] += cnorm(a0);
+        a[1] += cnorm(a1);
+        a[2] += a0.x * a1.x + a0.y * a1.y;
+        a[3] += a0.x * a1.y - a0.y * a1.x;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[c] = tsum(a[c]);
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) red[(threadIdx.x >> 6) * 4 + c] = a[c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        st[c] = 0.0;
+        for (int w = 0; w < T / 64; ++w) st[c] += red[w * 4 + c];
+    }
+    __syncthreads();  // every read of red is done before it is written again
+}
+
+template <int L, int T>
+__global__ void __launch_bounds__(T) k_traj_persist(const PStep* __restrict__ prog, uint32_t n_steps,
+                                                    const TileOp* __restrict__ gates,
+                                                    const ChannelDev* __restrict__ chans,
+                                                    const double* __restrict__ draws, uint64_t D,
+                                                    uint8_t* __restrict__ creg, uint32_t creg_w) {
+    constexpr int N = 1 << L;
+    constexpr int PER = N / T;
+    __shared__ amp_t tile[N];
+    __shared__ double red[4 * (T / 64)];
+    const double* u_shot = draws + uint64_t(blockIdx.x) * D;
+    uint8_t* c_shot = creg + uint64_t(blockIdx.x) * creg_w;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const uint32_t j = threadIdx.x + k * T;
+        tile[j] = make_double2(j == 0 ? 1.0 : 0.0, 0.0);
+    }
+    for (uint32_t c = threadIdx.x; c < creg_w; c += T) c_shot[c] = 0;
+    __syncthreads();
+    for (uint32_t i = 0; i < n_steps; ++i) {
+        const PStep st = prog[i];
+        if (st.kind == PS_GATE) {
+            tile_apply<L, T>(tile, gates[st.arg], 0);
+            __syncthreads();
+            continue;
+        }
+        const double u = u_shot[st.draw];
+        if (st.kind == PS_NOISE) {
+            const ChannelDev& ch = chans[st.arg];
+            if (ch.def < 0) continue;
+            if (ch.mixture) {
+                // sample_kraus_branch with p_i = c_i whatever the state
+                int k = -1;
+                double cum = 0.0;
+                for (uint32_t b = 0; b < ch.n; ++b) {
+                    if (ch.c[b] <= 0.0) continue;
+                    k = static_cast<int>(b);
+                    cum += ch.c[b];
+                    if (u < cum) break;
+                }
+                if (k == ch.def && ch.def_identity) continue;
+                lds_2x2<L, T>(tile, st.q, ch.unit[k][0], ch.unit[k][1], ch.unit[k][2], ch.unit[k][3]);
+                __syncthreads();
+                continue;
+            }
+            if (u < ch.bound) {  // branch `def` whatever the state (kept unnormalised)
+                const amp_t* K = ch.k[ch.def];
+                lds_2x2<L, T>(tile, st.q, K[0], K[1], K[2], K[3]);
+                __syncthreads();
+                continue;
+            }
+            double sv[4];
+            lds_stats<L, T>(tile, st.q, red, sv);
+            const double norm2 = sv[0] + sv[1];
+            int chosen = -1;
+            double chosen_p = 0.0, cum = 0.0;
+            for (uint32_t b = 0; b < ch.n; ++b) {
+                const amp_t* K = ch.k[b];
+                const double c00 = cnorm(K[0]) + cnorm(K[2]);
+                const double c11 = cnorm(K[1]) + cnorm(K[3]);
+                const double xr = K[0].x * K[1].x + K[0].y * K[1].y + K[2].x * K[3].x + K[2].y * K[3].y;
+                const double xi = K[0].x * K[1].y - K[0].y * K[1].x + K[2].x * K[3].y - K[2].y * K[3].x;
+                const double p = (c00 * sv[0] + c11 * sv[1] + 2.0 * (xr * sv[2] - xi * sv[3])) / norm2;
+                if (p <= 0.0) continue;
+                chosen = static_cast<int>(b);
+                chosen_p = p;
+                cum += p;
+                if (u < cum) break;
+            }
+            if (chosen < 0) continue;
+            const double inv = 1.0 / sqrt(chosen_p * norm2);
+            const amp_t* K = ch.k[chosen];
+            lds_2x2<L, T>(tile, st.q, cscale(K[0], inv), cscale(K[1], inv), cscale(K[2], inv), cscale(K[3], inv));
+            __syncthreads();
+            continue;
+        }
+        // measure / reset (collapse_qubit / reset_qubit): p1 = S11 / (S00 + S11)
+        double sv[4];
+        lds_stats<L, T>(tile, st.q, red, sv);
+        const bool one = u < sv[1] / (sv[0] + sv[1]);
+        const double scale = 1.0 / sqrt(one ? sv[1] : sv[0]);
+        const bool reset = st.kind == PS_RESET;
+        const amp_t z = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < kIters<N / 2, T>; ++k) {
+            const uint32_t p = threadIdx.x + k * T;
+            if (N / 2 < T && p >= N / 2) break;
+            const uint32_t i0 = static_cast<uint32_t>(insert_zero_bit(p, st.q)), i1 = i0 | (1u << st.q);
+            const amp_t a0 = tile[i0], a1 = tile[i1];
+            if (one) {
+                tile[i0] = reset ? cscale(a1, scale) : z;
+                tile[i1] = reset ? z : cscale(a1, scale);
+            } else {
+                tile[i0] = cscale(a0, scale);
+                tile[i1] = z;
+            }
+        }
+        if (!reset && threadIdx.x == 0) c_shot[st.arg] = one ? 1 : 0;
+        __syncthreads();
+    }
+}
+
+// Whether every op of the program runs as a tile op of a whole-register tile.
+bool persistent_fits(uint32_t nq, const std::vector<DecodedOp>& ops) {
+    if (nq > kPersistMaxQubits) return false;
+    for (const DecodedOp& d : ops)
+        if (tile_requirement(d) == ~uint64_t{0}) return false;
+    return true;
+}
+
+template <int L, int T>
+void launch_persist(unsigned shots, const PStep* prog, uint32_t n_steps, const TileOp* gates,
+                    const ChannelDev* chans, const double* draws, uint64_t D, uint8_t* creg, uint32_t creg_w) {
+    k_traj_persist<L, T><<<shots, T, 0, stream()>>>(prog, n_steps, gates, chans, draws, D, creg, creg_w);
+    LQCS_HIP(hipGetLastError());
+}
+
+// All shots through k_traj_persist, in launches of at most 2^16 shots.
+void run_persistent(uint32_t nq, uint32_t n_clbits, const std::vector<DecodedOp>& ops, const lqcs_traj_step* steps,
+                    uint64_t n_steps, const uint32_t* noise_qubits, const std::vector<KrausSet>& sets,
+                    const std::vector<ChannelPlan>& cplan, const std::vector<uint64_t>& off, const double* draws,
+                    uint64_t D, uint64_t n_shots, uint8_t* clbits_out) {
+    const uint32_t L = std::max<uint32_t>(6, nq);
+    const uint64_t smask = (uint64_t{1} << L) - 1;
+    std::vector<TileOp> gates;
+    std::vector<PStep> prog;
+    const uint32_t n_channels = static_cast<uint32_t>(sets.size());
+    for (uint64_t i = 0; i < n_steps; ++i) {
+        const lqcs_traj_step& st = steps[i];
+        PStep ps;
+        std::memset(&ps, 0, sizeof ps);
+        if (st.kind != LQCS_TOP_UNITARY) {
+            ps.kind = st.kind == LQCS_TOP_MEASURE ? PS_MEASURE : PS_RESET;
+            ps.q = st.qubit;
+            ps.arg = st.kind == LQCS_TOP_MEASURE ? st.clbit : 0;
+            ps.draw = off[i];
+            prog.push_back(ps);
+            continue;
+        }
+        ps.kind = PS_GATE;
+        ps.arg = static_cast<uint32_t>(gates.size());
+        gates.push_back(make_tile_op(ops[st.op_index], smask));
+        prog.push_back(ps);
+        uint64_t dd = off[i];
+        for (uint32_t c = 0; c < n_channels; ++c)
+            for (uint32_t j = 0; j < st.n_noise_qubits; ++j, ++dd) {
+                PStep ns;
+                std::memset(&ns, 0, sizeof ns);
+                ns.kind = PS_NOISE;
+                ns.q = noise_qubits[st.noise_qubit_offset + j];
+                ns.arg = c;
+                ns.draw = dd;
+                prog.push_back(ns);
+            }
+    }
+    if (prog.size() > UINT32_MAX) throw Error(LQCS_ERR_INVALID_ARGUMENT, "trajectory program too long");
+    std::vector<ChannelDev> chans(std::max<uint32_t>(1, n_channels));
+    for (uint32_t c = 0; c < n_channels; ++c) {
+        ChannelDev& cd = chans[c];
+        std::memset(&cd, 0, sizeof cd);
+        const ChannelPlan& cp = cplan[c];
+        cd.n = sets[c].n;
+        cd.mixture = cp.mixture ? 1u : 0u;
+        cd.def = cp.def;
+        cd.def_identity = cp.def_identity ? 1u : 0u;
+        cd.bound = cp.bound;
+        for (uint32_t b = 0; b < sets[c].n; ++b) {
+            for (int e = 0; e < 4; ++e) cd.k[b][e] = sets[c].k[b][e];
+            if (cp.mixture) {
+                cd.c[b] = cp.c[b];
+                for (int e = 0; e < 4; ++e) cd.unit[b][e] = cp.unit[b].m[e];
+            }
+        }
+    }
+    const uint32_t creg_w = std::max<uint32_t>(1, n_clbits);
+    const uint64_t chunk = std::min<uint64_t>(n_shots, uint64_t{1} << 16);
+    const PStep* d_prog = stage(prog.data(), prog.size());
+    const TileOp* d_gates = stage(gates.data(), std::max<size_t>(1, gates.size()));
+    const ChannelDev* d_chans = stage(chans.data(), chans.size());
+    double* d_draws = static_cast<double*>(arena(sizeof(double) * chunk * std::max<uint64_t>(D, 1)));
+    uint8_t* d_creg = static_cast<uint8_t*>(arena(chunk * creg_w));
+    std::vector<uint8_t> host_creg(chunk * creg_w);
+    const uint32_t np = static_cast<uint32_t>(prog.size());
+    for (uint64_t s0 = 0; s0 < n_shots; s0 += chunk) {
+        const uint64_t nb = std::min(chunk, n_shots - s0);
+        if (D)
+            LQCS_HIP(hipMemcpyAsync(d_draws, draws + s0 * D, nb * D * sizeof(double), hipMemcpyHostToDevice,
+                                    stream()));
+        const unsigned g = static_cast<unsigned>(nb);
+        switch (L) {
+            case 6: launch_persist<6, 64>(g, d_prog, np, d_gates, d_chans, d_draws, D, d_creg, creg_w); break;
+            case 7: launch_persist<7, 64>(g, d_prog, np, d_gates, d_chans, d_draws, D, d_creg, creg_w); break;
+            case 8: launch_persist<8, 64>(g, d_prog, np, d_gates, d_chans, d_draws, D, d_creg, creg_w); break;
+            case 9: launch_persist<9, 128>(g, d_prog, np, d_gates, d_chans, d_draws, D, d_creg, creg_w); break;
+            case 10: launch_persist<10, 256>(g, d_prog, np, d_gates, d_chans, d_draws, D, d_creg, creg_w); break;
+            case 11: launch_persist<11, 256>(g, d_prog, np, d_gates, d_chans, d_draws, D, d_creg, creg_w); break;
+            default: launch_persist<12, 256>(g, d_prog, np, d_gates, d_chans, d_draws, D, d_creg, creg_w); break;
+        }
+        LQCS_HIP(hipMemcpyAsync(host_creg.data(), d_creg, nb * creg_w, hipMemcpyDeviceToHost, stream()));
+        sync();
+        for (uint64_t b = 0; b < nb; ++b)
+            std::memcpy(clbits_out + (s0 + b) * n_clbits, host_creg.data() + b * creg_w, n_clbits);
+    }
+}
+
 template <typename T>
 T* dev_array(size_t n) {
     return static_cast<T*>(device_alloc(std::max<size_t>(n, 1) * sizeof(T)));
@@ -407,7 +692,7 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
                                              const uint32_t* noise_qubits, const lqcs_channel* channels,
                                              uint32_t n_channels, const double* draws,
                                              uint64_t D, uint64_t n_shots, uint32_t max_batch,
-                                             uint8_t* clbits_out) {
+                                             uint32_t kernel, uint8_t* clbits_out) {
     return run_guarded([&] {
         if (nq == 0 || nq > LQCS_DEVICE_MAX_QUBITS) throw Error(LQCS_ERR_INVALID_ARGUMENT, "bad register width");
         if (!prog || (n_steps && !steps)) throw Error(LQCS_ERR_INVALID_ARGUMENT, "null program or steps");
@@ -460,6 +745,16 @@ extern "C" lqcs_status lqcs_trajectories_run(uint32_t nq, uint32_t n_clbits, con
             }
         }
         if (d != D) throw Error(LQCS_ERR_INVALID_ARGUMENT, "draws_per_shot does not match the steps");
+        if (kernel > LQCS_TRAJ_PERSISTENT) throw Error(LQCS_ERR_INVALID_ARGUMENT, "unknown trajectory kernel");
+        const bool fits = persistent_fits(nq, ops);
+        if (kernel == LQCS_TRAJ_PERSISTENT && !fits)
+            throw Error(LQCS_ERR_INVALID_ARGUMENT, "persistent trajectories need at most 12 qubits and ops of at "
+                                                   "most two targets");
+        if (fits && kernel != LQCS_TRAJ_SHARED_RUNS) {
+            run_persistent(nq, n_clbits, ops, steps, n_steps, noise_qubits, sets, cplan, off, draws, D, n_shots,
+                           clbits_out);
+            return;
+        }
 
         // Trailing measurements (nothing but measurements after them) sample
         // without collapsing (k_term_*).

@@ -278,7 +278,8 @@ Result StatevectorSimulator::run(const QuantumCircuit& circuit, std::size_t shot
     const lqcs_program prog = b.program();
     check(lqcs_trajectories_run(static_cast<uint32_t>(opt.num_qubits()), static_cast<uint32_t>(ncl), &prog,
                                 steps.data(), steps.size(), noise_qubits.data(), chans.data(), nch, draws.data(),
-                                D, shots, options_.engine.max_batch, clbits.data()));
+                                D, shots, options_.engine.max_batch,
+                                static_cast<uint32_t>(options_.engine.trajectory_kernel), clbits.data()));
     std::string key(ncl, '0');
     for (std::size_t s = 0; s < shots; ++s) {
         for (std::size_t c = 0; c < ncl; ++c) key[ncl - 1 - c] = clbits[s * ncl + c] ? '1' : '0';
@@ -307,6 +308,7 @@ EngineOptions CompiledCircuit::resolve(const EngineOptions& e) const {
     EngineOptions t = *tuned_;
     t.max_qubits = e.max_qubits;
     t.max_batch = e.max_batch;
+    t.trajectory_kernel = e.trajectory_kernel;
     return t;
 }
 

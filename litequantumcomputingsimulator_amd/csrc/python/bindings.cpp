@@ -44,6 +44,13 @@ EngineOptions::Plan plan_of(const std::string& s) {
     throw std::invalid_argument("plan must be gate_by_gate, greedy, gain, learned or auto");
 }
 
+int trajectory_kernel_of(const std::string& s) {
+    if (s == "auto") return 0;
+    if (s == "shared") return 1;
+    if (s == "persistent") return 2;
+    throw std::invalid_argument("trajectory_kernel must be auto, shared or persistent");
+}
+
 int tile_kernel_of(const std::string& s) {
     if (s == "default") return 0;
     if (s == "lds") return 1;
@@ -253,8 +260,9 @@ PYBIND11_MODULE(_pylqcs, m) {
     py::class_<StatevectorSimulator>(m, "StatevectorSimulator")
         .def(py::init([](std::optional<std::uint64_t> seed, bool fuse_gates, int num_threads, NoiseModel noise,
                          const std::string& plan, int tile_qubits, int low_qubits, std::size_t max_qubits,
-                         std::uint32_t max_batch, bool eliminate_redundant) {
+                         std::uint32_t max_batch, bool eliminate_redundant, const std::string& trajectory_kernel) {
                  StatevectorSimulator::Options o;
+                 o.engine.trajectory_kernel = trajectory_kernel_of(trajectory_kernel);
                  o.engine.eliminate_redundant = eliminate_redundant;
                  o.seed = seed;
                  o.fuse_gates = fuse_gates;
@@ -269,7 +277,7 @@ PYBIND11_MODULE(_pylqcs, m) {
              }),
              "seed"_a = py::none(), "fuse_gates"_a = true, "num_threads"_a = 0, "noise"_a = NoiseModel{},
              "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0, "max_qubits"_a = 30, "max_batch"_a = 0,
-             "eliminate_redundant"_a = false)
+             "eliminate_redundant"_a = false, "trajectory_kernel"_a = "auto")
         .def("run", &StatevectorSimulator::run, "circuit"_a, "shots"_a = 1024,
              py::call_guard<py::gil_scoped_release>())
         .def("run_statevector", &StatevectorSimulator::run_statevector, "circuit"_a,

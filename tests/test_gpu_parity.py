@@ -138,7 +138,7 @@ def test_wave_tiles_noisy_trajectories_match_oracle_counts(lq):
     c, chans = cases.noisy_ghz(12)
     want = O.run_counts(c, seed=13, shots=256, channels=chans)
     got = lq.StatevectorSimulator(seed=13, noise=_noise_model(lq), plan="greedy", tile_qubits=10,
-                                  low_qubits=3).run(c.to_product(lq), 256).counts()
+                                  low_qubits=3, trajectory_kernel="shared").run(c.to_product(lq), 256).counts()
     assert dict(got) == want
 
 
@@ -174,15 +174,17 @@ def test_run_counts_match_oracle(lq, fuse):
     assert dict(got) == want
 
 
-def test_midcircuit_counts_match_oracle(lq):
+@pytest.mark.parametrize("kernel", ["shared", "persistent"])
+def test_midcircuit_counts_match_oracle(lq, kernel):
     c = O.Circuit(2, 2)
     c.h(0).t(0).measure(0, 0).h(1).cx(0, 1).reset(0).h(0).measure(1, 1)
     want = O.run_counts(c, seed=9, shots=256)
-    got = lq.StatevectorSimulator(seed=9).run(c.to_product(lq), 256).counts()
+    got = lq.StatevectorSimulator(seed=9, trajectory_kernel=kernel).run(c.to_product(lq), 256).counts()
     assert dict(got) == want
 
 
-def test_noisy_counts_match_oracle(lq):
+@pytest.mark.parametrize("kernel", ["shared", "persistent"])
+def test_noisy_counts_match_oracle(lq, kernel):
     c = O.Circuit(3, 3)
     c.h(0).cx(0, 1).t(1).cx(1, 2).u(0.3, 0.2, 0.1, 2).measure_all()
     chans = [O.depolarizing(0.1), O.amplitude_damping(0.2)]
@@ -190,7 +192,7 @@ def test_noisy_counts_match_oracle(lq):
     noise = lq.NoiseModel()
     noise.add_all_qubit_channel(lq.KrausChannel.depolarizing(0.1))
     noise.add_all_qubit_channel(lq.KrausChannel.amplitude_damping(0.2))
-    got = lq.StatevectorSimulator(seed=99, noise=noise).run(c.to_product(lq), 400).counts()
+    got = lq.StatevectorSimulator(seed=99, noise=noise, trajectory_kernel=kernel).run(c.to_product(lq), 400).counts()
     assert dict(got) == want
 
 
@@ -415,19 +417,22 @@ def _noise_model(lq, p=1e-3, gamma=1e-3):
     return noise
 
 
-def test_noisy_trajectories_12q_match_oracle_counts(lq):
-    """The noisy-trajectory workload at 12 qubits: batched device trajectories
-    reproduce the oracle's per-shot counts exactly (same draws)."""
+@pytest.mark.parametrize("kernel", ["shared", "persistent"])
+def test_noisy_trajectories_12q_match_oracle_counts(lq, kernel):
+    """The noisy-trajectory workload at 12 qubits: device trajectories (shared
+    tiled runs over a batch, or one workgroup per shot with the register in
+    LDS) reproduce the oracle's per-shot counts exactly (same draws)."""
     from tests import cases
 
     c, chans = cases.noisy_ghz(12)
     want = O.run_counts(c, seed=11, shots=256, channels=chans)
-    got = lq.StatevectorSimulator(seed=11, noise=_noise_model(lq)).run(c.to_product(lq), 256).counts()
-    assert dict(got) == want
+    sim = lq.StatevectorSimulator(seed=11, noise=_noise_model(lq), trajectory_kernel=kernel)
+    assert dict(sim.run(c.to_product(lq), 256).counts()) == want
 
 
+@pytest.mark.parametrize("kernel", ["shared", "persistent"])
 @pytest.mark.parametrize("fixture,n,rate", [("dm_noisy_ghz12.npy", 12, 1e-3), ("dm_noisy_ghz10_strong.npy", 10, 0.05)])
-def test_noisy_trajectories_match_density_matrix(lq, fixture, n, rate):
+def test_noisy_trajectories_match_density_matrix(lq, fixture, n, rate, kernel):
     """2048 device trajectories of the noisy workload against the exact
     distribution from the reference's density-matrix simulator
     (tests/golden/make_dm_fixture.py): every single-qubit marginal and the
@@ -444,7 +449,8 @@ def test_noisy_trajectories_match_density_matrix(lq, fixture, n, rate):
     shots = 2048
     p = np.load(os.path.join(os.path.dirname(__file__), "golden", fixture))
     c, _ = cases.noisy_ghz(n, p=rate, gamma=rate)
-    counts = lq.StatevectorSimulator(seed=2024, noise=_noise_model(lq, rate, rate)).run(c.to_product(lq), shots).counts()
+    sim = lq.StatevectorSimulator(seed=2024, noise=_noise_model(lq, rate, rate), trajectory_kernel=kernel)
+    counts = sim.run(c.to_product(lq), shots).counts()
     freq = np.zeros(1 << n)
     for key, v in dict(counts).items():
         freq[int(key, 2)] += v

@@ -6,7 +6,9 @@ oracle's, which collapses, draw for draw. Orders that are not the low-qubit
 prefix, repeated qubits, clbit remapping, and a measurement followed by
 gates (so only the last ones are trailing) are covered. Events: branches of
 state-independent channels (unitary default or not) and state-dependent
-draws, at batch sizes that put one, few or many events on a site."""
+draws, at batch sizes that put one, few or many events on a site. Each
+case runs through both trajectory kernels: the persistent one (a workgroup
+per shot, register in LDS) and the shared tiled runs over a batch in HBM."""
 import numpy as np
 import pytest
 
@@ -27,18 +29,23 @@ def noisy_body(n, seed):
     return c
 
 
-def counts(lq, c, seed, shots, chans):
+KERNELS = ["shared", "persistent"]
+
+
+def counts(lq, c, seed, shots, chans, kernel):
     noise = lq.NoiseModel()
     for name, p in chans:
         noise.add_all_qubit_channel(getattr(lq.KrausChannel, name)(p))
-    return dict(lq.StatevectorSimulator(seed=seed, noise=noise).run(c.to_product(lq), shots).counts())
+    sim = lq.StatevectorSimulator(seed=seed, noise=noise, trajectory_kernel=kernel)
+    return dict(sim.run(c.to_product(lq), shots).counts())
 
 
 CH = [("depolarizing", 0.02), ("amplitude_damping", 0.03)]
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("order", ["prefix", "reversed", "scrambled", "repeat"])
-def test_trailing_measurements_match_oracle(lq, order):
+def test_trailing_measurements_match_oracle(lq, order, kernel):
     n, seed, shots = 9, 17, 300
     c = noisy_body(n, 5)
     qs = list(range(n))
@@ -51,26 +58,30 @@ def test_trailing_measurements_match_oracle(lq, order):
     for k, q in enumerate(qs):
         c.measure(q, (k * 5) % n)
     want = O.run_counts(c, seed=seed, shots=shots, channels=[getattr(O, a)(b) for a, b in CH])
-    assert counts(lq, c, seed, shots, CH) == want
+    assert counts(lq, c, seed, shots, CH, kernel) == want
 
 
-def test_measurement_then_gates_then_trailing(lq):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_measurement_then_gates_then_trailing(lq, kernel):
     n, seed, shots = 8, 23, 300
     c = noisy_body(n, 9)
     c.measure(2, 0).h(2).cx(2, 5).measure(4, 1)
     c.rx(0.4, 4).measure(5, 2).measure(4, 3).measure(0, 4)
     want = O.run_counts(c, seed=seed, shots=shots, channels=[getattr(O, a)(b) for a, b in CH])
-    assert counts(lq, c, seed, shots, CH) == want
+    assert counts(lq, c, seed, shots, CH, kernel) == want
 
 
-def test_noiseless_midcircuit_trailing(lq):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_noiseless_midcircuit_trailing(lq, kernel):
     c = O.Circuit(3, 3)
     c.h(0).cx(0, 1).measure(0, 0).h(0).t(2).h(2).measure(2, 1).measure(1, 2)
     want = O.run_counts(c, seed=4, shots=400)
-    assert dict(lq.StatevectorSimulator(seed=4).run(c.to_product(lq), 400).counts()) == want
+    sim = lq.StatevectorSimulator(seed=4, trajectory_kernel=kernel)
+    assert dict(sim.run(c.to_product(lq), 400).counts()) == want
 
 
-def test_mixture_with_unitary_default_branch(lq):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_mixture_with_unitary_default_branch(lq, kernel):
     """A state-independent channel whose default branch is not the identity
     (sqrt(1-p) S, sqrt(p) X): the default rides on every register and an
     event register takes X S^dagger after it, in the shared run."""
@@ -86,14 +97,16 @@ def test_mixture_with_unitary_default_branch(lq):
     noise = lq.NoiseModel()
     noise.add_all_qubit_channel(lq.KrausChannel([np.array(o, complex) for o in ops]))
     noise.add_all_qubit_channel(lq.KrausChannel.amplitude_damping(0.04))
-    got = dict(lq.StatevectorSimulator(seed=seed, noise=noise).run(c.to_product(lq), shots).counts())
+    sim = lq.StatevectorSimulator(seed=seed, noise=noise, trajectory_kernel=kernel)
+    got = dict(sim.run(c.to_product(lq), shots).counts())
     assert got == want
 
 
 @pytest.mark.parametrize("max_batch", [1, 3, 64])
 def test_event_handling_independent_of_batch_size(lq, max_batch):
     """Single-register ops, deferred defaults and skip launches depend on how
-    many events share a site within a batch; the counts do not."""
+    many events share a site within a batch (shared tiled runs); the counts
+    do not."""
     n, seed, shots = 8, 12, 200
     c = noisy_body(n, 4)
     c.measure_all()
@@ -102,5 +115,5 @@ def test_event_handling_independent_of_batch_size(lq, max_batch):
     noise = lq.NoiseModel()
     noise.add_all_qubit_channel(lq.KrausChannel.depolarizing(0.06))
     noise.add_all_qubit_channel(lq.KrausChannel.amplitude_damping(0.06))
-    sim = lq.StatevectorSimulator(seed=seed, noise=noise, max_batch=max_batch)
+    sim = lq.StatevectorSimulator(seed=seed, noise=noise, max_batch=max_batch, trajectory_kernel="shared")
     assert dict(sim.run(c.to_product(lq), shots).counts()) == want
