@@ -623,6 +623,9 @@ def main() -> int:
     ap.add_argument("--plan", default="greedy")
     ap.add_argument("--tile-qubits", type=int, default=0)
     ap.add_argument("--low-qubits", type=int, default=0)
+    ap.add_argument("--tile-kernel", default="register",
+                    help="kernel of the tiled passes: register (interpreting register tile) or specialized "
+                         "(one kernel per pass structure, compiled at run time before the timed steps)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adjoint-circuits", type=int, default=256)
@@ -662,12 +665,19 @@ def main() -> int:
 
     def step():
         sv.set_basis(0)
-        comp.apply(sv, plan=args.plan, tile_qubits=args.tile_qubits, low_qubits=args.low_qubits)
+        comp.apply(sv, plan=args.plan, tile_qubits=args.tile_qubits, low_qubits=args.low_qubits,
+                   tile_kernel=args.tile_kernel)
 
+    # The first apply plans the circuit and, for specialized passes, compiles
+    # one kernel per distinct pass structure; it is reported, not timed.
+    first_apply_s = None
     for w in range(args.warmup):
+        t0 = time.perf_counter()
         step()
         lq.synchronize()
-        log(f"warmup {w + 1}/{args.warmup} done")
+        if first_apply_s is None:
+            first_apply_s = time.perf_counter() - t0
+        log(f"warmup {w + 1}/{args.warmup} done ({time.perf_counter() - t0:.3f} s)")
     stats = lq.last_plan_stats()
 
     lq.profile_enable(True)
@@ -783,7 +793,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"random U3+CX layers, {n} qubits, {layers} layers, greedy 1q fusion",
                    "qubits": n, "layers": layers, "gates": gates, "plan": args.plan,
-                   "tile_qubits": tile_q, "low_qubits": low_q,
+                   "tile_qubits": tile_q, "low_qubits": low_q, "tile_kernel": args.tile_kernel,
+                   "first_apply_s": None if first_apply_s is None else round(first_apply_s, 3),
                    "passes_per_step": stats["passes"], "seed": args.seed},
         "roofline": roofline,
         "cpu_baseline": cpu,

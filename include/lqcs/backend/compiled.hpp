@@ -37,6 +37,15 @@ public:
     };
     std::vector<PassInfo> describe(const EngineOptions& engine = {}) const;
 
+    // Specialized kernel of pass `step` of that plan (no device needed): the
+    // kernel text the engine compiles for it and the op words it reads, as
+    // (re, im) pairs; empty text when the pass runs on another kernel.
+    struct SpecializedPass {
+        std::string text;
+        std::vector<double> words;
+    };
+    SpecializedPass specialized_pass(std::uint64_t step, const EngineOptions& engine = {}) const;
+
     // Cost-model features of every pass of that plan (kCostFeatures each,
     // row-major), for `batch` registers per launch.
     std::vector<double> features(const EngineOptions& engine = {}, std::uint32_t batch = 1) const;

@@ -22,7 +22,9 @@ struct EngineOptions {
     // 2 persistent (one workgroup per shot, at most 12 qubits).
     int trajectory_kernel = 0;
     // Tiled-pass kernel: 0 engine default, 1 LDS tile, 2 register tile,
-    // 3 register tile streaming, 4 register tile with an LDS landing buffer.
+    // 3 register tile streaming, 4 register tile with an LDS landing buffer,
+    // 5 register tile specialized per pass structure (compiled at run time on
+    // first use and cached for the process).
     int tile_kernel = 0;
     // Remove adjacent inverse pairs (RedundantGateElimination) before fusion
     // and planning. Off by default: the reference's simulator applies fusion only.

@@ -115,7 +115,10 @@ enum {
     LQCS_TILE_LDS = 1,
     LQCS_TILE_REGISTER = 2,
     LQCS_TILE_STREAM = 3,    /* register tile, next tile prefetched into registers */
-    LQCS_TILE_LAND = 4       /* register tile, next tile prefetched into LDS by DMA */
+    LQCS_TILE_LAND = 4,      /* register tile, next tile prefetched into LDS by DMA */
+    LQCS_TILE_SPECIALIZED = 5 /* register tile, one kernel per pass structure compiled at run time
+                                 (hipRTC) and cached for the process; 12-qubit tiles, other
+                                 passes run on the register tile */
 };
 
 typedef struct lqcs_exec_options {
@@ -157,10 +160,26 @@ lqcs_status lqcs_plan_describe(uint32_t n_qubits, const lqcs_program* prog, cons
                                uint64_t* n_steps, uint64_t* ops_per_step, uint64_t* tile_masks,
                                uint64_t* phases_per_step, uint64_t max_steps);
 
+/* Specialized kernel of one planned step of lqcs_state_apply (no device
+ * needed; no counterpart in the reference): the kernel text the engine
+ * compiles for it (after the device templates of spec_pass.hpp) and the op
+ * words it reads, as (re, im) pairs of doubles. text_len is 0 when the step
+ * runs on another kernel. Null text or words (or too small a words buffer)
+ * skip the copy; text receives at most text_max - 1 characters and a NUL.
+ * opts selects the plan as for lqcs_state_apply. */
+lqcs_status lqcs_plan_specialized_pass(uint32_t n_qubits, const lqcs_program* prog, const lqcs_exec_options* opts,
+                                       uint64_t step, char* text, size_t text_max, size_t* text_len, double* words,
+                                       uint64_t words_max, uint64_t* n_words);
+
+/* Number of specialized pass kernels compiled and loaded in this process
+ * (cached by kernel text; no counterpart in the reference). */
+lqcs_status lqcs_specialized_kernels(uint64_t* cached);
+
 /* Kernel used for tiled passes: 1 = register-resident tile, one workgroup
  * per tile (default); 2 = register tile, streaming (each workgroup walks
  * tiles and prefetches the next one into registers); 3 = register tile,
  * landing (one workgroup per CU, the next tile prefetched into LDS by DMA);
+ * 4 = register tile specialized per pass structure (LQCS_TILE_SPECIALIZED);
  * 0 = LDS tile. A switch for comparisons; results agree to rounding. */
 lqcs_status lqcs_set_tile_kernel(int kernel);
 

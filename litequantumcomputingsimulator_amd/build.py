@@ -49,18 +49,38 @@ def _compile_all(jobs: list[list[str]]) -> None:
             f.result()
 
 
+GEN = os.path.join(ROOT, "_build", "gen")
+
+
+def embed_spec_header() -> None:
+    """Writes the specialized-pass device templates (spec_pass.hpp) as a raw
+    string literal that spec_pass.hip compiles in and hands to hipRTC."""
+    os.makedirs(GEN, exist_ok=True)
+    with open(os.path.join(CSRC, "device", "spec_pass.hpp")) as f:
+        text = f.read()
+    if ")LQCSHDR\"" in text:
+        raise RuntimeError("spec_pass.hpp contains the raw-string delimiter")
+    out = os.path.join(GEN, "spec_pass_hpp.inc")
+    body = 'R"LQCSHDR(' + text + ')LQCSHDR"\n'
+    if not os.path.exists(out) or open(out).read() != body:
+        with open(out, "w") as f:
+            f.write(body)
+
+
 def build_device() -> str:
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIB, exist_ok=True)
+    embed_spec_header()
     srcs = sorted(glob.glob(os.path.join(CSRC, "device", "*.hip")))
     objs, jobs = [], []
     for s in srcs:
         o = os.path.join(OBJ, os.path.basename(s) + ".o")
         objs.append(o)
-        jobs.append([HIPCC, *HIP_FLAGS, "-c", s, "-o", o])
+        jobs.append([HIPCC, *HIP_FLAGS, f"-I{GEN}", "-c", s, "-o", o])
     _compile_all(jobs)
     out = os.path.join(LIB, "liblqcs_device.so")
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", out + ".tmp"])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-L/opt/rocm/lib", "-lhiprtc",
+          "-Wl,-rpath,/opt/rocm/lib", "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out
 

@@ -486,16 +486,22 @@ Plan plan_greedy(const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uin
     return plan;
 }
 
-StagedPlan stage_plan(const Plan& plan, uint32_t nq, uint32_t L) {
-    // Register-tile programs where possible, LDS tile programs otherwise.
+StagedPlan stage_plan(const Plan& plan, uint32_t nq, uint32_t L, uint32_t batch) {
+    // Register-tile programs where possible, LDS tile programs otherwise;
+    // specialized kernels for the register-tile programs when selected.
     StagedPlan sp;
+    std::vector<std::string> spec_src;
+    const bool nt = (uint64_t(batch) << nq) * sizeof(amp_t) >= (uint64_t{1} << 30);
     for (const auto& st : plan.steps) {
+        spec_src.emplace_back();
         if (!st.tiled) {
             sp.where.push_back({0, 0, 0, 0});
             continue;
         }
         RTileProgram rp;
         if (g_use_rtile && build_rtile(st.tile_ops, L, st.smask, nq, rp)) {
+            if (g_rtile_spec && !g_rtile_stream && !g_rtile_land && L == 12)
+                spec_src.back() = spec_source(rp, nq, st.smask, nt);
             sp.where.push_back({2, sp.rph.size(), rp.phases.size(), sp.rwords.size()});
             sp.rph.insert(sp.rph.end(), rp.phases.begin(), rp.phases.end());
             sp.rwords.insert(sp.rwords.end(), rp.words.begin(), rp.words.end());
@@ -504,6 +510,7 @@ StagedPlan stage_plan(const Plan& plan, uint32_t nq, uint32_t L) {
             sp.flat.insert(sp.flat.end(), st.tile_ops.begin(), st.tile_ops.end());
         }
     }
+    if (g_rtile_spec) sp.spec = spec_kernels(spec_src);
     return sp;
 }
 
@@ -519,7 +526,9 @@ void launch_staged(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan, const st
             LQCS_HIP(hipEventRecord(e0, stream()));
         }
         const StagedPlan::Where& w = sp.where[i];
-        if (w.kind == 2) {
+        if (w.kind == 2 && i < sp.spec.size() && sp.spec[i]) {
+            launch_spec(sp.spec[i], s, nq, d_rwords + w.words, batch);
+        } else if (w.kind == 2) {
             launch_rtile(s, nq, L, st.smask, d_ph + w.first, static_cast<uint32_t>(w.count),
                          d_rwords ? d_rwords + w.words : nullptr, batch);
         } else if (w.kind == 1) {
@@ -536,7 +545,7 @@ void launch_staged(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan, const st
 
 void execute_plan(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan,
                   const std::vector<DecodedOp>& ops, uint32_t batch) {
-    const StagedPlan sp = stage_plan(plan, nq, L);
+    const StagedPlan sp = stage_plan(plan, nq, L, batch);
     const TileOp* d_ops = sp.flat.empty() ? nullptr : stage(sp.flat.data(), sp.flat.size());
     const RPhase* d_ph = sp.rph.empty() ? nullptr : stage(sp.rph.data(), sp.rph.size());
     const amp_t* d_rwords = sp.rwords.empty() ? nullptr : stage(sp.rwords.data(), sp.rwords.size());
@@ -587,24 +596,28 @@ bool rtile_enabled() { return g_use_rtile; }
 // The tile kernel of one call (lqcs_exec_options.tile_kernel), restored on exit.
 class KernelScope {
 public:
-    explicit KernelScope(const lqcs_exec_options* o) : use_(g_use_rtile), stream_(g_rtile_stream), land_(g_rtile_land) {
+    explicit KernelScope(const lqcs_exec_options* o)
+        : use_(g_use_rtile), stream_(g_rtile_stream), land_(g_rtile_land), spec_(g_rtile_spec) {
         const int k = o ? o->tile_kernel : LQCS_TILE_DEFAULT;
         if (k == LQCS_TILE_DEFAULT) return;
-        if (k < LQCS_TILE_LDS || k > LQCS_TILE_LAND) throw Error(LQCS_ERR_INVALID_ARGUMENT, "unknown tile kernel");
+        if (k < LQCS_TILE_LDS || k > LQCS_TILE_SPECIALIZED)
+            throw Error(LQCS_ERR_INVALID_ARGUMENT, "unknown tile kernel");
         g_use_rtile = k != LQCS_TILE_LDS;
         g_rtile_stream = k == LQCS_TILE_STREAM;
         g_rtile_land = k == LQCS_TILE_LAND;
+        g_rtile_spec = k == LQCS_TILE_SPECIALIZED;
     }
     ~KernelScope() {
         g_use_rtile = use_;
         g_rtile_stream = stream_;
         g_rtile_land = land_;
+        g_rtile_spec = spec_;
     }
     KernelScope(const KernelScope&) = delete;
     KernelScope& operator=(const KernelScope&) = delete;
 
 private:
-    bool use_, stream_, land_;
+    bool use_, stream_, land_, spec_;
 };
 
 Plan make_plan(int kind, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t low) {
@@ -710,12 +723,14 @@ lqcs_status lqcs_dev_synchronize(void) {
 
 lqcs_status lqcs_set_tile_kernel(int kernel) {
     return guarded([&] {
-        if (kernel < 0 || kernel > 3)
+        if (kernel < 0 || kernel > 4)
             throw Error(LQCS_ERR_INVALID_ARGUMENT,
-                        "tile kernel must be 0 (LDS), 1 (register), 2 (register, streaming) or 3 (register, landing)");
+                        "tile kernel must be 0 (LDS), 1 (register), 2 (register, streaming), 3 (register, landing) "
+                        "or 4 (register, specialized per pass)");
         g_use_rtile = kernel >= 1;
         g_rtile_stream = kernel == 2;
         g_rtile_land = kernel == 3;
+        g_rtile_spec = kernel == 4;
     });
 }
 
@@ -985,6 +1000,47 @@ lqcs_status lqcs_plan_describe(uint32_t n_qubits, const lqcs_program* prog, cons
                                          : 0;
             }
         }
+    });
+}
+
+lqcs_status lqcs_specialized_kernels(uint64_t* cached) {
+    return guarded([&] {
+        need(cached, "cached");
+        *cached = spec_cache_size();
+    });
+}
+
+lqcs_status lqcs_plan_specialized_pass(uint32_t n_qubits, const lqcs_program* prog, const lqcs_exec_options* opts,
+                                       uint64_t step, char* text, size_t text_max, size_t* text_len, double* words,
+                                       uint64_t words_max, uint64_t* n_words) {
+    return guarded([&] {
+        need(prog, "program");
+        need(text_len, "text_len");
+        need(n_words, "n_words");
+        if (n_qubits == 0 || n_qubits > LQCS_DEVICE_MAX_QUBITS) throw Error(LQCS_ERR_INVALID_ARGUMENT, "bad width");
+        std::vector<DecodedOp> ops;
+        for (uint64_t k = 0; k < prog->n_ops; ++k) ops.push_back(decode(*prog, k, n_qubits));
+        const KernelScope ks(opts);
+        uint32_t L = 0;
+        const Plan plan = choose_plan(opts, ops, n_qubits, 1, L);
+        if (step >= plan.steps.size()) throw Error(LQCS_ERR_OUT_OF_RANGE, "step beyond the plan");
+        const PlanStep& st = plan.steps[step];
+        std::string src;
+        RTileProgram rp;
+        if (st.tiled && g_use_rtile && L == 12 && build_rtile(st.tile_ops, L, st.smask, n_qubits, rp)) {
+            const bool nt = (uint64_t{1} << n_qubits) * sizeof(amp_t) >= (uint64_t{1} << 30);
+            src = spec_source(rp, n_qubits, st.smask, nt);
+        }
+        if (src.empty()) rp.words.clear();
+        *text_len = src.size();
+        *n_words = rp.words.size();
+        if (text && text_max) {
+            const size_t n = std::min(text_max - 1, src.size());
+            std::memcpy(text, src.data(), n);
+            text[n] = '\0';
+        }
+        if (words && words_max >= rp.words.size())
+            std::memcpy(words, rp.words.data(), rp.words.size() * sizeof(amp_t));
     });
 }
 

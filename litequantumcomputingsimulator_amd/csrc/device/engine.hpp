@@ -5,6 +5,7 @@
 #include "tile.hpp"
 
 #include <functional>
+#include <string>
 #include <vector>
 
 #include "../../../include/lqcs_device.h"
@@ -76,8 +77,13 @@ struct StagedPlan {
     std::vector<RPhase> rph;
     std::vector<amp_t> rwords;
     std::vector<Where> where;
+    // Per step: the specialized kernel of its register-tile program (null:
+    // the interpreting kernel runs it).
+    std::vector<hipFunction_t> spec;
 };
-StagedPlan stage_plan(const Plan& plan, uint32_t nq, uint32_t L);
+// `batch` registers are passed over per launch (it picks non-temporal
+// accesses for specialized passes over 1 GiB or more).
+StagedPlan stage_plan(const Plan& plan, uint32_t nq, uint32_t L, uint32_t batch = 1);
 // Launches every step of a staged plan whose programs are at d_ops / d_ph /
 // d_rwords on the device; `timed` brackets each launch with profile events.
 void launch_staged(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan, const std::vector<DecodedOp>& ops,
@@ -99,6 +105,15 @@ Plan choose_plan(const lqcs_exec_options* opts, const std::vector<DecodedOp>& op
 void step_features(const PlanStep& st, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t batch,
                    double* f);
 void last_plan_choice(int* plan, int* low, double* predicted_ms);
+// Specialized register-tile passes (spec_pass.hip): on for the calls that
+// select them; kernel text of a program (empty: a form the generator does not
+// handle, or a tile width other than 12); the kernels of a list of sources,
+// compiled on first use; their launch.
+extern bool g_rtile_spec;
+std::string spec_source(const RTileProgram& rp, uint32_t nq, uint64_t smask, bool nt);
+std::vector<hipFunction_t> spec_kernels(const std::vector<std::string>& sources);
+void launch_spec(hipFunction_t f, amp_t* s, uint32_t nq, const amp_t* d_words, uint32_t batch);
+size_t spec_cache_size();
 bool rtile_enabled();
 // Installed cost model (mlp.hip).
 bool cost_model_installed();

@@ -1,0 +1,434 @@
+// Specialized register-tile passes: kernel text generated per pass structure
+// and compiled at run time with hipRTC.
+//
+// The interpreting kernel (rtile.hip) decodes every op of a pass at run time
+// and keeps two register sets, because every case of its op dispatch must
+// leave the tile in the same registers. A specialized pass knows each op's
+// kind and register bits when it is compiled: it calls the op templates of
+// spec_pass.hpp in program order on one register set updated in place. That
+// removes the dispatch (scalar decode and a branch chain per op) and about 40
+// VGPRs, so four waves per SIMD fit instead of three.
+//
+// In-place transposes. The tile moves through 32 KiB of LDS in two halves,
+// split by a tile bit h that is a thread bit of both layouts. A thread writes
+// its registers in the round of its old half and reads in the round of its
+// new half; in place, no thread may read before it has written, so h must sit
+// at the same thread-id bit in both layouts. Here every layout orders its
+// thread bits so that the split bit of the transpose into it and the one out
+// of it sit at thread-id bits 6 and 7 (alternating per transpose, so the two
+// never collide); the other thread bits fill the remaining positions in
+// ascending order, which keeps the lowest tile qubits on the lanes of the
+// load and store layouts.
+//
+// The generated kernel reads its matrix entries from the pass's op words (the
+// same words the interpreting kernel reads), so passes that differ only in
+// angles share one kernel. Kernels are cached by source text for the life of
+// the process; distinct sources of one plan compile in parallel host threads.
+#include "common.hpp"
+#include "engine.hpp"
+#include "spec_pass.hpp"
+#include "tile.hpp"
+
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cstring>
+#include <future>
+#include <memory>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace lqcs_dev {
+
+// Specialized passes for the calls that ask for them (tile kernel option).
+bool g_rtile_spec = false;
+
+namespace {
+
+// The device templates, embedded at build time (spec_pass_hpp.inc holds the
+// text of spec_pass.hpp as a raw string literal).
+const char* const kSpecHeader =
+#include "spec_pass_hpp.inc"
+    ;
+
+constexpr uint32_t kL = 12;   // tile bits
+constexpr uint32_t kT = 8;    // thread bits (256 threads)
+
+std::string hex(uint64_t v, bool wide = true) {
+    char buf[32];
+    std::snprintf(buf, sizeof buf, wide ? "0x%llxull" : "0x%llxu", static_cast<unsigned long long>(v));
+    return buf;
+}
+
+std::string dec(uint64_t v) { return std::to_string(v); }
+
+// Expression for sum_i ((x >> i) & 1) << dst[i] over the entries of dst
+// (dst[i] < 0: bit i unused), grouping runs of consecutive bits.
+std::string scatter_expr(const std::string& x, const std::vector<int>& dst, bool wide) {
+    std::string out;
+    const std::string cast = wide ? "(sp_u64)" : "";
+    for (size_t i = 0; i < dst.size();) {
+        if (dst[i] < 0) {
+            ++i;
+            continue;
+        }
+        size_t j = i + 1;
+        while (j < dst.size() && dst[j] == dst[j - 1] + 1) ++j;
+        const uint64_t mask = ((uint64_t{1} << (j - i)) - 1) << i;
+        const int shift = dst[i] - static_cast<int>(i);
+        std::string term = "(" + cast + "(" + x + " & " + hex(mask, wide) + ")";
+        if (shift > 0) term += " << " + dec(static_cast<uint64_t>(shift));
+        if (shift < 0) term += " >> " + dec(static_cast<uint64_t>(-shift));
+        term += ")";
+        out += (out.empty() ? "" : " | ") + term;
+        i = j;
+    }
+    return out.empty() ? std::string(wide ? "(sp_u64)0" : "0u") : out;
+}
+
+// Register bits (ascending) of a phase, read off its register tile indices.
+void reg_bits(const RPhase& ph, int (&rb)[4]) {
+    for (int i = 0; i < 4; ++i) rb[i] = __builtin_ctz(ph.jr[1 << i]);
+}
+
+struct Layout {
+    int rb[4];
+    uint32_t tmask;
+    int tpos[kT];        // tile bit at each thread-id bit
+    uint32_t jr[16];     // tile index bits of register r
+};
+
+// The pass's layouts with thread-bit orders for in-place transposes, and the
+// split bit of each transpose (split[k]: into layout k; -1 none).
+bool plan_layouts(const RTileProgram& rp, std::vector<Layout>& lay, std::vector<int>& split, std::vector<int>& spos) {
+    const size_t P = rp.phases.size();
+    lay.assign(P, Layout{});
+    split.assign(P, -1);
+    spos.assign(P, -1);
+    const uint32_t all = (1u << kL) - 1;
+    for (size_t k = 0; k < P; ++k) {
+        reg_bits(rp.phases[k], lay[k].rb);
+        uint32_t rm = 0;
+        for (int i = 0; i < 4; ++i) rm |= 1u << lay[k].rb[i];
+        lay[k].tmask = all & ~rm;
+        for (int r = 0; r < 16; ++r) lay[k].jr[r] = rp.phases[k].jr[r];
+    }
+    int n_t = 0;
+    for (size_t k = 1; k < P; ++k) {
+        if (!rp.phases[k].transpose) continue;
+        const uint32_t common = lay[k - 1].tmask & lay[k].tmask;
+        uint32_t cand = common;
+        if (split[k - 1] >= 0) cand &= ~(1u << split[k - 1]);  // pinned at the other position in layout k-1
+        // Keep the low tile qubits on the lanes of the load and store layouts.
+        const bool edge = (k - 1 == 0) || (k == P - 1);
+        uint32_t pref = cand;
+        if (edge) {
+            for (size_t e : {size_t{0}, P - 1}) {
+                if (e != k - 1 && e != k) continue;
+                uint32_t low = 0, t = lay[e].tmask;
+                for (int c = 0; c < 6 && t; ++c) {
+                    low |= t & (~t + 1);
+                    t &= t - 1;
+                }
+                pref &= ~low;
+            }
+        }
+        const uint32_t pick_from = pref ? pref : cand;
+        if (!pick_from) return false;
+        split[k] = 31 - __builtin_clz(pick_from);
+        spos[k] = (n_t++ % 2 == 0) ? 7 : 6;
+    }
+    // Thread-bit orders: pinned split bits, the rest ascending.
+    for (size_t k = 0; k < P; ++k) {
+        int fixed[kT];
+        for (uint32_t p = 0; p < kT; ++p) fixed[p] = -1;
+        uint32_t used = 0;
+        auto pin = [&](int bit, int pos) {
+            fixed[pos] = bit;
+            used |= 1u << bit;
+        };
+        // A layout without its own transpose repeats the previous one.
+        if (k > 0 && !rp.phases[k].transpose) {
+            std::memcpy(lay[k].tpos, lay[k - 1].tpos, sizeof lay[k].tpos);
+            continue;
+        }
+        if (split[k] >= 0) pin(split[k], spos[k]);
+        for (size_t n = k + 1; n < P; ++n) {  // the next transpose out of this layout
+            if (!rp.phases[n].transpose) continue;
+            pin(split[n], spos[n]);
+            break;
+        }
+        uint32_t rest = lay[k].tmask & ~used;
+        for (uint32_t p = 0; p < kT; ++p) {
+            if (fixed[p] >= 0) {
+                lay[k].tpos[p] = fixed[p];
+                continue;
+            }
+            if (!rest) return false;
+            lay[k].tpos[p] = __builtin_ctz(rest);
+            rest &= rest - 1;
+        }
+    }
+    return true;
+}
+
+std::string jt_expr(const Layout& l) {
+    std::vector<int> dst(kT);
+    for (uint32_t p = 0; p < kT; ++p) dst[p] = l.tpos[p];
+    return scatter_expr("tid", dst, false);
+}
+
+// Global index bits of the thread in layout l (tile bit b is qubit gq[b]).
+std::string tglobal_expr(const Layout& l, const int (&gq)[kL]) {
+    std::vector<int> dst(kT);
+    for (uint32_t p = 0; p < kT; ++p) dst[p] = gq[l.tpos[p]];
+    return scatter_expr("tid", dst, true);
+}
+
+uint64_t reg_global(const Layout& l, int r, const int (&gq)[kL]) {
+    uint64_t g = 0;
+    for (int i = 0; i < 4; ++i)
+        if (r & (1 << i)) g |= uint64_t{1} << gq[l.rb[i]];
+    return g;
+}
+
+std::string jr_array(const Layout& l, const std::string& name) {
+    std::string s = "    const sp_u32 " + name + "[16] = {";
+    for (int r = 0; r < 16; ++r) s += (r ? ", " : "") + dec(l.jr[r]) + "u";
+    return s + "};\n";
+}
+
+// One op of the pass as a call of the templates; empty if not handled.
+bool emit_op(std::string& out, const RegOpHdr& h, uint32_t off, const std::string& jr_name) {
+    const uint32_t kind = h.code & 15u, b0 = (h.code >> 4) & 3u, b1 = (h.code >> 6) & 3u, xr = (h.code >> 8) & 15u,
+                   rctl = (h.code >> 12) & 15u;
+    const std::string M = "w + " + dec(off);
+    const std::string tp = h.tctl ? "((jt & " + dec(h.tctl) + "u) == " + dec(h.tctl) + "u)" : "true";
+    std::string call;
+    switch (kind) {
+        case RK_DENSE1:
+            if (rctl == 0 && h.tctl == 0) call = "sp::d1<" + dec(b0) + ">(v, " + M + ");";
+            else call = "sp::d1_c<" + dec(b0) + ", " + dec(rctl) + ">(v, " + M + ", " + tp + ");";
+            break;
+        case RK_SEL1:
+            if (rctl || h.tctl) return false;
+            if (xr) call = "sp::sel_r<" + dec(b0) + ", " + dec(xr - 1) + ">(v, " + M + ");";
+            else if (h.la) call = "sp::sel_l<" + dec(b0) + ">(v, " + M + ", (jt & " + dec(h.la) + "u) != 0);";
+            else call = "sp::d1<" + dec(b0) + ">(v, " + M + " + ((tile & " + dec(h.ga) + "u) ? 4 : 0));";
+            break;
+        case RK_X:
+            call = "sp::x_c<" + dec(b0) + ", " + dec(rctl) + ", " + (h.tctl ? "false" : "true") + ">(v, " + tp + ");";
+            break;
+        case RK_DENSE2:
+        case RK_DENSE2R:
+            call = "sp::d2<" + dec(b0) + ", " + dec(b1) + ", " + dec(rctl) + ", " + (kind == RK_DENSE2R ? "true" : "false") +
+                   ">(v, " + M + ", " + tp + ");";
+            break;
+        case RK_DIAG1:
+            if (xr) {
+                call = "sp::dg1_r<" + dec(xr - 1) + ", " + (b1 ? "true" : "false") + ", " + dec(rctl) + ">(v, " + M + ", " +
+                       tp + ");";
+            } else {
+                const std::string s = h.la ? "((jt & " + dec(h.la) + "u) != 0)" : "((tile & " + dec(h.ga) + "u) != 0)";
+                call = "sp::dg1_u<" + dec(rctl) + ">(v, " + M + ", " + s + ", " + tp + ");";
+            }
+            break;
+        case RK_DIAG: {
+            const std::string g = "(((tile & " + dec(h.ga) + "u) ? 1u : 0u) | ((tile & " + dec(h.gb) + "u) ? 2u : 0u))";
+            call = "sp::dg_gen<" + dec(rctl) + ">(v, " + M + ", jt, " + jr_name + ", " + dec(h.la) + "u, " + dec(h.lb) +
+                   "u, " + g + ", " + tp + ");";
+            break;
+        }
+        case RK_PAULI: {
+            const std::string gp = "(__popc(tile & " + dec(h.ga) + "u) & 1u)";
+            if (xr == 0) call = "sp::pauli_z(v, " + M + ", jt, " + jr_name + ", " + dec(h.la) + "u, " + gp + ");";
+            else call = "sp::pauli_x<" + dec(xr) + ">(v, " + M + ", jt, " + jr_name + ", " + dec(h.la) + "u, " + gp + ");";
+            break;
+        }
+        default:
+            return false;
+    }
+    std::string cond;
+    if (h.gctrl) cond = "(tile & " + dec(h.gctrl) + "u) == " + dec(h.gctrl) + "u";
+    if (h.member) {
+        const uint32_t mid = (h.member & 0x7fffffffu) - 1;
+        const std::string m = (h.member >> 31) ? "blockIdx.y == " + dec(mid) + "u" : "blockIdx.y != " + dec(mid) + "u";
+        cond = cond.empty() ? m : cond + " && " + m;
+    }
+    if (cond.empty()) out += "    " + call + "\n";
+    else out += "    if (" + cond + ") { " + call + " }\n";
+    return true;
+}
+
+}  // namespace
+
+std::string spec_source(const RTileProgram& rp, uint32_t nq, uint64_t smask, bool nt) {
+    if (rp.phases.empty() || nq < kL) return {};
+    std::vector<Layout> lay;
+    std::vector<int> split, spos;
+    if (!plan_layouts(rp, lay, split, spos)) return {};
+    int gq[kL];
+    {
+        int t = 0;
+        for (int q = 0; q < 64 && t < static_cast<int>(kL); ++q)
+            if (smask & (uint64_t{1} << q)) gq[t++] = q;
+        if (t != static_cast<int>(kL)) return {};
+    }
+    // Tile number -> base index: its bits on the qubits outside the tile set.
+    std::vector<int> outside_dst;
+    for (uint32_t q = 0; q < nq; ++q)
+        if (!(smask & (uint64_t{1} << q))) outside_dst.push_back(static_cast<int>(q));
+
+    const size_t P = rp.phases.size();
+    const std::string NT = nt ? "true" : "false";
+    std::string s;
+    s += "extern \"C\" __global__ void __launch_bounds__(256) lqcs_pass(sp_amp* __restrict__ s, const sp_amp* __restrict__ w) {\n";
+    s += "    __shared__ sp_amp lds[2048];\n";
+    s += "    char* const lb = reinterpret_cast<char*>(lds);\n";
+    s += "    const sp_u32 tid = threadIdx.x;\n";
+    s += "    const sp_u32 tile = sp::xcd_tile(blockIdx.x, gridDim.x);\n";
+    s += "    sp_amp* const b = s + ((sp_u64)blockIdx.y << " + dec(nq) + ") + (" +
+         scatter_expr("(sp_u64)tile", outside_dst, true) + ");\n";
+    s += "    sp::Regs v;\n";
+    s += "    {\n        const sp_amp* p = b + (" + tglobal_expr(lay[0], gq) + ");\n";
+    for (int r = 0; r < 16; ++r) s += "        v[" + dec(r) + "] = sp::ld<" + NT + ">(p + " + hex(reg_global(lay[0], r, gq)) + ");\n";
+    s += "    }\n";
+    bool lds_used = false;
+    for (size_t k = 0; k < P; ++k) {
+        const RPhase& ph = rp.phases[k];
+        s += "    {  // phase " + dec(k) + "\n";
+        if (k > 0 && ph.transpose) {
+            const int h = split[k];
+            const uint32_t pbit = 1u << spos[k];
+            s += "        const sp_u32 sp0 = sp::slot<" + dec(h) + ">(" + jt_expr(lay[k - 1]) + ");\n";
+            s += "        const sp_u32 sc0 = sp::slot<" + dec(h) + ">(" + jt_expr(lay[k]) + ");\n";
+            for (int half = 0; half < 2; ++half) {
+                const std::string cond = half == 0 ? "!(tid & " + dec(pbit) + "u)" : "(tid & " + dec(pbit) + "u)";
+                if (lds_used) s += "        __syncthreads();\n";
+                s += "        if (" + cond + ") {\n";
+                for (int r = 0; r < 16; ++r)
+                    s += "            *reinterpret_cast<sp_amp*>(lb + (sp0 ^ " + dec(rtile_slot(lay[k - 1].jr[r], h) << 4) +
+                         "u)) = v[" + dec(r) + "];\n";
+                s += "        }\n        __syncthreads();\n        if (" + cond + ") {\n";
+                for (int r = 0; r < 16; ++r)
+                    s += "            v[" + dec(r) + "] = *reinterpret_cast<const sp_amp*>(lb + (sc0 ^ " +
+                         dec(rtile_slot(lay[k].jr[r], h) << 4) + "u));\n";
+                s += "        }\n";
+                lds_used = true;
+            }
+        }
+        s += "        const sp_u32 jt = " + jt_expr(lay[k]) + ";\n";
+        s += "        (void)jt;\n";
+        std::string ops;
+        const std::string jr_name = "jr" + dec(k);
+        bool need_jr = false;
+        uint32_t off = ph.op_off;
+        for (uint32_t i = 0; i < ph.n_ops; ++i) {
+            RegOpHdr h;
+            std::memcpy(&h, &rp.words[off], sizeof h);
+            const uint32_t kind = h.code & 15u;
+            if (kind == RK_DIAG || kind == RK_PAULI) need_jr = true;
+            if (!emit_op(ops, h, off + 2, jr_name)) return {};
+            off += 2 + ((h.code >> 16) & 0xFFFu);
+        }
+        if (need_jr) s += jr_array(lay[k], jr_name);
+        s += ops;
+        s += "    }\n";
+    }
+    s += "    {\n        sp_amp* p = b + (" + tglobal_expr(lay[P - 1], gq) + ");\n";
+    for (int r = 0; r < 16; ++r)
+        s += "        sp::st<" + NT + ">(p + " + hex(reg_global(lay[P - 1], r, gq)) + ", v[" + dec(r) + "]);\n";
+    s += "    }\n}\n";
+    return s;
+}
+
+namespace {
+
+struct SpecCache {
+    std::unordered_map<std::string, hipFunction_t> fn;
+    std::vector<hipModule_t> modules;
+};
+SpecCache& cache() {
+    static SpecCache c;
+    return c;
+}
+
+// hipRTC compile of one pass; the code object, or an exception with the log.
+std::vector<char> compile(const std::string& body) {
+    const std::string src = std::string(kSpecHeader) + "\n" + body;
+    hiprtcProgram prog = nullptr;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "lqcs_pass.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+        throw Error(LQCS_ERR_DEVICE, "hipRTC: cannot create a program");
+    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17",
+                          // the same code-generation switch as every HIP source of the engine
+                          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    const hiprtcResult r = hiprtcCompileProgram(prog, static_cast<int>(sizeof opts / sizeof opts[0]), opts);
+    std::vector<char> code;
+    if (r == HIPRTC_SUCCESS) {
+        size_t n = 0;
+        hiprtcGetCodeSize(prog, &n);
+        code.resize(n);
+        hiprtcGetCode(prog, code.data());
+    } else {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(prog, &n);
+        std::string log(n, '\0');
+        if (n) hiprtcGetProgramLog(prog, &log[0]);
+        hiprtcDestroyProgram(&prog);
+        throw Error(LQCS_ERR_DEVICE, "hipRTC compile of a specialized pass failed: " + log.substr(0, 2000));
+    }
+    hiprtcDestroyProgram(&prog);
+    return code;
+}
+
+}  // namespace
+
+std::vector<hipFunction_t> spec_kernels(const std::vector<std::string>& sources) {
+    SpecCache& c = cache();
+    std::vector<std::string> todo;
+    for (const std::string& s : sources)
+        if (!s.empty() && !c.fn.count(s) && std::find(todo.begin(), todo.end(), s) == todo.end()) todo.push_back(s);
+    if (!todo.empty()) {
+        const size_t workers = std::max<size_t>(1, std::min<size_t>(todo.size(), std::min(16u, std::thread::hardware_concurrency())));
+        std::vector<std::vector<char>> code(todo.size());
+        std::vector<std::string> err(todo.size());
+        std::vector<std::thread> pool;
+        std::atomic<size_t> next{0};
+        for (size_t t = 0; t < workers; ++t)
+            pool.emplace_back([&] {
+                for (size_t i; (i = next.fetch_add(1)) < todo.size();) {
+                    try {
+                        code[i] = compile(todo[i]);
+                    } catch (const std::exception& e) {
+                        err[i] = e.what();
+                    }
+                }
+            });
+        for (auto& th : pool) th.join();
+        for (size_t i = 0; i < todo.size(); ++i) {
+            if (!err[i].empty()) throw Error(LQCS_ERR_DEVICE, err[i]);
+            hipModule_t mod = nullptr;
+            hipFunction_t fn = nullptr;
+            LQCS_HIP(hipModuleLoadData(&mod, code[i].data()));
+            c.modules.push_back(mod);
+            LQCS_HIP(hipModuleGetFunction(&fn, mod, "lqcs_pass"));
+            c.fn.emplace(todo[i], fn);
+        }
+    }
+    std::vector<hipFunction_t> out(sources.size(), nullptr);
+    for (size_t i = 0; i < sources.size(); ++i)
+        if (!sources[i].empty()) out[i] = c.fn.at(sources[i]);
+    return out;
+}
+
+size_t spec_cache_size() { return cache().fn.size(); }
+
+void launch_spec(hipFunction_t f, amp_t* s, uint32_t nq, const amp_t* d_words, uint32_t batch) {
+    const uint64_t tiles = uint64_t{1} << (nq - kL);
+    void* args[] = {&s, &d_words};
+    LQCS_HIP(hipModuleLaunchKernel(f, static_cast<unsigned>(tiles), batch, 1, 256, 1, 1, 0, stream(), args, nullptr));
+}
+
+}  // namespace lqcs_dev

@@ -386,6 +386,21 @@ std::vector<CompiledCircuit::PassInfo> CompiledCircuit::describe(const EngineOpt
     return out;
 }
 
+CompiledCircuit::SpecializedPass CompiledCircuit::specialized_pass(std::uint64_t step, const EngineOptions& engine) const {
+    const lqcs_program prog = prog_->program();
+    const lqcs_exec_options o = exec_options(engine);
+    size_t len = 0;
+    uint64_t nw = 0;
+    check(lqcs_plan_specialized_pass(static_cast<uint32_t>(nq_), &prog, &o, step, nullptr, 0, &len, nullptr, 0, &nw));
+    SpecializedPass out;
+    std::vector<char> text(len + 1);
+    out.words.resize(2 * nw);
+    check(lqcs_plan_specialized_pass(static_cast<uint32_t>(nq_), &prog, &o, step, text.data(), text.size(), &len,
+                                     out.words.data(), nw, &nw));
+    out.text.assign(text.data(), len);
+    return out;
+}
+
 std::vector<double> CompiledCircuit::features(const EngineOptions& engine, std::uint32_t batch) const {
     const lqcs_program prog = prog_->program();
     const lqcs_exec_options o = exec_options(engine);

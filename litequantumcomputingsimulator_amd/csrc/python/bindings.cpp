@@ -57,16 +57,17 @@ int tile_kernel_of(const std::string& s) {
     if (s == "register") return 2;
     if (s == "stream") return 3;
     if (s == "land") return 4;
-    throw std::invalid_argument("tile_kernel must be default, lds, register, stream or land");
+    if (s == "specialized") return 5;
+    throw std::invalid_argument("tile_kernel must be default, lds, register, stream, land or specialized");
 }
 
 py::dict options_dict(const EngineOptions& e) {
     static const char* plans[] = {"gate_by_gate", "greedy", "learned", "auto", "gain"};
-    static const char* kernels[] = {"default", "lds", "register", "stream", "land"};
+    static const char* kernels[] = {"default", "lds", "register", "stream", "land", "specialized"};
     const int p = static_cast<int>(e.plan);
     return py::dict("plan"_a = std::string(p >= 0 && p <= 4 ? plans[p] : "?"), "tile_qubits"_a = e.tile_qubits,
                     "low_qubits"_a = e.low_qubits,
-                    "tile_kernel"_a = std::string(e.tile_kernel >= 0 && e.tile_kernel <= 4 ? kernels[e.tile_kernel] : "?"));
+                    "tile_kernel"_a = std::string(e.tile_kernel >= 0 && e.tile_kernel <= 5 ? kernels[e.tile_kernel] : "?"));
 }
 
 }  // namespace
@@ -542,6 +543,20 @@ PYBIND11_MODULE(_pylqcs, m) {
                  return out;
              },
              "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0)
+        .def("specialized_pass",
+             [](const CompiledCircuit& c, std::uint64_t step, const std::string& plan, int tile_qubits,
+                int low_qubits) {
+                 EngineOptions e;
+                 e.plan = plan_of(plan);
+                 e.tile_qubits = tile_qubits;
+                 e.low_qubits = low_qubits;
+                 const CompiledCircuit::SpecializedPass sp = c.specialized_pass(step, e);
+                 py::array_t<double> words({static_cast<py::ssize_t>(sp.words.size() / 2), py::ssize_t{2}});
+                 if (!sp.words.empty()) std::memcpy(words.mutable_data(), sp.words.data(), sp.words.size() * sizeof(double));
+                 return py::make_tuple(sp.text, words);
+             },
+             "step"_a, "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0,
+             "kernel text and op words of the specialized kernel of one planned pass (no device needed)")
         .def("features",
              [](const CompiledCircuit& c, const std::string& plan, int tile_qubits, int low_qubits,
                 std::uint32_t batch) {
@@ -585,10 +600,17 @@ PYBIND11_MODULE(_pylqcs, m) {
     });
     m.def("profile_enable", &profile_enable, "on"_a);
     m.def("set_tile_kernel", [](const std::string& k) {
-        const int code = k == "land" ? 3 : k == "stream" ? 2 : k == "register" ? 1 : k == "lds" ? 0 : -1;
-        if (code < 0) throw std::invalid_argument("tile kernel must be 'register', 'stream', 'land' or 'lds'");
+        const int code = k == "specialized" ? 4 : k == "land" ? 3 : k == "stream" ? 2 : k == "register" ? 1
+                         : k == "lds" ? 0 : -1;
+        if (code < 0)
+            throw std::invalid_argument("tile kernel must be 'register', 'stream', 'land', 'specialized' or 'lds'");
         if (lqcs_set_tile_kernel(code) != LQCS_OK) throw std::runtime_error(lqcs_dev_last_error());
     }, "kernel"_a);
+    m.def("specialized_kernels_cached", [] {
+        uint64_t n = 0;
+        if (lqcs_specialized_kernels(&n) != LQCS_OK) throw std::runtime_error(lqcs_dev_last_error());
+        return n;
+    }, "specialized pass kernels compiled and loaded in this process");
     m.def("set_dense_mfma_min_targets", [](uint32_t k) {
         if (lqcs_set_dense_mfma_min_targets(k) != LQCS_OK) throw std::invalid_argument(lqcs_dev_last_error());
     }, "min_targets"_a);

@@ -32,6 +32,7 @@ profile_enable = _native.profile_enable
 profile_read = _native.profile_read
 set_tile_kernel = _native.set_tile_kernel
 set_dense_mfma_min_targets = _native.set_dense_mfma_min_targets
+specialized_kernels_cached = _native.specialized_kernels_cached
 profile_read_launches = _native.profile_read_launches
 COST_FEATURES = _native.COST_FEATURES
 mlp_num_params = _native.mlp_num_params
@@ -51,7 +52,7 @@ __all__ = [
     "statevector", "synchronize", "profile_enable", "profile_read", "set_tile_kernel", "ExpectationFunction",
     "expectation", "profile_read_launches", "COST_FEATURES", "mlp_num_params", "mlp_forward", "mlp_loss_grad",
     "mlp_train", "costmodel_install", "costmodel_uninstall", "costmodel_load", "last_plan_choice",
-    "CircuitGraph", "set_dense_mfma_min_targets",
+    "CircuitGraph", "set_dense_mfma_min_targets", "specialized_kernels_cached",
 ]
 
 # The packaged cost model (if trained) steers plan="learned".
