@@ -32,6 +32,8 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 #include <cstring>
 #include <future>
 #include <memory>
@@ -174,10 +176,10 @@ bool plan_layouts(const RTileProgram& rp, std::vector<Layout>& lay, std::vector<
     return true;
 }
 
-std::string jt_expr(const Layout& l) {
+std::string jt_expr(const Layout& l, const char* tid = "t") {
     std::vector<int> dst(kT);
     for (uint32_t p = 0; p < kT; ++p) dst[p] = l.tpos[p];
-    return scatter_expr("tid", dst, false);
+    return scatter_expr(tid, dst, false);
 }
 
 // Global index bits of the thread in layout l (tile bit b is qubit gq[b]).
@@ -194,37 +196,83 @@ uint64_t reg_global(const Layout& l, int r, const int (&gq)[kL]) {
     return g;
 }
 
+// Global loads (or stores) of the tile in layout l. When the thread's part
+// of the byte offset fits 32 bits, each register's address is the tile's
+// base plus a constant (scalar) and one 32-bit per-lane offset, the form the
+// load instructions take directly; otherwise 64-bit per-lane addresses.
+std::string edge_access(const Layout& l, const int (&gq)[kL], bool load, const std::string& NT) {
+    uint64_t tmax = 0;
+    for (uint32_t p = 0; p < kT; ++p) tmax |= uint64_t{1} << gq[l.tpos[p]];
+    std::string s;
+    if ((tmax * sizeof(amp_t)) >> 32 == 0) {
+        std::vector<int> dst(kT);
+        for (uint32_t p = 0; p < kT; ++p) dst[p] = gq[l.tpos[p]] + 4;  // bytes
+        s += "        const sp_u32 o = " + scatter_expr("tid", dst, false) + ";\n";
+        s += "        char* const pb = reinterpret_cast<char*>(b);\n";
+        for (int r = 0; r < 16; ++r) {
+            const std::string a = "reinterpret_cast<sp_amp*>(pb + " + hex(reg_global(l, r, gq) * sizeof(amp_t)) + " + o)";
+            if (load) s += "        v[" + dec(r) + "] = sp::ld<" + NT + ">(" + a + ");\n";
+            else s += "        sp::st<" + NT + ">(" + a + ", v[" + dec(r) + "]);\n";
+        }
+        return s;
+    }
+    s += "        sp_amp* const p = b + (" + tglobal_expr(l, gq) + ");\n";
+    for (int r = 0; r < 16; ++r) {
+        const std::string a = "p + " + hex(reg_global(l, r, gq));
+        if (load) s += "        v[" + dec(r) + "] = sp::ld<" + NT + ">(" + a + ");\n";
+        else s += "        sp::st<" + NT + ">(" + a + ", v[" + dec(r) + "]);\n";
+    }
+    return s;
+}
+
 std::string jr_array(const Layout& l, const std::string& name) {
     std::string s = "    const sp_u32 " + name + "[16] = {";
     for (int r = 0; r < 16; ++r) s += (r ? ", " : "") + dec(l.jr[r]) + "u";
     return s + "};\n";
 }
 
-// One op of the pass as a call of the templates; empty if not handled.
-bool emit_op(std::string& out, const RegOpHdr& h, uint32_t off, const std::string& jr_name) {
+// Flags of n matrix entries, two bits each: bit 0, the imaginary part is
+// exactly zero; bit 1, the entry is exactly zero.
+uint32_t entry_flags(const amp_t* m, int n) {
+    uint32_t f = 0;
+    for (int i = 0; i < n; ++i) {
+        const uint32_t e = (m[i].y == 0.0 ? 1u : 0u) | (m[i].x == 0.0 && m[i].y == 0.0 ? 2u : 0u);
+        f |= e << (2 * i);
+    }
+    return f;
+}
+
+// One op of the pass as a call of the templates (m: its matrix entries);
+// false if not handled.
+bool emit_op(std::string& out, const RegOpHdr& h, uint32_t off, const amp_t* m, const std::string& jr_name) {
     const uint32_t kind = h.code & 15u, b0 = (h.code >> 4) & 3u, b1 = (h.code >> 6) & 3u, xr = (h.code >> 8) & 15u,
                    rctl = (h.code >> 12) & 15u;
     const std::string M = "w + " + dec(off);
     const std::string tp = h.tctl ? "((jt & " + dec(h.tctl) + "u) == " + dec(h.tctl) + "u)" : "true";
     std::string call;
     switch (kind) {
-        case RK_DENSE1:
-            if (rctl == 0 && h.tctl == 0) call = "sp::d1<" + dec(b0) + ">(v, " + M + ");";
-            else call = "sp::d1_c<" + dec(b0) + ", " + dec(rctl) + ">(v, " + M + ", " + tp + ");";
+        case RK_DENSE1: {
+            const std::string F = dec(entry_flags(m, 4)) + "u";
+            if (rctl == 0 && h.tctl == 0) call = "sp::d1<" + dec(b0) + ", " + F + ">(v, " + M + ");";
+            else call = "sp::d1_c<" + dec(b0) + ", " + dec(rctl) + ", " + F + ">(v, " + M + ", " + tp + ");";
             break;
-        case RK_SEL1:
+        }
+        case RK_SEL1: {
             if (rctl || h.tctl) return false;
-            if (xr) call = "sp::sel_r<" + dec(b0) + ", " + dec(xr - 1) + ">(v, " + M + ");";
-            else if (h.la) call = "sp::sel_l<" + dec(b0) + ">(v, " + M + ", (jt & " + dec(h.la) + "u) != 0);";
-            else call = "sp::d1<" + dec(b0) + ">(v, " + M + " + ((tile & " + dec(h.ga) + "u) ? 4 : 0));";
+            const uint32_t f0 = entry_flags(m, 4), f1 = entry_flags(m + 4, 4);
+            const std::string both = dec(f0 & f1) + "u";
+            if (xr) call = "sp::sel_r<" + dec(b0) + ", " + dec(xr - 1) + ", " + dec(f0 | (f1 << 8)) + "u>(v, " + M + ");";
+            else if (h.la) call = "sp::sel_l<" + dec(b0) + ", " + both + ">(v, " + M + ", (jt & " + dec(h.la) + "u) != 0);";
+            else call = "sp::d1<" + dec(b0) + ", " + both + ">(v, " + M + " + ((tile & " + dec(h.ga) + "u) ? 4 : 0));";
             break;
+        }
         case RK_X:
             call = "sp::x_c<" + dec(b0) + ", " + dec(rctl) + ", " + (h.tctl ? "false" : "true") + ">(v, " + tp + ");";
             break;
         case RK_DENSE2:
         case RK_DENSE2R:
-            call = "sp::d2<" + dec(b0) + ", " + dec(b1) + ", " + dec(rctl) + ", " + (kind == RK_DENSE2R ? "true" : "false") +
-                   ">(v, " + M + ", " + tp + ");";
+            call = "sp::d2<" + dec(b0) + ", " + dec(b1) + ", " + dec(rctl) + ", " + dec(entry_flags(m, 16)) + "u>(v, " + M +
+                   ", " + tp + ");";
             break;
         case RK_DIAG1:
             if (xr) {
@@ -262,6 +310,16 @@ bool emit_op(std::string& out, const RegOpHdr& h, uint32_t off, const std::strin
     return true;
 }
 
+// Waves per SIMD the compiler is asked to fit (LQCS_SPEC_WAVES, 4 or 5;
+// unset: the compiler's own choice under 256-thread workgroups).
+std::string waves_attr() {
+    const char* e = std::getenv("LQCS_SPEC_WAVES");
+    if (!e) return "";
+    const int w = std::atoi(e);
+    if (w < 4 || w > 8) return "";
+    return " __attribute__((amdgpu_waves_per_eu(" + std::to_string(w) + ", 8)))";
+}
+
 }  // namespace
 
 std::string spec_source(const RTileProgram& rp, uint32_t nq, uint64_t smask, bool nt) {
@@ -284,7 +342,8 @@ std::string spec_source(const RTileProgram& rp, uint32_t nq, uint64_t smask, boo
     const size_t P = rp.phases.size();
     const std::string NT = nt ? "true" : "false";
     std::string s;
-    s += "extern \"C\" __global__ void __launch_bounds__(256) lqcs_pass(sp_amp* __restrict__ s, const sp_amp* __restrict__ w) {\n";
+    s += "extern \"C\" __global__ void __launch_bounds__(256)" + waves_attr() +
+         " lqcs_pass(sp_amp* __restrict__ s, const sp_amp* __restrict__ w) {\n";
     s += "    __shared__ sp_amp lds[2048];\n";
     s += "    char* const lb = reinterpret_cast<char*>(lds);\n";
     s += "    const sp_u32 tid = threadIdx.x;\n";
@@ -292,20 +351,19 @@ std::string spec_source(const RTileProgram& rp, uint32_t nq, uint64_t smask, boo
     s += "    sp_amp* const b = s + ((sp_u64)blockIdx.y << " + dec(nq) + ") + (" +
          scatter_expr("(sp_u64)tile", outside_dst, true) + ");\n";
     s += "    sp::Regs v;\n";
-    s += "    {\n        const sp_amp* p = b + (" + tglobal_expr(lay[0], gq) + ");\n";
-    for (int r = 0; r < 16; ++r) s += "        v[" + dec(r) + "] = sp::ld<" + NT + ">(p + " + hex(reg_global(lay[0], r, gq)) + ");\n";
-    s += "    }\n";
+    s += "    {\n" + edge_access(lay[0], gq, true, NT) + "    }\n";
     bool lds_used = false;
     for (size_t k = 0; k < P; ++k) {
         const RPhase& ph = rp.phases[k];
         s += "    {  // phase " + dec(k) + "\n";
+        s += "        const sp_u32 t = sp::fresh(tid);\n";
         if (k > 0 && ph.transpose) {
             const int h = split[k];
             const uint32_t pbit = 1u << spos[k];
             s += "        const sp_u32 sp0 = sp::slot<" + dec(h) + ">(" + jt_expr(lay[k - 1]) + ");\n";
             s += "        const sp_u32 sc0 = sp::slot<" + dec(h) + ">(" + jt_expr(lay[k]) + ");\n";
             for (int half = 0; half < 2; ++half) {
-                const std::string cond = half == 0 ? "!(tid & " + dec(pbit) + "u)" : "(tid & " + dec(pbit) + "u)";
+                const std::string cond = half == 0 ? "!(t & " + dec(pbit) + "u)" : "(t & " + dec(pbit) + "u)";
                 if (lds_used) s += "        __syncthreads();\n";
                 s += "        if (" + cond + ") {\n";
                 for (int r = 0; r < 16; ++r)
@@ -330,17 +388,14 @@ std::string spec_source(const RTileProgram& rp, uint32_t nq, uint64_t smask, boo
             std::memcpy(&h, &rp.words[off], sizeof h);
             const uint32_t kind = h.code & 15u;
             if (kind == RK_DIAG || kind == RK_PAULI) need_jr = true;
-            if (!emit_op(ops, h, off + 2, jr_name)) return {};
+            if (!emit_op(ops, h, off + 2, &rp.words[off + 2], jr_name)) return {};
             off += 2 + ((h.code >> 16) & 0xFFFu);
         }
         if (need_jr) s += jr_array(lay[k], jr_name);
         s += ops;
         s += "    }\n";
     }
-    s += "    {\n        sp_amp* p = b + (" + tglobal_expr(lay[P - 1], gq) + ");\n";
-    for (int r = 0; r < 16; ++r)
-        s += "        sp::st<" + NT + ">(p + " + hex(reg_global(lay[P - 1], r, gq)) + ", v[" + dec(r) + "]);\n";
-    s += "    }\n}\n";
+    s += "    {\n" + edge_access(lay[P - 1], gq, false, NT) + "    }\n}\n";
     return s;
 }
 

@@ -11,8 +11,10 @@
 // Fixed structure buys two things over the interpreting kernel:
 // * no op dispatch: no scalar decode, no branch chain per op;
 // * one register set updated in place, since no dispatch join forces a
-//   second set: about 90 VGPRs instead of 153, so more waves per SIMD hide
-//   the HBM latency behind other waves' FP64 work.
+//   second set: about 110 VGPRs instead of 163, so four waves per SIMD hide
+//   the HBM latency behind other waves' FP64 work instead of three;
+// * compile-time matrix entry flags (entries with a zero imaginary part, or
+//   zero entries), whose products are left out.
 //
 // This header is self-contained: it is compiled by hipRTC (which provides
 // the HIP device declarations) and, for the build-time check, by hipcc.
@@ -35,6 +37,15 @@ SP_INLINE sp_amp cmac2(sp_amp m0, sp_amp a0, sp_amp m1, sp_amp a1) {
 SP_INLINE sp_amp cmul(sp_amp a, sp_amp b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 SP_INLINE sp_amp pick(bool c, sp_amp p, sp_amp q) { return make_double2(c ? p.x : q.x, c ? p.y : q.y); }
 
+// The thread id as a value the compiler cannot see through: each phase
+// derives its layout's thread bits and LDS offsets from a fresh copy, so
+// they are computed where used instead of being kept live from the start of
+// the kernel (every long-lived value costs a VGPR in every wave).
+SP_INLINE sp_u32 fresh(sp_u32 x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 // One matrix entry of the op words: the same address in every lane, so a
 // scalar load.
 SP_INLINE sp_amp uni(const sp_amp* __restrict__ p) {
@@ -48,48 +59,81 @@ SP_INLINE sp_amp uni(const sp_amp* __restrict__ p) {
     return make_double2(__longlong_as_double(static_cast<long long>(rx)), __longlong_as_double(static_cast<long long>(ry)));
 }
 
-// ---- 2x2 ops on register bit B ------------------------------------------------
+// ---- entry flags ----------------------------------------------------------------
+//
+// Two bits per matrix entry, fixed when the pass is compiled: bit 0, the
+// imaginary part is exactly zero (a U3's top-left entry, every entry of H or
+// RY); bit 1, the entry is exactly zero. The products of such parts are left
+// out; what remains is summed in the same order as cmac2.
 
-template <int B>
+template <int F>
+SP_INLINE sp_amp cm1(sp_amp m, sp_amp a) {
+    if constexpr ((F & 1) != 0) return make_double2(m.x * a.x, m.x * a.y);
+    else return make_double2(m.x * a.x - m.y * a.y, m.x * a.y + m.y * a.x);
+}
+
+template <int F>
+SP_INLINE sp_amp cma(sp_amp acc, sp_amp m, sp_amp a) {
+    if constexpr ((F & 1) != 0) return make_double2(acc.x + m.x * a.x, acc.y + m.x * a.y);
+    else return make_double2(acc.x + m.x * a.x - m.y * a.y, acc.y + m.x * a.y + m.y * a.x);
+}
+
+// m0*a0 + m1*a1 under the entries' flags F0, F1.
+template <int F0, int F1>
+SP_INLINE sp_amp cmac2f(sp_amp m0, sp_amp a0, sp_amp m1, sp_amp a1) {
+    if constexpr ((F0 & 2) != 0 && (F1 & 2) != 0) return make_double2(0.0, 0.0);
+    else if constexpr ((F0 & 2) != 0) return cm1<F1>(m1, a1);
+    else if constexpr ((F1 & 2) != 0) return cm1<F0>(m0, a0);
+    else return cma<F1>(cm1<F0>(m0, a0), m1, a1);
+}
+
+// ---- 2x2 ops on register bit B (F: flags of entries 0..3, two bits each) --------
+
+template <int B, unsigned F>
 SP_INLINE void d1v(Regs& v, sp_amp m0, sp_amp m1, sp_amp m2, sp_amp m3) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         if (r & (1 << B)) continue;
         const sp_amp a0 = v[r], a1 = v[r | (1 << B)];
-        v[r] = cmac2(m0, a0, m1, a1);
-        v[r | (1 << B)] = cmac2(m2, a0, m3, a1);
+        v[r] = cmac2f<F & 3, (F >> 2) & 3>(m0, a0, m1, a1);
+        v[r | (1 << B)] = cmac2f<(F >> 4) & 3, (F >> 6) & 3>(m2, a0, m3, a1);
     }
 }
 
-template <int B>
+template <int B, unsigned F>
 SP_INLINE void d1(Regs& v, const sp_amp* __restrict__ m) {
-    d1v<B>(v, uni(m), uni(m + 1), uni(m + 2), uni(m + 3));
+    d1v<B, F>(v, uni(m), uni(m + 1), uni(m + 2), uni(m + 3));
 }
 
 // The matrix chosen by register bit S: entries 0..3 where it is 0, 4..7
-// where it is 1.
-template <int B, int S>
+// where it is 1 (F: flags of all eight entries).
+template <int B, int S, unsigned F>
 SP_INLINE void sel_r(Regs& v, const sp_amp* __restrict__ m) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         if (r & (1 << B)) continue;
-        const sp_amp* mr = (r & (1 << S)) ? m + 4 : m;
         const sp_amp a0 = v[r], a1 = v[r | (1 << B)];
-        v[r] = cmac2(uni(mr), a0, uni(mr + 1), a1);
-        v[r | (1 << B)] = cmac2(uni(mr + 2), a0, uni(mr + 3), a1);
+        if (r & (1 << S)) {
+            v[r] = cmac2f<(F >> 8) & 3, (F >> 10) & 3>(uni(m + 4), a0, uni(m + 5), a1);
+            v[r | (1 << B)] = cmac2f<(F >> 12) & 3, (F >> 14) & 3>(uni(m + 6), a0, uni(m + 7), a1);
+        } else {
+            v[r] = cmac2f<F & 3, (F >> 2) & 3>(uni(m), a0, uni(m + 1), a1);
+            v[r | (1 << B)] = cmac2f<(F >> 4) & 3, (F >> 6) & 3>(uni(m + 2), a0, uni(m + 3), a1);
+        }
     }
 }
 
-// The matrix chosen per lane (select bit among the thread's tile bits).
-template <int B>
+// The matrix chosen per lane (select bit among the thread's tile bits); F:
+// the flags both matrices share.
+template <int B, unsigned F>
 SP_INLINE void sel_l(Regs& v, const sp_amp* __restrict__ m, bool s) {
-    d1v<B>(v, pick(s, uni(m + 4), uni(m)), pick(s, uni(m + 5), uni(m + 1)), pick(s, uni(m + 6), uni(m + 2)),
-           pick(s, uni(m + 7), uni(m + 3)));
+    d1v<B, F>(v, pick(s, uni(m + 4), uni(m)), pick(s, uni(m + 5), uni(m + 1)), pick(s, uni(m + 6), uni(m + 2)),
+              pick(s, uni(m + 7), uni(m + 3)));
 }
 
 // Controlled 2x2: pairs whose register bits miss a control (RC), or lanes
 // whose thread bits miss one (tp false), keep their values.
-template <int B, int RC>
+template <int B, int RC, unsigned F>
 SP_INLINE void d1_c(Regs& v, const sp_amp* __restrict__ m, bool tp) {
     const sp_amp m0 = uni(m), m1 = uni(m + 1), m2 = uni(m + 2), m3 = uni(m + 3);
 #pragma unroll
@@ -97,7 +141,8 @@ SP_INLINE void d1_c(Regs& v, const sp_amp* __restrict__ m, bool tp) {
         if (r & (1 << B)) continue;
         if ((r & RC) != RC) continue;
         const sp_amp a0 = v[r], a1 = v[r | (1 << B)];
-        const sp_amp n0 = cmac2(m0, a0, m1, a1), n1 = cmac2(m2, a0, m3, a1);
+        const sp_amp n0 = cmac2f<F & 3, (F >> 2) & 3>(m0, a0, m1, a1);
+        const sp_amp n1 = cmac2f<(F >> 4) & 3, (F >> 6) & 3>(m2, a0, m3, a1);
         v[r] = pick(tp, n0, a0);
         v[r | (1 << B)] = pick(tp, n1, a1);
     }
@@ -124,21 +169,36 @@ SP_INLINE void x_c(Regs& v, bool tp) {
 
 // ---- 4x4 ops on register bits B0 (matrix bit 0) and B1 -------------------------
 
+// Row of a 4x4 op times x under the flags F of its four entries.
+template <unsigned F>
 SP_INLINE sp_amp row4(const sp_amp* __restrict__ m, const sp_amp (&x)[4]) {
     const sp_amp m0 = uni(m), m1 = uni(m + 1), m2 = uni(m + 2), m3 = uni(m + 3);
-    return make_double2(m0.x * x[0].x - m0.y * x[0].y + m1.x * x[1].x - m1.y * x[1].y + m2.x * x[2].x -
-                            m2.y * x[2].y + m3.x * x[3].x - m3.y * x[3].y,
-                        m0.x * x[0].y + m0.y * x[0].x + m1.x * x[1].y + m1.y * x[1].x + m2.x * x[2].y +
-                            m2.y * x[2].x + m3.x * x[3].y + m3.y * x[3].x);
+    if constexpr ((F & 0xFFu) == 0xAAu) {
+        return make_double2(0.0, 0.0);
+    } else {
+        // The first nonzero entry starts the sum; the others follow in order.
+        constexpr int f0 = F & 3, f1 = (F >> 2) & 3, f2 = (F >> 4) & 3, f3 = (F >> 6) & 3;
+        sp_amp acc;
+        if constexpr ((f0 & 2) == 0) acc = cm1<f0>(m0, x[0]);
+        if constexpr ((f1 & 2) == 0) {
+            if constexpr ((f0 & 2) == 0) acc = cma<f1>(acc, m1, x[1]);
+            else acc = cm1<f1>(m1, x[1]);
+        }
+        if constexpr ((f2 & 2) == 0) {
+            if constexpr ((f0 & 2) == 0 || (f1 & 2) == 0) acc = cma<f2>(acc, m2, x[2]);
+            else acc = cm1<f2>(m2, x[2]);
+        }
+        if constexpr ((f3 & 2) == 0) {
+            if constexpr ((f0 & 2) == 0 || (f1 & 2) == 0 || (f2 & 2) == 0) acc = cma<f3>(acc, m3, x[3]);
+            else acc = cm1<f3>(m3, x[3]);
+        }
+        return acc;
+    }
 }
 
-SP_INLINE sp_amp row4r(const sp_amp* __restrict__ m, const sp_amp (&x)[4]) {
-    const double m0 = uni(m).x, m1 = uni(m + 1).x, m2 = uni(m + 2).x, m3 = uni(m + 3).x;
-    return make_double2(m0 * x[0].x + m1 * x[1].x + m2 * x[2].x + m3 * x[3].x,
-                        m0 * x[0].y + m1 * x[1].y + m2 * x[2].y + m3 * x[3].y);
-}
-
-template <int B0, int B1, int RC, bool REAL>
+// 4x4 op on register bits B0 (matrix bit 0) and B1; F: flags of the 16
+// entries, row-major.
+template <int B0, int B1, int RC, unsigned F>
 SP_INLINE void d2(Regs& v, const sp_amp* __restrict__ m, bool tp) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
@@ -148,11 +208,12 @@ SP_INLINE void d2(Regs& v, const sp_amp* __restrict__ m, bool tp) {
         sp_amp x[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) x[k] = v[idx[k]];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const sp_amp n = REAL ? row4r(m + 4 * k, x) : row4(m + 4 * k, x);
-            v[idx[k]] = pick(tp, n, x[k]);
-        }
+        const sp_amp n0 = row4<F & 0xFFu>(m, x), n1 = row4<(F >> 8) & 0xFFu>(m + 4, x);
+        const sp_amp n2 = row4<(F >> 16) & 0xFFu>(m + 8, x), n3 = row4<(F >> 24) & 0xFFu>(m + 12, x);
+        v[idx[0]] = pick(tp, n0, x[0]);
+        v[idx[1]] = pick(tp, n1, x[1]);
+        v[idx[2]] = pick(tp, n2, x[2]);
+        v[idx[3]] = pick(tp, n3, x[3]);
     }
 }
 
