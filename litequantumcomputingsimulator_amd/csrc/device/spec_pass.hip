@@ -13,12 +13,14 @@
 // split by a tile bit h that is a thread bit of both layouts. A thread writes
 // its registers in the round of its old half and reads in the round of its
 // new half; in place, no thread may read before it has written, so h must sit
-// at the same thread-id bit in both layouts. Here every layout orders its
-// thread bits so that the split bit of the transpose into it and the one out
-// of it sit at thread-id bits 6 and 7 (alternating per transpose, so the two
-// never collide); the other thread bits fill the remaining positions in
-// ascending order, which keeps the lowest tile qubits on the lanes of the
-// load and store layouts.
+// at the same thread-id bit in both layouts. Every layout orders its thread
+// bits so that the split bits of the transposes into and out of it sit at
+// thread-id bits 7 and 6 (wave bits), alternating so the two never collide;
+// the other thread bits fill the remaining positions in ascending order,
+// which keeps the tile's contiguous low qubits on the lanes of the load and
+// store layouts. When two layouts share no thread bit but such low qubits,
+// the split is a low qubit at its own lane position. Each transpose has its
+// own linear LDS slot map, conflict-free for the lanes of both layouts.
 //
 // The generated kernel reads its matrix entries from the pass's op words (the
 // same words the interpreting kernel reads), so passes that differ only in
@@ -32,6 +34,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
@@ -103,13 +106,25 @@ struct Layout {
 };
 
 // The pass's layouts with thread-bit orders for in-place transposes, and the
-// split bit of each transpose (split[k]: into layout k; -1 none).
-bool plan_layouts(const RTileProgram& rp, std::vector<Layout>& lay, std::vector<int>& split, std::vector<int>& spos) {
+// split bit of each transpose (split[k]: into layout k; -1 none) with its
+// thread-id position (spos[k], the same in both layouts). `run` is the number
+// of the tile's contiguous low qubits (tile bits 0..run-1): in the load and
+// store layouts they stay on lanes 0..run-1, in order, so that each group of
+// 2^run lanes reads or writes one contiguous run of amplitudes.
+//
+// A split bit goes to thread-id bit 7 or 6 (a wave bit: whole waves take
+// turns), alternating so that the splits into and out of a layout never share
+// a position. When the only thread bits the two layouts share are low bits of
+// an edge layout, the split is such a bit at its own lane position; the two
+// rounds then run on half the lanes of every wave.
+bool plan_layouts(const RTileProgram& rp, uint32_t run, std::vector<Layout>& lay, std::vector<int>& split,
+                  std::vector<int>& spos) {
     const size_t P = rp.phases.size();
     lay.assign(P, Layout{});
     split.assign(P, -1);
     spos.assign(P, -1);
     const uint32_t all = (1u << kL) - 1;
+    run = std::min<uint32_t>(run, 6);
     for (size_t k = 0; k < P; ++k) {
         reg_bits(rp.phases[k], lay[k].rb);
         uint32_t rm = 0;
@@ -117,55 +132,70 @@ bool plan_layouts(const RTileProgram& rp, std::vector<Layout>& lay, std::vector<
         lay[k].tmask = all & ~rm;
         for (int r = 0; r < 16; ++r) lay[k].jr[r] = rp.phases[k].jr[r];
     }
-    int n_t = 0;
+    // pins[k][p]: tile bit fixed at thread-id position p of layout k (-1 free)
+    std::vector<std::array<int, kT>> pins(P);
+    for (auto& a : pins) a.fill(-1);
+    auto edge = [&](size_t k) { return k == 0 || k == P - 1; };
+    // A layout without a transpose of its own repeats the previous one: it
+    // shares that layout's pins (handled by copying orders below).
+    int wave_pos = 7;
     for (size_t k = 1; k < P; ++k) {
         if (!rp.phases[k].transpose) continue;
-        const uint32_t common = lay[k - 1].tmask & lay[k].tmask;
-        uint32_t cand = common;
-        if (split[k - 1] >= 0) cand &= ~(1u << split[k - 1]);  // pinned at the other position in layout k-1
-        // Keep the low tile qubits on the lanes of the load and store layouts.
-        const bool edge = (k - 1 == 0) || (k == P - 1);
-        uint32_t pref = cand;
-        if (edge) {
-            for (size_t e : {size_t{0}, P - 1}) {
-                if (e != k - 1 && e != k) continue;
-                uint32_t low = 0, t = lay[e].tmask;
-                for (int c = 0; c < 6 && t; ++c) {
-                    low |= t & (~t + 1);
-                    t &= t - 1;
-                }
-                pref &= ~low;
-            }
-        }
-        const uint32_t pick_from = pref ? pref : cand;
-        if (!pick_from) return false;
-        split[k] = 31 - __builtin_clz(pick_from);
-        spos[k] = (n_t++ % 2 == 0) ? 7 : 6;
-    }
-    // Thread-bit orders: pinned split bits, the rest ascending.
-    for (size_t k = 0; k < P; ++k) {
-        int fixed[kT];
-        for (uint32_t p = 0; p < kT; ++p) fixed[p] = -1;
-        uint32_t used = 0;
-        auto pin = [&](int bit, int pos) {
-            fixed[pos] = bit;
-            used |= 1u << bit;
+        size_t prev = k - 1;
+        while (prev > 0 && !rp.phases[prev].transpose) --prev;  // the layout in force before k
+        const uint32_t common = lay[prev].tmask & lay[k].tmask;
+        auto pinned_bit = [&](size_t l, int bit) {
+            for (int p = 0; p < static_cast<int>(kT); ++p)
+                if (pins[l][p] == bit) return true;
+            return false;
         };
-        // A layout without its own transpose repeats the previous one.
+        auto low_in_edge = [&](int bit) {
+            return bit < static_cast<int>(run) && (edge(prev) || edge(k));
+        };
+        // Candidates: not pinned in the old layout already; high bits first
+        // (highest first), then low bits.
+        std::vector<int> cand;
+        for (int b = kL - 1; b >= 0; --b)
+            if ((common >> b & 1u) && !pinned_bit(prev, b) && !low_in_edge(b)) cand.push_back(b);
+        for (int b = kL - 1; b >= 0; --b)
+            if ((common >> b & 1u) && !pinned_bit(prev, b) && low_in_edge(b)) cand.push_back(b);
+        bool placed = false;
+        for (int h : cand) {
+            int positions[2] = {wave_pos, 13 - wave_pos};
+            int np = 2;
+            if (low_in_edge(h)) {
+                positions[0] = h;
+                np = 1;
+            }
+            for (int i = 0; i < np && !placed; ++i) {
+                const int p = positions[i];
+                if (pins[prev][p] >= 0 || pins[k][p] >= 0) continue;
+                // keep the edge layouts' low lanes for their own bits
+                if (p < static_cast<int>(run) && p != h && (edge(prev) || edge(k))) continue;
+                pins[prev][p] = h;
+                pins[k][p] = h;
+                split[k] = h;
+                spos[k] = p;
+                placed = true;
+                if (p == wave_pos) wave_pos = 13 - wave_pos;
+            }
+            if (placed) break;
+        }
+        if (!placed) return false;
+    }
+    // Thread-bit orders: pinned bits, the rest ascending in the free positions.
+    for (size_t k = 0; k < P; ++k) {
         if (k > 0 && !rp.phases[k].transpose) {
             std::memcpy(lay[k].tpos, lay[k - 1].tpos, sizeof lay[k].tpos);
             continue;
         }
-        if (split[k] >= 0) pin(split[k], spos[k]);
-        for (size_t n = k + 1; n < P; ++n) {  // the next transpose out of this layout
-            if (!rp.phases[n].transpose) continue;
-            pin(split[n], spos[n]);
-            break;
-        }
+        uint32_t used = 0;
+        for (uint32_t p = 0; p < kT; ++p)
+            if (pins[k][p] >= 0) used |= 1u << pins[k][p];
         uint32_t rest = lay[k].tmask & ~used;
         for (uint32_t p = 0; p < kT; ++p) {
-            if (fixed[p] >= 0) {
-                lay[k].tpos[p] = fixed[p];
+            if (pins[k][p] >= 0) {
+                lay[k].tpos[p] = pins[k][p];
                 continue;
             }
             if (!rest) return false;
@@ -223,6 +253,80 @@ std::string edge_access(const Layout& l, const int (&gq)[kL], bool load, const s
         else s += "        sp::st<" + NT + ">(" + a + ", v[" + dec(r) + "]);\n";
     }
     return s;
+}
+
+// LDS slots of one half-tile transpose split by tile bit h. The index of
+// tile index j is c(j), j without bit h (11 bits); its slot is M c(j) for an
+// invertible matrix M over GF(2) chosen per transpose so that the 16 lanes
+// of a writing or reading group (thread-id bits 0..3 of the old and of the
+// new layout) fall on 16 distinct slots modulo 16: 16-byte accesses then hit
+// distinct banks. Rows 0..3 of M: row b picks the old layout's lane bit b,
+// plus one new-layout lane bit that is not an old lane bit, placed in a row
+// that no shared lane bit occupies, so that both 4x4 blocks are permutations;
+// rows 4..10: the bits that are not old lane bits, in order. M is linear, so
+// slot(jt | jr) = slot(jt) ^ slot(jr): the thread's part is computed once
+// per layout, the register's part is a constant.
+struct SlotMap {
+    uint32_t col[11];  // M's column of each index bit
+    uint32_t cidx(int tile_bit, int h) const { return static_cast<uint32_t>(tile_bit < h ? tile_bit : tile_bit - 1); }
+    uint32_t bytes(uint32_t j, int h) const {
+        uint32_t c = (j & ((1u << h) - 1u)) | ((j >> (h + 1)) << h), s = 0;
+        for (int x = 0; x < 11; ++x)
+            if (c & (1u << x)) s ^= col[x];
+        return s << 4;
+    }
+};
+
+SlotMap slot_map(const Layout& prev, const Layout& cur, int h) {
+    SlotMap sm;
+    uint32_t row[11];
+    // The first four lane bits of each layout other than the split bit (a
+    // split among the lanes halves the active lanes of each group).
+    int W[4], R[4];
+    for (int b = 0, p = 0; b < 4; ++p)
+        if (prev.tpos[p] != h) W[b++] = static_cast<int>(sm.cidx(prev.tpos[p], h));
+    for (int b = 0, p = 0; b < 4; ++p)
+        if (cur.tpos[p] != h) R[b++] = static_cast<int>(sm.cidx(cur.tpos[p], h));
+    bool taken[4] = {false, false, false, false};
+    for (int b = 0; b < 4; ++b) row[b] = 1u << W[b];
+    for (int j = 0; j < 4; ++j)
+        for (int b = 0; b < 4; ++b)
+            if (R[j] == W[b]) taken[b] = true;
+    for (int j = 0; j < 4; ++j) {
+        bool shared = false;
+        for (int b = 0; b < 4; ++b) shared |= R[j] == W[b];
+        if (shared) continue;
+        for (int b = 0; b < 4; ++b)
+            if (!taken[b]) {
+                taken[b] = true;
+                row[b] |= 1u << R[j];
+                break;
+            }
+    }
+    int n = 4;
+    for (int x = 0; x < 11; ++x) {
+        bool lane = false;
+        for (int b = 0; b < 4; ++b) lane |= W[b] == x;
+        if (!lane) row[n++] = 1u << x;
+    }
+    for (int x = 0; x < 11; ++x) {
+        sm.col[x] = 0;
+        for (int b = 0; b < 11; ++b)
+            if (row[b] & (1u << x)) sm.col[x] |= 1u << b;
+    }
+    return sm;
+}
+
+// The thread's part of the slot byte offset in layout l: the XOR of the
+// columns of its set thread-id bits (the split bit h excluded).
+std::string slot_thread_expr(const Layout& l, const SlotMap& sm, int h) {
+    std::string e;
+    for (uint32_t p = 0; p < kT; ++p) {
+        if (l.tpos[p] == h) continue;
+        const uint32_t c = sm.col[sm.cidx(l.tpos[p], h)] << 4;
+        e += (e.empty() ? "" : " ^ ") + std::string("(((t >> ") + dec(p) + ") & 1u) * " + dec(c) + "u)";
+    }
+    return e.empty() ? "0u" : e;
 }
 
 std::string jr_array(const Layout& l, const std::string& name) {
@@ -326,7 +430,7 @@ std::string spec_source(const RTileProgram& rp, uint32_t nq, uint64_t smask, boo
     if (rp.phases.empty() || nq < kL) return {};
     std::vector<Layout> lay;
     std::vector<int> split, spos;
-    if (!plan_layouts(rp, lay, split, spos)) return {};
+    if (!plan_layouts(rp, static_cast<uint32_t>(__builtin_ctzll(~smask)), lay, split, spos)) return {};
     int gq[kL];
     {
         int t = 0;
@@ -355,24 +459,31 @@ std::string spec_source(const RTileProgram& rp, uint32_t nq, uint64_t smask, boo
     bool lds_used = false;
     for (size_t k = 0; k < P; ++k) {
         const RPhase& ph = rp.phases[k];
-        s += "    {  // phase " + dec(k) + "\n";
+        {
+            std::string d = "    {  // phase " + dec(k) + ": register bits";
+            for (int i = 0; i < 4; ++i) d += " " + dec(lay[k].rb[i]);
+            d += "; thread bits";
+            for (uint32_t p = 0; p < kT; ++p) d += " " + dec(lay[k].tpos[p]);
+            s += d + "\n";
+        }
         s += "        const sp_u32 t = sp::fresh(tid);\n";
         if (k > 0 && ph.transpose) {
             const int h = split[k];
             const uint32_t pbit = 1u << spos[k];
-            s += "        const sp_u32 sp0 = sp::slot<" + dec(h) + ">(" + jt_expr(lay[k - 1]) + ");\n";
-            s += "        const sp_u32 sc0 = sp::slot<" + dec(h) + ">(" + jt_expr(lay[k]) + ");\n";
+            const SlotMap sm = slot_map(lay[k - 1], lay[k], h);
+            s += "        const sp_u32 sp0 = " + slot_thread_expr(lay[k - 1], sm, h) + ";\n";
+            s += "        const sp_u32 sc0 = " + slot_thread_expr(lay[k], sm, h) + ";\n";
             for (int half = 0; half < 2; ++half) {
                 const std::string cond = half == 0 ? "!(t & " + dec(pbit) + "u)" : "(t & " + dec(pbit) + "u)";
                 if (lds_used) s += "        __syncthreads();\n";
                 s += "        if (" + cond + ") {\n";
                 for (int r = 0; r < 16; ++r)
-                    s += "            *reinterpret_cast<sp_amp*>(lb + (sp0 ^ " + dec(rtile_slot(lay[k - 1].jr[r], h) << 4) +
+                    s += "            *reinterpret_cast<sp_amp*>(lb + (sp0 ^ " + dec(sm.bytes(lay[k - 1].jr[r], h)) +
                          "u)) = v[" + dec(r) + "];\n";
                 s += "        }\n        __syncthreads();\n        if (" + cond + ") {\n";
                 for (int r = 0; r < 16; ++r)
                     s += "            v[" + dec(r) + "] = *reinterpret_cast<const sp_amp*>(lb + (sc0 ^ " +
-                         dec(rtile_slot(lay[k].jr[r], h) << 4) + "u));\n";
+                         dec(sm.bytes(lay[k].jr[r], h)) + "u));\n";
                 s += "        }\n";
                 lds_used = true;
             }

@@ -323,14 +323,4 @@ SP_INLINE sp_u32 xcd_tile(sp_u32 b, sp_u32 count) {
     return (b & 7u) * (count >> 3) + (b >> 3);
 }
 
-// LDS slot (byte offset) of tile index j in the half-tile transpose split by
-// bit H: j without bit H, its low 3 bits XOR-folded with the higher ones so
-// that lanes differing in high bits spread over the banks. XOR-linear, so
-// slot(jt | jr) = slot(jt) ^ slot(jr).
-template <int H>
-SP_INLINE sp_u32 slot(sp_u32 j) {
-    const sp_u32 c = (j & ((1u << H) - 1u)) | ((j >> (H + 1)) << H);
-    return (c ^ (((c >> 3) ^ (c >> 6) ^ (c >> 9)) & 7u)) << 4;
-}
-
 }  // namespace sp

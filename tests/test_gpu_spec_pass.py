@@ -64,6 +64,7 @@ def _u3cx(n, layers, seed):
 CASES = [
     ("u3cx", 16, lambda: _u3cx(16, 8, 3), dict(plan="greedy")),
     ("u3cx_gain", 18, lambda: _u3cx(18, 8, 4), dict(plan="gain", low_qubits=3)),
+    ("u3cx_lane_split", 15, lambda: _u3cx(15, 6, 16), dict(plan="greedy")),
     ("random16kinds", 14, lambda: _spread(cases.random_circuit(14, 200, 4), 14, 5), dict(plan="greedy")),
     ("every_gate", 13, lambda: _spread(cases.every_gate_circuit(13), 13, 6), dict(plan="greedy")),
     ("rotations", 15, lambda: _spread(_rotations(15, 5), 15, 7), dict(plan="greedy")),

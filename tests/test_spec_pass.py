@@ -11,6 +11,7 @@ op calls are checked without a device. A second check compiles the 30-qubit
 benchmark's passes for gfx950 offline and reads their register use.
 """
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -114,6 +115,9 @@ def _spread(c, n, seed):
 @pytest.mark.parametrize("name,n,make,plan", [
     ("u3cx", 14, lambda: _u3cx(14, 6, 3), {"plan": "greedy"}),
     ("u3cx_gain", 14, lambda: _u3cx(14, 6, 4), {"plan": "gain", "low_qubits": 3}),
+    # a pass whose first two layouts share only low qubits: its transpose is
+    # split by a lane bit (half the lanes of every wave per round)
+    ("u3cx_lane_split", 15, lambda: _u3cx(15, 6, 16), {"plan": "greedy"}),
     ("random16kinds", 13, lambda: cases.random_circuit(13, 150, 4), {"plan": "greedy"}),
     ("rotations", 13, lambda: _rotations(13, 5), {"plan": "greedy"}),
     ("diagonals", 13, lambda: _diag_rich(13, 6), {"plan": "greedy", "low_qubits": 2}),
@@ -122,6 +126,8 @@ def test_generated_passes_equal_the_oracle(tmp_path, name, n, make, plan):
     circ = _spread(make(), n, 17)
     passes = _passes(circ, n, **plan)
     assert passes and all(t for t, _ in passes)
+    if name == "u3cx_lane_split":
+        assert any(re.search(r"if \(!?\(?t & [1248]u\)", t) for t, _ in passes)
     e0 = np.zeros(1 << n, complex)
     e0[0] = 1.0
     got = _emulate(tmp_path, passes, n, e0, name)
