@@ -365,9 +365,22 @@ bool emit_op(std::string& out, const RegOpHdr& h, uint32_t off, const amp_t* m, 
             if (rctl || h.tctl) return false;
             const uint32_t f0 = entry_flags(m, 4), f1 = entry_flags(m + 4, 4);
             const std::string both = dec(f0 & f1) + "u";
-            if (xr) call = "sp::sel_r<" + dec(b0) + ", " + dec(xr - 1) + ", " + dec(f0 | (f1 << 8)) + "u>(v, " + M + ");";
-            else if (h.la) call = "sp::sel_l<" + dec(b0) + ", " + both + ">(v, " + M + ", (jt & " + dec(h.la) + "u) != 0);";
-            else call = "sp::d1<" + dec(b0) + ", " + both + ">(v, " + M + " + ((tile & " + dec(h.ga) + "u) ? 4 : 0));";
+            auto same = [](amp_t a, amp_t b) { return a.x == b.x && a.y == b.y; };
+            const bool cols = same(m[4], m[1]) && same(m[5], m[0]) && same(m[6], m[3]) && same(m[7], m[2]);
+            const bool rows = same(m[4], m[2]) && same(m[5], m[3]) && same(m[6], m[0]) && same(m[7], m[1]);
+            if (xr) {
+                call = "sp::sel_r<" + dec(b0) + ", " + dec(xr - 1) + ", " + dec(f0 | (f1 << 8)) + "u>(v, " + M + ");";
+            } else if (h.la) {
+                const std::string sel = "(jt & " + dec(h.la) + "u) != 0";
+                if (cols || rows)
+                    call = "sp::sel_swap<" + dec(b0) + ", " + dec(f0) + "u, " + (cols ? "true" : "false") + ">(v, " + M +
+                           ", " + sel + ");";
+                else
+                    call = "sp::sel_l<" + dec(b0) + ", " + both + ">(v, " + M + ", " + sel + ");";
+            } else {
+                // the select bit is a tile-number bit: one matrix per tile
+                call = "sp::d1<" + dec(b0) + ", " + both + ">(v, " + M + " + ((tile & " + dec(h.ga) + "u) ? 4 : 0));";
+            }
             break;
         }
         case RK_X:
@@ -414,13 +427,16 @@ bool emit_op(std::string& out, const RegOpHdr& h, uint32_t off, const amp_t* m, 
     return true;
 }
 
-// Waves per SIMD the compiler is asked to fit (LQCS_SPEC_WAVES, 4 or 5;
-// unset: the compiler's own choice under 256-thread workgroups).
+// Waves per SIMD the compiler must fit: four (at most 128 VGPRs; most passes
+// need 95-120, and a pass that needs more would otherwise drop to three).
+// LQCS_SPEC_WAVES=5 asks for five (96 VGPRs, with a few spilled) for
+// comparisons.
 std::string waves_attr() {
-    const char* e = std::getenv("LQCS_SPEC_WAVES");
-    if (!e) return "";
-    const int w = std::atoi(e);
-    if (w < 4 || w > 8) return "";
+    int w = 4;
+    if (const char* e = std::getenv("LQCS_SPEC_WAVES")) {
+        const int v = std::atoi(e);
+        if (v >= 4 && v <= 8) w = v;
+    }
     return " __attribute__((amdgpu_waves_per_eu(" + std::to_string(w) + ", 8)))";
 }
 

@@ -123,6 +123,34 @@ SP_INLINE void sel_r(Regs& v, const sp_amp* __restrict__ m) {
     }
 }
 
+// The second matrix is the first with its columns swapped (U X) or its rows
+// swapped (X U), as when a CX is merged into the 2x2 op on its target: the lanes whose select bit is set swap each pair's inputs
+// (COLS) or outputs before or after the first matrix, so the matrix stays in
+// scalar registers instead of a per-lane copy.
+template <int B, unsigned F, bool COLS>
+SP_INLINE void sel_swap(Regs& v, const sp_amp* __restrict__ m, bool s) {
+    const sp_amp m0 = uni(m), m1 = uni(m + 1), m2 = uni(m + 2), m3 = uni(m + 3);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        if (r & (1 << B)) continue;
+        sp_amp a0 = v[r], a1 = v[r | (1 << B)];
+        if (COLS) {
+            const sp_amp t0 = pick(s, a1, a0), t1 = pick(s, a0, a1);
+            a0 = t0;
+            a1 = t1;
+        }
+        const sp_amp n0 = cmac2f<F & 3, (F >> 2) & 3>(m0, a0, m1, a1);
+        const sp_amp n1 = cmac2f<(F >> 4) & 3, (F >> 6) & 3>(m2, a0, m3, a1);
+        if (COLS) {
+            v[r] = n0;
+            v[r | (1 << B)] = n1;
+        } else {
+            v[r] = pick(s, n1, n0);
+            v[r | (1 << B)] = pick(s, n0, n1);
+        }
+    }
+}
+
 // The matrix chosen per lane (select bit among the thread's tile bits); F:
 // the flags both matrices share.
 template <int B, unsigned F>

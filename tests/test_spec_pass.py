@@ -41,7 +41,8 @@ def _emulate(tmp_path, passes, n, state, tag):
     src = ['#include "spec_shim.hpp"', '#include "spec_pass.hpp"']
     for i, (text, _) in enumerate(passes):
         assert text, f"pass {i} has no specialized kernel"
-        src.append(text.replace("lqcs_pass(", f"lqcs_pass_{i}("))
+        host = re.sub(r"__attribute__\(\(amdgpu_waves_per_eu\([^)]*\)\)\)", "", text)  # a device-only hint
+        src.append(host.replace("lqcs_pass(", f"lqcs_pass_{i}("))
         src.append(f'extern "C" void run_pass_{i}(double* s, const double* w, unsigned tiles) {{\n'
                    f'    shim_launch([&] {{ lqcs_pass_{i}(reinterpret_cast<sp_amp*>(s), '
                    f'reinterpret_cast<const sp_amp*>(w)); }}, tiles, 1);\n}}')
@@ -137,12 +138,12 @@ def test_generated_passes_equal_the_oracle(tmp_path, name, n, make, plan):
 
 @pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not available")
 def test_benchmark_passes_fit_four_waves_per_simd(tmp_path):
-    """The 30-qubit benchmark's first passes compiled for gfx950: at most 128
-    VGPRs (four waves per SIMD) and no scratch spills."""
+    """The 30-qubit benchmark's passes compiled for gfx950: at most 128 VGPRs
+    (four waves per SIMD) and no scratch spills."""
     circ = _u3cx(30, 20, 1234)
     comp = lq.CompiledCircuit(circ.to_product(lq), True)
     hdr = open(os.path.join(DEVICE_SRC, "spec_pass.hpp")).read()
-    for step in range(3):
+    for step in range(len(comp.describe(plan="greedy"))):
         text, words = comp.specialized_pass(step, plan="greedy")
         assert text and len(words) > 0
         src = tmp_path / f"pass{step}.hip"
