@@ -4,8 +4,11 @@
 Workload (one step): |0...0> then 20 layers of [U(theta, phi, lambda) on every
 qubit, angles uniform in [0, 2pi), then CX on a random perfect matching] --
 900 gates -- with the reference's greedy single-qubit fusion on, evolved in
-place in HBM by the engine's LDS-tiled passes. The register stays resident
-across steps; the timed region excludes host transfers.
+place in HBM by the engine's register-tile passes. By default each pass runs
+on a kernel specialized to its structure and compiled at run time during the
+warmup step (reported as first_apply_s, never timed); --tile-kernel register
+runs the interpreting kernel instead. The register stays resident across
+steps; the timed region excludes host transfers.
 
 Prints one JSON line: gates/s (whole job), the roofline of the dominant
 kernel (tiled pass, HIP-event timed), and the reference's OpenMP path timed
@@ -623,9 +626,9 @@ def main() -> int:
     ap.add_argument("--plan", default="greedy")
     ap.add_argument("--tile-qubits", type=int, default=0)
     ap.add_argument("--low-qubits", type=int, default=0)
-    ap.add_argument("--tile-kernel", default="register",
-                    help="kernel of the tiled passes: register (interpreting register tile) or specialized "
-                         "(one kernel per pass structure, compiled at run time before the timed steps)")
+    ap.add_argument("--tile-kernel", default="specialized",
+                    help="kernel of the tiled passes: specialized (one kernel per pass structure, compiled at run "
+                         "time during the warmup step) or register (the interpreting register tile)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adjoint-circuits", type=int, default=256)
@@ -717,7 +720,8 @@ def main() -> int:
     del sv2
     if prof["tile_launches"]:
         avg_s = prof["tile_ms"] / 1e3 / prof["tile_launches"]
-        kernel = "k_rtile"  # the register-tile kernel runs every tiled pass of this plan
+        # every tiled pass of this plan runs on the selected register-tile kernel
+        kernel = "lqcs_pass" if args.tile_kernel == "specialized" else "k_rtile"
     else:
         avg_s = prof["other_ms"] / 1e3 / max(1, prof["other_launches"])
         kernel = "gate-by-gate kernels"
@@ -750,10 +754,33 @@ def main() -> int:
                     greedy_ms = min(greedy_ms, ms)
                 else:
                     learned_ms = min(learned_ms, ms)
-        planner = {"greedy_ms_per_step": round(greedy_ms, 3),
+        planner = {"kernel": "k_rtile (interpreting register tile: the kernel the cost model is trained on)",
+                   "greedy_ms_per_step": round(greedy_ms, 3),
                    "learned_ms_per_step": round(learned_ms, 3), "learned_pick": choice,
                    "learned_vs_greedy": round(learned_ms / greedy_ms, 4)}
         log(f"planner: learned {learned_ms:.1f} ms/step ({choice})")
+        # Other plans on the specialized passes: gain grouping at 3 low qubits
+        # on 12- and 13-qubit tiles (13 runs only on specialized passes).
+        spec = {}
+        for name, L, plan, low in (("greedy_12q_low4", 12, "greedy", 4), ("gain_12q_low3", 12, "gain", 3),
+                                   ("gain_13q_low3", 13, "gain", 3)):
+            kw = dict(plan=plan, tile_qubits=L, low_qubits=low, tile_kernel="specialized")
+            t0 = time.perf_counter()
+            comp.apply(sv, **kw)  # compiles its passes
+            lq.synchronize()
+            compile_s = time.perf_counter() - t0
+            best = float("inf")
+            for _ in range(3):
+                sv.set_basis(0)
+                lq.synchronize()
+                t0 = time.perf_counter()
+                comp.apply(sv, **kw)
+                lq.synchronize()
+                best = min(best, (time.perf_counter() - t0) * 1e3)
+            spec[name] = {"ms_per_step": round(best, 3), "gates_per_s": round(gates / best * 1e3, 1),
+                          "passes": lq.last_plan_stats()["passes"], "first_apply_s": round(compile_s, 3)}
+            log(f"specialized {name}: {best:.1f} ms/step")
+        planner["specialized_plans"] = spec
 
     adjoint = None
     trajectories = None

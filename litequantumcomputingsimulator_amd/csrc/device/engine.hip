@@ -500,12 +500,18 @@ StagedPlan stage_plan(const Plan& plan, uint32_t nq, uint32_t L, uint32_t batch)
         }
         RTileProgram rp;
         if (g_use_rtile && build_rtile(st.tile_ops, L, st.smask, nq, rp)) {
-            if (g_rtile_spec && !g_rtile_stream && !g_rtile_land && L == 12)
-                spec_src.back() = spec_source(rp, nq, st.smask, nt);
+            if (g_rtile_spec && !g_rtile_stream && !g_rtile_land && L >= 12)
+                spec_src.back() = spec_source(rp, nq, L, st.smask, nt);
+            if (L > 12 && spec_src.back().empty())
+                throw Error(LQCS_ERR_LOGIC, "13-qubit tile pass without a specialized kernel");
             sp.where.push_back({2, sp.rph.size(), rp.phases.size(), sp.rwords.size()});
             sp.rph.insert(sp.rph.end(), rp.phases.begin(), rp.phases.end());
             sp.rwords.insert(sp.rwords.end(), rp.words.begin(), rp.words.end());
         } else {
+            if (L > 12)
+                throw Error(LQCS_ERR_INVALID_ARGUMENT,
+                            "13-qubit tiles: an op acts densely on more than 4 tile qubits (a wide Pauli rotation); "
+                            "use tiles of at most 12 qubits for this circuit");
             sp.where.push_back({1, sp.flat.size(), st.tile_ops.size(), 0});
             sp.flat.insert(sp.flat.end(), st.tile_ops.begin(), st.tile_ops.end());
         }
@@ -527,7 +533,7 @@ void launch_staged(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan, const st
         }
         const StagedPlan::Where& w = sp.where[i];
         if (w.kind == 2 && i < sp.spec.size() && sp.spec[i]) {
-            launch_spec(sp.spec[i], s, nq, d_rwords + w.words, batch);
+            launch_spec(sp.spec[i], s, nq, L, d_rwords + w.words, batch);
         } else if (w.kind == 2) {
             launch_rtile(s, nq, L, st.smask, d_ph + w.first, static_cast<uint32_t>(w.count),
                          d_rwords ? d_rwords + w.words : nullptr, batch);
@@ -630,7 +636,10 @@ Plan make_plan(int kind, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_
         }
         return plan;
     }
-    if (L < 6 || L > 12 || L > nq) throw Error(LQCS_ERR_INVALID_ARGUMENT, "tile width must be 6..12 and <= n");
+    const uint32_t max_l = g_rtile_spec ? 13u : 12u;  // 13-qubit tiles run on specialized passes only
+    if (L < 6 || L > max_l || L > nq)
+        throw Error(LQCS_ERR_INVALID_ARGUMENT, g_rtile_spec ? "tile width must be 6..13 and <= n"
+                                                            : "tile width must be 6..12 and <= n (13 with specialized passes)");
     if (low + 2 > L) throw Error(LQCS_ERR_INVALID_ARGUMENT, "low qubits must leave two free tile slots");
     return kind == LQCS_PLAN_GAIN ? plan_gain(ops, nq, L, low) : plan_greedy(ops, nq, L, low);
 }
@@ -1027,9 +1036,9 @@ lqcs_status lqcs_plan_specialized_pass(uint32_t n_qubits, const lqcs_program* pr
         const PlanStep& st = plan.steps[step];
         std::string src;
         RTileProgram rp;
-        if (st.tiled && g_use_rtile && L == 12 && build_rtile(st.tile_ops, L, st.smask, n_qubits, rp)) {
+        if (st.tiled && g_use_rtile && L >= 12 && build_rtile(st.tile_ops, L, st.smask, n_qubits, rp)) {
             const bool nt = (uint64_t{1} << n_qubits) * sizeof(amp_t) >= (uint64_t{1} << 30);
-            src = spec_source(rp, n_qubits, st.smask, nt);
+            src = spec_source(rp, n_qubits, L, st.smask, nt);
         }
         if (src.empty()) rp.words.clear();
         *text_len = src.size();

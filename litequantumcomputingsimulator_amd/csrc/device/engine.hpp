@@ -106,13 +106,13 @@ void step_features(const PlanStep& st, const std::vector<DecodedOp>& ops, uint32
                    double* f);
 void last_plan_choice(int* plan, int* low, double* predicted_ms);
 // Specialized register-tile passes (spec_pass.hip): on for the calls that
-// select them; kernel text of a program (empty: a form the generator does not
-// handle, or a tile width other than 12); the kernels of a list of sources,
-// compiled on first use; their launch.
+// select them; kernel text of a program on L = 12 or 13 tile qubits (empty:
+// a form the generator does not handle); the kernels of a list of sources,
+// compiled on first use; their launch (2^(L-4) threads per tile).
 extern bool g_rtile_spec;
-std::string spec_source(const RTileProgram& rp, uint32_t nq, uint64_t smask, bool nt);
+std::string spec_source(const RTileProgram& rp, uint32_t nq, uint32_t L, uint64_t smask, bool nt);
 std::vector<hipFunction_t> spec_kernels(const std::vector<std::string>& sources);
-void launch_spec(hipFunction_t f, amp_t* s, uint32_t nq, const amp_t* d_words, uint32_t batch);
+void launch_spec(hipFunction_t f, amp_t* s, uint32_t nq, uint32_t L, const amp_t* d_words, uint32_t batch);
 size_t spec_cache_size();
 bool rtile_enabled();
 // Installed cost model (mlp.hip).

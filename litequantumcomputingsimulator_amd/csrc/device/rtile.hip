@@ -1056,7 +1056,11 @@ uint32_t merge_selects(std::vector<TileOp>& ops, std::vector<uint8_t>& sel) {
 }  // namespace
 
 bool build_rtile(const std::vector<TileOp>& tops_in, uint32_t L, uint64_t smask, uint32_t nq, RTileProgram& out) {
-    if (L != static_cast<uint32_t>(kRL) && L != 10u) return false;
+    if (L != static_cast<uint32_t>(kRL) && L != 10u && L != 13u) return false;
+    // 13-qubit programs run only as specialized passes: the fields below that
+    // only the interpreting kernels read (thread positions, tile-number
+    // insertion masks, transpose slots) stay zero for them.
+    const bool interp = L <= static_cast<uint32_t>(kRL);
     out.phases.clear();
     out.words.clear();
     std::vector<TileOp> tops = tops_in;
@@ -1118,7 +1122,7 @@ bool build_rtile(const std::vector<TileOp>& tops_in, uint32_t L, uint64_t smask,
         for (int p = 0; p < static_cast<int>(L); ++p)
             if (rmask & (1u << p)) rb[k++] = p;
         for (int i = 0; i < 4; ++i) ph.hole_low[i] = (1u << rb[i]) - 1u;
-        {
+        if (interp) {
             int t = 0;
             for (int q = 0; q < 64; ++q)
                 if (ph.tmask_global & (uint64_t{1} << q)) ph.tpos[t++] = static_cast<uint8_t>(q);
@@ -1134,7 +1138,7 @@ bool build_rtile(const std::vector<TileOp>& tops_in, uint32_t L, uint64_t smask,
             ph.jr[r] = j;
             ph.gofs[r] = pdep_host(j, smask);
         }
-        if (ph.transpose) {
+        if (ph.transpose && interp) {
             // Split bit: a thread bit of both layouts, preferring one that is
             // the top thread-id bit in both (then whole waves take turns).
             const RPhase& pp = out.phases.back();

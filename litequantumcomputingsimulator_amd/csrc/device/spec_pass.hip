@@ -9,13 +9,17 @@
 // removes the dispatch (scalar decode and a branch chain per op) and about 40
 // VGPRs, so four waves per SIMD fit instead of three.
 //
-// In-place transposes. The tile moves through 32 KiB of LDS in two halves,
+// Tiles of 12 qubits (256 threads) or 13 (512 threads; the interpreting
+// kernels stop at 12): with 13, a plan can keep five contiguous low qubits in
+// every tile (512-byte runs in HBM) and still have eight free tile qubits.
+//
+// In-place transposes. The tile moves through LDS (32 or 64 KiB) in two halves,
 // split by a tile bit h that is a thread bit of both layouts. A thread writes
 // its registers in the round of its old half and reads in the round of its
 // new half; in place, no thread may read before it has written, so h must sit
 // at the same thread-id bit in both layouts. Every layout orders its thread
 // bits so that the split bits of the transposes into and out of it sit at
-// thread-id bits 7 and 6 (wave bits), alternating so the two never collide;
+// the top two thread-id bits (wave bits), alternating so the two never collide;
 // the other thread bits fill the remaining positions in ascending order,
 // which keeps the tile's contiguous low qubits on the lanes of the load and
 // store layouts. When two layouts share no thread bit but such low qubits,
@@ -58,8 +62,10 @@ const char* const kSpecHeader =
 #include "spec_pass_hpp.inc"
     ;
 
-constexpr uint32_t kL = 12;   // tile bits
-constexpr uint32_t kT = 8;    // thread bits (256 threads)
+// Tiles of L = 12 or 13 qubits: 16 registers per thread, T = L - 4 thread
+// bits (256 or 512 threads).
+constexpr uint32_t kLMax = 13;
+constexpr uint32_t kTMax = 9;
 
 std::string hex(uint64_t v, bool wide = true) {
     char buf[32];
@@ -101,8 +107,9 @@ void reg_bits(const RPhase& ph, int (&rb)[4]) {
 struct Layout {
     int rb[4];
     uint32_t tmask;
-    int tpos[kT];        // tile bit at each thread-id bit
-    uint32_t jr[16];     // tile index bits of register r
+    uint32_t nt;            // thread bits (L - 4)
+    int tpos[kTMax];        // tile bit at each thread-id bit
+    uint32_t jr[16];        // tile index bits of register r
 };
 
 // The pass's layouts with thread-bit orders for in-place transposes, and the
@@ -117,28 +124,31 @@ struct Layout {
 // a position. When the only thread bits the two layouts share are low bits of
 // an edge layout, the split is such a bit at its own lane position; the two
 // rounds then run on half the lanes of every wave.
-bool plan_layouts(const RTileProgram& rp, uint32_t run, std::vector<Layout>& lay, std::vector<int>& split,
+bool plan_layouts(const RTileProgram& rp, uint32_t L, uint32_t run, std::vector<Layout>& lay, std::vector<int>& split,
                   std::vector<int>& spos) {
     const size_t P = rp.phases.size();
+    const uint32_t kT = L - 4;
     lay.assign(P, Layout{});
     split.assign(P, -1);
     spos.assign(P, -1);
-    const uint32_t all = (1u << kL) - 1;
+    const uint32_t all = (1u << L) - 1;
     run = std::min<uint32_t>(run, 6);
     for (size_t k = 0; k < P; ++k) {
         reg_bits(rp.phases[k], lay[k].rb);
         uint32_t rm = 0;
         for (int i = 0; i < 4; ++i) rm |= 1u << lay[k].rb[i];
         lay[k].tmask = all & ~rm;
+        lay[k].nt = kT;
         for (int r = 0; r < 16; ++r) lay[k].jr[r] = rp.phases[k].jr[r];
     }
     // pins[k][p]: tile bit fixed at thread-id position p of layout k (-1 free)
-    std::vector<std::array<int, kT>> pins(P);
+    std::vector<std::array<int, kTMax>> pins(P);
     for (auto& a : pins) a.fill(-1);
     auto edge = [&](size_t k) { return k == 0 || k == P - 1; };
     // A layout without a transpose of its own repeats the previous one: it
     // shares that layout's pins (handled by copying orders below).
-    int wave_pos = 7;
+    int wave_pos = static_cast<int>(kT) - 1;  // the top two thread-id bits, alternating
+    const int wave_sum = 2 * static_cast<int>(kT) - 3;
     for (size_t k = 1; k < P; ++k) {
         if (!rp.phases[k].transpose) continue;
         size_t prev = k - 1;
@@ -155,13 +165,13 @@ bool plan_layouts(const RTileProgram& rp, uint32_t run, std::vector<Layout>& lay
         // Candidates: not pinned in the old layout already; high bits first
         // (highest first), then low bits.
         std::vector<int> cand;
-        for (int b = kL - 1; b >= 0; --b)
+        for (int b = static_cast<int>(L) - 1; b >= 0; --b)
             if ((common >> b & 1u) && !pinned_bit(prev, b) && !low_in_edge(b)) cand.push_back(b);
-        for (int b = kL - 1; b >= 0; --b)
+        for (int b = static_cast<int>(L) - 1; b >= 0; --b)
             if ((common >> b & 1u) && !pinned_bit(prev, b) && low_in_edge(b)) cand.push_back(b);
         bool placed = false;
         for (int h : cand) {
-            int positions[2] = {wave_pos, 13 - wave_pos};
+            int positions[2] = {wave_pos, wave_sum - wave_pos};
             int np = 2;
             if (low_in_edge(h)) {
                 positions[0] = h;
@@ -177,7 +187,7 @@ bool plan_layouts(const RTileProgram& rp, uint32_t run, std::vector<Layout>& lay
                 split[k] = h;
                 spos[k] = p;
                 placed = true;
-                if (p == wave_pos) wave_pos = 13 - wave_pos;
+                if (p == wave_pos) wave_pos = wave_sum - wave_pos;
             }
             if (placed) break;
         }
@@ -207,19 +217,19 @@ bool plan_layouts(const RTileProgram& rp, uint32_t run, std::vector<Layout>& lay
 }
 
 std::string jt_expr(const Layout& l, const char* tid = "t") {
-    std::vector<int> dst(kT);
-    for (uint32_t p = 0; p < kT; ++p) dst[p] = l.tpos[p];
+    std::vector<int> dst(l.nt);
+    for (uint32_t p = 0; p < l.nt; ++p) dst[p] = l.tpos[p];
     return scatter_expr(tid, dst, false);
 }
 
 // Global index bits of the thread in layout l (tile bit b is qubit gq[b]).
-std::string tglobal_expr(const Layout& l, const int (&gq)[kL]) {
-    std::vector<int> dst(kT);
-    for (uint32_t p = 0; p < kT; ++p) dst[p] = gq[l.tpos[p]];
+std::string tglobal_expr(const Layout& l, const int (&gq)[kLMax]) {
+    std::vector<int> dst(l.nt);
+    for (uint32_t p = 0; p < l.nt; ++p) dst[p] = gq[l.tpos[p]];
     return scatter_expr("tid", dst, true);
 }
 
-uint64_t reg_global(const Layout& l, int r, const int (&gq)[kL]) {
+uint64_t reg_global(const Layout& l, int r, const int (&gq)[kLMax]) {
     uint64_t g = 0;
     for (int i = 0; i < 4; ++i)
         if (r & (1 << i)) g |= uint64_t{1} << gq[l.rb[i]];
@@ -227,20 +237,23 @@ uint64_t reg_global(const Layout& l, int r, const int (&gq)[kL]) {
 }
 
 // Global loads (or stores) of the tile in layout l. When the thread's part
-// of the byte offset fits 32 bits, each register's address is the tile's
-// base plus a constant (scalar) and one 32-bit per-lane offset, the form the
-// load instructions take directly; otherwise 64-bit per-lane addresses.
-std::string edge_access(const Layout& l, const int (&gq)[kL], bool load, const std::string& NT) {
+// of the byte offset fits 32 bits, each register's address is a scalar base
+// (the tile's base plus the register's constant offset) and one 32-bit
+// per-lane offset, the form the load and store instructions take directly
+// (no 64-bit address arithmetic per lane); otherwise 64-bit per-lane
+// addresses.
+std::string edge_access(const Layout& l, const int (&gq)[kLMax], bool load, const std::string& NT) {
     uint64_t tmax = 0;
-    for (uint32_t p = 0; p < kT; ++p) tmax |= uint64_t{1} << gq[l.tpos[p]];
+    for (uint32_t p = 0; p < l.nt; ++p) tmax |= uint64_t{1} << gq[l.tpos[p]];
     std::string s;
     if ((tmax * sizeof(amp_t)) >> 32 == 0) {
-        std::vector<int> dst(kT);
-        for (uint32_t p = 0; p < kT; ++p) dst[p] = gq[l.tpos[p]] + 4;  // bytes
+        std::vector<int> dst(l.nt);
+        for (uint32_t p = 0; p < l.nt; ++p) dst[p] = gq[l.tpos[p]] + 4;  // bytes
         s += "        const sp_u32 o = " + scatter_expr("tid", dst, false) + ";\n";
         s += "        char* const pb = reinterpret_cast<char*>(b);\n";
         for (int r = 0; r < 16; ++r) {
-            const std::string a = "reinterpret_cast<sp_amp*>(pb + " + hex(reg_global(l, r, gq) * sizeof(amp_t)) + " + o)";
+            const std::string a = "(__attribute__((address_space(1))) sp_amp*)(sp::sbase(pb + " +
+                                  hex(reg_global(l, r, gq) * sizeof(amp_t)) + ") + o)";
             if (load) s += "        v[" + dec(r) + "] = sp::ld<" + NT + ">(" + a + ");\n";
             else s += "        sp::st<" + NT + ">(" + a + ", v[" + dec(r) + "]);\n";
         }
@@ -256,30 +269,33 @@ std::string edge_access(const Layout& l, const int (&gq)[kL], bool load, const s
 }
 
 // LDS slots of one half-tile transpose split by tile bit h. The index of
-// tile index j is c(j), j without bit h (11 bits); its slot is M c(j) for an
+// tile index j is c(j), j without bit h (L - 1 bits); its slot is M c(j) for an
 // invertible matrix M over GF(2) chosen per transpose so that the 16 lanes
 // of a writing or reading group (thread-id bits 0..3 of the old and of the
 // new layout) fall on 16 distinct slots modulo 16: 16-byte accesses then hit
 // distinct banks. Rows 0..3 of M: row b picks the old layout's lane bit b,
 // plus one new-layout lane bit that is not an old lane bit, placed in a row
 // that no shared lane bit occupies, so that both 4x4 blocks are permutations;
-// rows 4..10: the bits that are not old lane bits, in order. M is linear, so
+// rows 4 and up: the bits that are not old lane bits, in order. M is linear, so
 // slot(jt | jr) = slot(jt) ^ slot(jr): the thread's part is computed once
 // per layout, the register's part is a constant.
 struct SlotMap {
-    uint32_t col[11];  // M's column of each index bit
+    uint32_t n = 11;              // index bits (L - 1)
+    uint32_t col[kLMax - 1] = {};  // M's column of each index bit
     uint32_t cidx(int tile_bit, int h) const { return static_cast<uint32_t>(tile_bit < h ? tile_bit : tile_bit - 1); }
     uint32_t bytes(uint32_t j, int h) const {
         uint32_t c = (j & ((1u << h) - 1u)) | ((j >> (h + 1)) << h), s = 0;
-        for (int x = 0; x < 11; ++x)
+        for (uint32_t x = 0; x < n; ++x)
             if (c & (1u << x)) s ^= col[x];
         return s << 4;
     }
 };
 
-SlotMap slot_map(const Layout& prev, const Layout& cur, int h) {
+SlotMap slot_map(const Layout& prev, const Layout& cur, int h, uint32_t L) {
     SlotMap sm;
-    uint32_t row[11];
+    sm.n = L - 1;
+    const int nb = static_cast<int>(sm.n);
+    uint32_t row[kLMax - 1];
     // The first four lane bits of each layout other than the split bit (a
     // split among the lanes halves the active lanes of each group).
     int W[4], R[4];
@@ -304,14 +320,14 @@ SlotMap slot_map(const Layout& prev, const Layout& cur, int h) {
             }
     }
     int n = 4;
-    for (int x = 0; x < 11; ++x) {
+    for (int x = 0; x < nb; ++x) {
         bool lane = false;
         for (int b = 0; b < 4; ++b) lane |= W[b] == x;
         if (!lane) row[n++] = 1u << x;
     }
-    for (int x = 0; x < 11; ++x) {
+    for (int x = 0; x < nb; ++x) {
         sm.col[x] = 0;
-        for (int b = 0; b < 11; ++b)
+        for (int b = 0; b < nb; ++b)
             if (row[b] & (1u << x)) sm.col[x] |= 1u << b;
     }
     return sm;
@@ -321,7 +337,7 @@ SlotMap slot_map(const Layout& prev, const Layout& cur, int h) {
 // columns of its set thread-id bits (the split bit h excluded).
 std::string slot_thread_expr(const Layout& l, const SlotMap& sm, int h) {
     std::string e;
-    for (uint32_t p = 0; p < kT; ++p) {
+    for (uint32_t p = 0; p < l.nt; ++p) {
         if (l.tpos[p] == h) continue;
         const uint32_t c = sm.col[sm.cidx(l.tpos[p], h)] << 4;
         e += (e.empty() ? "" : " ^ ") + std::string("(((t >> ") + dec(p) + ") & 1u) * " + dec(c) + "u)";
@@ -429,31 +445,24 @@ bool emit_op(std::string& out, const RegOpHdr& h, uint32_t off, const amp_t* m, 
 
 // Waves per SIMD the compiler must fit: four (at most 128 VGPRs; most passes
 // need 95-120, and a pass that needs more would otherwise drop to three).
-// LQCS_SPEC_WAVES=5 asks for five (96 VGPRs, with a few spilled) for
-// comparisons.
-std::string waves_attr() {
-    int w = 4;
-    if (const char* e = std::getenv("LQCS_SPEC_WAVES")) {
-        const int v = std::atoi(e);
-        if (v >= 4 && v <= 8) w = v;
-    }
-    return " __attribute__((amdgpu_waves_per_eu(" + std::to_string(w) + ", 8)))";
-}
+// Five (96 VGPRs) spills a few registers and measured slower.
+std::string waves_attr() { return " __attribute__((amdgpu_waves_per_eu(4, 8)))"; }
 
 }  // namespace
 
-std::string spec_source(const RTileProgram& rp, uint32_t nq, uint64_t smask, bool nt) {
-    if (rp.phases.empty() || nq < kL) return {};
+std::string spec_source(const RTileProgram& rp, uint32_t nq, uint32_t L, uint64_t smask, bool nt) {
+    if (rp.phases.empty() || (L != 12 && L != 13) || nq < L) return {};
     std::vector<Layout> lay;
     std::vector<int> split, spos;
-    if (!plan_layouts(rp, static_cast<uint32_t>(__builtin_ctzll(~smask)), lay, split, spos)) return {};
-    int gq[kL];
+    if (!plan_layouts(rp, L, static_cast<uint32_t>(__builtin_ctzll(~smask)), lay, split, spos)) return {};
+    int gq[kLMax];
     {
         int t = 0;
-        for (int q = 0; q < 64 && t < static_cast<int>(kL); ++q)
+        for (int q = 0; q < 64 && t < static_cast<int>(L); ++q)
             if (smask & (uint64_t{1} << q)) gq[t++] = q;
-        if (t != static_cast<int>(kL)) return {};
+        if (t != static_cast<int>(L)) return {};
     }
+    const uint32_t threads = 1u << (L - 4);
     // Tile number -> base index: its bits on the qubits outside the tile set.
     std::vector<int> outside_dst;
     for (uint32_t q = 0; q < nq; ++q)
@@ -462,9 +471,9 @@ std::string spec_source(const RTileProgram& rp, uint32_t nq, uint64_t smask, boo
     const size_t P = rp.phases.size();
     const std::string NT = nt ? "true" : "false";
     std::string s;
-    s += "extern \"C\" __global__ void __launch_bounds__(256)" + waves_attr() +
+    s += "extern \"C\" __global__ void __launch_bounds__(" + dec(threads) + ")" + waves_attr() +
          " lqcs_pass(sp_amp* __restrict__ s, const sp_amp* __restrict__ w) {\n";
-    s += "    __shared__ sp_amp lds[2048];\n";
+    s += "    __shared__ sp_amp lds[" + dec(uint64_t{1} << (L - 1)) + "];\n";
     s += "    char* const lb = reinterpret_cast<char*>(lds);\n";
     s += "    const sp_u32 tid = threadIdx.x;\n";
     s += "    const sp_u32 tile = sp::xcd_tile(blockIdx.x, gridDim.x);\n";
@@ -479,14 +488,14 @@ std::string spec_source(const RTileProgram& rp, uint32_t nq, uint64_t smask, boo
             std::string d = "    {  // phase " + dec(k) + ": register bits";
             for (int i = 0; i < 4; ++i) d += " " + dec(lay[k].rb[i]);
             d += "; thread bits";
-            for (uint32_t p = 0; p < kT; ++p) d += " " + dec(lay[k].tpos[p]);
+            for (uint32_t p = 0; p < lay[k].nt; ++p) d += " " + dec(lay[k].tpos[p]);
             s += d + "\n";
         }
         s += "        const sp_u32 t = sp::fresh(tid);\n";
         if (k > 0 && ph.transpose) {
             const int h = split[k];
             const uint32_t pbit = 1u << spos[k];
-            const SlotMap sm = slot_map(lay[k - 1], lay[k], h);
+            const SlotMap sm = slot_map(lay[k - 1], lay[k], h, L);
             s += "        const sp_u32 sp0 = " + slot_thread_expr(lay[k - 1], sm, h) + ";\n";
             s += "        const sp_u32 sc0 = " + slot_thread_expr(lay[k], sm, h) + ";\n";
             for (int half = 0; half < 2; ++half) {
@@ -607,10 +616,11 @@ std::vector<hipFunction_t> spec_kernels(const std::vector<std::string>& sources)
 
 size_t spec_cache_size() { return cache().fn.size(); }
 
-void launch_spec(hipFunction_t f, amp_t* s, uint32_t nq, const amp_t* d_words, uint32_t batch) {
-    const uint64_t tiles = uint64_t{1} << (nq - kL);
+void launch_spec(hipFunction_t f, amp_t* s, uint32_t nq, uint32_t L, const amp_t* d_words, uint32_t batch) {
+    const uint64_t tiles = uint64_t{1} << (nq - L);
     void* args[] = {&s, &d_words};
-    LQCS_HIP(hipModuleLaunchKernel(f, static_cast<unsigned>(tiles), batch, 1, 256, 1, 1, 0, stream(), args, nullptr));
+    LQCS_HIP(hipModuleLaunchKernel(f, static_cast<unsigned>(tiles), batch, 1, 1u << (L - 4), 1, 1, 0, stream(), args,
+                                   nullptr));
 }
 
 }  // namespace lqcs_dev

@@ -46,6 +46,15 @@ SP_INLINE sp_u32 fresh(sp_u32 x) {
     return x;
 }
 
+// A wave-uniform address kept in scalar registers: loads and stores then take
+// it as their base with one 32-bit per-lane offset.
+typedef __attribute__((address_space(1))) char sp_gchar;
+SP_INLINE sp_gchar* sbase(char* p) {
+    sp_gchar* g = (sp_gchar*)p;
+    asm volatile("" : "+s"(g));
+    return g;
+}
+
 // One matrix entry of the op words: the same address in every lane, so a
 // scalar load.
 SP_INLINE sp_amp uni(const sp_amp* __restrict__ p) {

@@ -139,7 +139,8 @@ struct RTileProgram {
 // than 4 register bits or the tile width is not the register tile's.
 // Register programs: L = 12 (four waves per tile) or L = 10 (one wave per
 // tile, so transposes involve no other wave); both transpose through LDS in
-// two halves. False for other widths or ops that do not fit.
+// two halves. L = 13 (eight waves per tile) builds programs that only the
+// specialized passes run. False for other widths or ops that do not fit.
 bool build_rtile(const std::vector<TileOp>& tops, uint32_t L, uint64_t smask, uint32_t nq, RTileProgram& out);
 void launch_rtile(amp_t* s, uint32_t nq, uint32_t L, uint64_t smask, const RPhase* d_phases, uint32_t n_phases,
                   const amp_t* d_words, uint32_t batch = 1);

@@ -532,30 +532,33 @@ PYBIND11_MODULE(_pylqcs, m) {
         .def("num_qubits", &CompiledCircuit::num_qubits)
         .def("num_ops", &CompiledCircuit::num_ops)
         .def("describe",
-             [](const CompiledCircuit& c, const std::string& plan, int tile_qubits, int low_qubits) {
+             [](const CompiledCircuit& c, const std::string& plan, int tile_qubits, int low_qubits,
+                const std::string& tile_kernel) {
                  EngineOptions e;
                  e.plan = plan_of(plan);
                  e.tile_qubits = tile_qubits;
                  e.low_qubits = low_qubits;
+                 e.tile_kernel = tile_kernel_of(tile_kernel);
                  py::list out;
                  for (const auto& p : c.describe(e))
                      out.append(py::dict("ops"_a = p.ops, "tile_mask"_a = p.tile_mask, "phases"_a = p.phases));
                  return out;
              },
-             "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0)
+             "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0, "tile_kernel"_a = "default")
         .def("specialized_pass",
              [](const CompiledCircuit& c, std::uint64_t step, const std::string& plan, int tile_qubits,
-                int low_qubits) {
+                int low_qubits, const std::string& tile_kernel) {
                  EngineOptions e;
                  e.plan = plan_of(plan);
                  e.tile_qubits = tile_qubits;
                  e.low_qubits = low_qubits;
+                 e.tile_kernel = tile_kernel_of(tile_kernel);
                  const CompiledCircuit::SpecializedPass sp = c.specialized_pass(step, e);
                  py::array_t<double> words({static_cast<py::ssize_t>(sp.words.size() / 2), py::ssize_t{2}});
                  if (!sp.words.empty()) std::memcpy(words.mutable_data(), sp.words.data(), sp.words.size() * sizeof(double));
                  return py::make_tuple(sp.text, words);
              },
-             "step"_a, "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0,
+             "step"_a, "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0, "tile_kernel"_a = "default",
              "kernel text and op words of the specialized kernel of one planned pass (no device needed)")
         .def("features",
              [](const CompiledCircuit& c, const std::string& plan, int tile_qubits, int low_qubits,

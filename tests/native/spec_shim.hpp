@@ -37,24 +37,25 @@ inline int __popc(unsigned v) { return __builtin_popcount(v); }
 inline long long __double_as_longlong(double v) { return std::bit_cast<long long>(v); }
 inline double __longlong_as_double(long long v) { return std::bit_cast<double>(v); }
 
-// Runs kernel k on every workgroup of a (tiles x batch) grid of 256 threads.
+// Runs kernel k on every workgroup of a (tiles x batch) grid of `threads`
+// threads each.
 template <class K>
-void shim_launch(K k, unsigned tiles, unsigned batch) {
+void shim_launch(K k, unsigned tiles, unsigned batch, unsigned threads = 256) {
     gridDim.x = tiles;
     gridDim.y = batch;
     for (unsigned by = 0; by < batch; ++by)
         for (unsigned bx = 0; bx < tiles; ++bx) {
-            std::barrier<> bar(256);
+            std::barrier<> bar(threads);
             shim_barrier = &bar;
-            std::vector<std::thread> threads;
-            threads.reserve(256);
-            for (unsigned t = 0; t < 256; ++t)
-                threads.emplace_back([&, t] {
+            std::vector<std::thread> pool;
+            pool.reserve(threads);
+            for (unsigned t = 0; t < threads; ++t)
+                pool.emplace_back([&, t] {
                     threadIdx.x = t;
                     blockIdx.x = bx;
                     blockIdx.y = by;
                     k();
                 });
-            for (auto& th : threads) th.join();
+            for (auto& th : pool) th.join();
         }
 }

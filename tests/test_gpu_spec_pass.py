@@ -65,6 +65,9 @@ CASES = [
     ("u3cx", 16, lambda: _u3cx(16, 8, 3), dict(plan="greedy")),
     ("u3cx_gain", 18, lambda: _u3cx(18, 8, 4), dict(plan="gain", low_qubits=3)),
     ("u3cx_lane_split", 15, lambda: _u3cx(15, 6, 16), dict(plan="greedy")),
+    ("u3cx_13q_tiles", 17, lambda: _u3cx(17, 8, 5), dict(plan="greedy", tile_qubits=13, low_qubits=5)),
+    ("random16kinds_13q_tiles", 15, lambda: _spread(cases.random_circuit(15, 200, 9), 15, 9),
+     dict(plan="gain", tile_qubits=13)),
     ("random16kinds", 14, lambda: _spread(cases.random_circuit(14, 200, 4), 14, 5), dict(plan="greedy")),
     ("every_gate", 13, lambda: _spread(cases.every_gate_circuit(13), 13, 6), dict(plan="greedy")),
     ("rotations", 15, lambda: _spread(_rotations(15, 5), 15, 7), dict(plan="greedy")),
@@ -78,12 +81,14 @@ def test_specialized_passes_match_oracle_and_interpreter(lq, name, n, make, plan
     circ = make()
     comp = lq.CompiledCircuit(circ.to_product(lq), True)
     want = O.run_statevector(circ, fuse=True)
+    plan = dict(plan)
+    L = plan.pop("tile_qubits", 12)
     spec = lq.statevector(n)
-    comp.apply(spec, tile_qubits=12, tile_kernel="specialized", **plan)
+    comp.apply(spec, tile_qubits=L, tile_kernel="specialized", **plan)
     got = np.array(spec.to_numpy())
     assert np.max(np.abs(got - want)) <= 1e-12
-    interp = lq.statevector(n)
-    comp.apply(interp, tile_qubits=12, tile_kernel="register", **plan)
+    interp = lq.statevector(n)  # the interpreting kernel stops at 12-qubit tiles
+    comp.apply(interp, tile_qubits=min(L, 12), tile_kernel="register", **plan)
     assert spec.max_abs_diff(interp) <= 1e-12
 
 
@@ -101,6 +106,30 @@ def test_specialized_passes_on_a_4gib_register(lq):
     comp.apply(b, plan="greedy", tile_kernel="register")
     assert a.max_abs_diff(b) <= 1e-12
     assert abs(a.norm() - 1.0) <= 1e-10
+
+
+def test_13_qubit_tiles_on_a_4gib_register(lq):
+    """28 qubits on 13-qubit tiles (512 threads, 64 KiB of LDS per tile) with
+    five contiguous low qubits, against the interpreting kernel on 12-qubit
+    tiles."""
+    n = 28
+    print(f"specialized passes: 13-qubit tiles n={n}", flush=True)
+    qc = lq.QuantumCircuit(n)
+    W.random_u3_cx_layers(qc, n, 6, seed=29)
+    comp = lq.CompiledCircuit(qc, True)
+    a = lq.statevector(n)
+    comp.apply(a, plan="greedy", tile_qubits=13, low_qubits=5, tile_kernel="specialized")
+    b = lq.statevector(n)
+    comp.apply(b, plan="greedy", tile_kernel="register")
+    assert a.max_abs_diff(b) <= 1e-12
+
+
+def test_13_qubit_tiles_need_specialized_passes(lq):
+    qc = lq.QuantumCircuit(16)
+    W.random_u3_cx_layers(qc, 16, 2, seed=1)
+    comp = lq.CompiledCircuit(qc, True)
+    with pytest.raises(Exception, match="tile width"):
+        comp.apply(lq.statevector(16), plan="greedy", tile_qubits=13, tile_kernel="register")
 
 
 def test_specialized_passes_replay_in_a_captured_graph(lq):
