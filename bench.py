@@ -309,7 +309,7 @@ def _ps_cpu_baseline(W, sizes, seed: int, budget_s: float = 30.0) -> dict | None
             **cpu_fields(cores), "kind": kind, "sample": sample}
 
 
-def tile_roofline(prof: dict, traffic_json: str | None = None) -> dict | None:
+def tile_roofline(prof: dict, traffic_json: str | None = None, kernel: str = "k_rtile") -> dict | None:
     """Roofline of the tiled passes in a launch profile: every pass reads and
     writes each amplitude of the registers it covers once (32 bytes per
     amplitude), timed by HIP events around each launch on the engine stream."""
@@ -329,7 +329,7 @@ def tile_roofline(prof: dict, traffic_json: str | None = None) -> dict | None:
             pass
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": traffic, "traffic_source": source,
-            "kernel": "k_rtile", "bytes_per_launch": by / prof["tile_launches"],
+            "kernel": kernel, "bytes_per_launch": by / prof["tile_launches"],
             "avg_launch_ms": round(prof["tile_ms"] / prof["tile_launches"], 4),
             "launches": prof["tile_launches"], "share_of_kernel_ms":
                 round(prof["tile_ms"] / (prof["tile_ms"] + prof["other_ms"]), 4)}
@@ -421,7 +421,8 @@ def trajectories_small_bench(lq, W, seed: int, shots: int = 2048, widths=(8, 12)
     return out
 
 
-def density_matrix_bench(lq, W, seed: int, widths=(12, 14, 16), cpu: bool = True, cpu_width: int = 10) -> dict:
+def density_matrix_bench(lq, W, seed: int, widths=(12, 14, 16), cpu: bool = True, cpu_width: int = 10,
+                         tile_kernel: str = "specialized") -> dict:
     """The noisy workload's exact form: rho of GHZ prep + 10 random U3/CZ
     layers with depolarizing(1e-3) then amplitude_damping(1e-3) after every
     gate, on the device (a 2n-qubit register, tiled passes), at the
@@ -433,12 +434,16 @@ def density_matrix_bench(lq, W, seed: int, widths=(12, 14, 16), cpu: bool = True
     noise.add_all_qubit_channel(lq.KrausChannel.depolarizing(1e-3))
     noise.add_all_qubit_channel(lq.KrausChannel.amplitude_damping(1e-3))
     out = {"workload": "rho of GHZ prep + 10 random U3/CZ layers, depolarizing(1e-3) + amplitude_damping(1e-3) "
-                       "after every gate"}
+                       "after every gate", "tile_kernel": tile_kernel}
+    kname = "lqcs_pass" if tile_kernel == "specialized" else "k_rtile"
     for n in widths:
         qc = lq.QuantumCircuit(n, n)
         W.noisy_ghz_layers(qc, n, 10, seed)
-        sim = lq.DensityMatrixSimulator(noise=noise, max_qubits=16)
-        sim.run_density_matrix(qc)  # warm-up
+        sim = lq.DensityMatrixSimulator(noise=noise, max_qubits=16, tile_kernel=tile_kernel)
+        t0 = time.perf_counter()
+        sim.run_density_matrix(qc)  # warm-up (compiles specialized passes)
+        lq.synchronize()
+        first_s = time.perf_counter() - t0
         lq.synchronize()
         lq.profile_enable(True)
         lq.profile_read()
@@ -450,10 +455,11 @@ def density_matrix_bench(lq, W, seed: int, widths=(12, 14, 16), cpu: bool = True
         lq.profile_enable(False)
         passes = prof["tile_launches"] + prof["other_launches"]
         kms = prof["tile_ms"] + prof["other_ms"]
-        out[f"n{n}"] = {"seconds": round(dt, 4), "gates": qc.num_instructions(), "launches": passes,
+        out[f"n{n}"] = {"seconds": round(dt, 4), "first_run_s": round(first_s, 3), "gates": qc.num_instructions(),
+                        "launches": passes,
                         "kernel_ms": round(kms, 3), "trace": rho.trace(),
                         "GB/s": round(passes * 32 * 4**n / (kms * 1e6), 1) if kms else None,
-                        "roofline": tile_roofline(prof, f"k_rtile_density_matrix_{n}q.json")}
+                        "roofline": tile_roofline(prof, f"{kname}_density_matrix_{n}q.json", kname)}
         log(f"density matrix {n} qubits: {dt:.3f} s, {passes} launches, {kms:.1f} ms of kernels")
         del rho
     out["roofline"] = out[f"n{widths[-1]}"]["roofline"]
@@ -796,7 +802,7 @@ def main() -> int:
         planner["at_32_qubits"] = planner_32q(lq, W, args.seed)
     dm = None
     if not args.no_density_matrix:
-        dm = density_matrix_bench(lq, W, args.seed, cpu=not args.no_cpu_baseline)
+        dm = density_matrix_bench(lq, W, args.seed, cpu=not args.no_cpu_baseline, tile_kernel=args.tile_kernel)
     dense = None
     if not args.no_dense:
         dense = dense_blocks_bench(lq, args.seed, cpu=not args.no_cpu_baseline)

@@ -261,9 +261,11 @@ PYBIND11_MODULE(_pylqcs, m) {
     py::class_<StatevectorSimulator>(m, "StatevectorSimulator")
         .def(py::init([](std::optional<std::uint64_t> seed, bool fuse_gates, int num_threads, NoiseModel noise,
                          const std::string& plan, int tile_qubits, int low_qubits, std::size_t max_qubits,
-                         std::uint32_t max_batch, bool eliminate_redundant, const std::string& trajectory_kernel) {
+                         std::uint32_t max_batch, bool eliminate_redundant, const std::string& trajectory_kernel,
+                         const std::string& tile_kernel) {
                  StatevectorSimulator::Options o;
                  o.engine.trajectory_kernel = trajectory_kernel_of(trajectory_kernel);
+                 o.engine.tile_kernel = tile_kernel_of(tile_kernel);
                  o.engine.eliminate_redundant = eliminate_redundant;
                  o.seed = seed;
                  o.fuse_gates = fuse_gates;
@@ -278,7 +280,7 @@ PYBIND11_MODULE(_pylqcs, m) {
              }),
              "seed"_a = py::none(), "fuse_gates"_a = true, "num_threads"_a = 0, "noise"_a = NoiseModel{},
              "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0, "max_qubits"_a = 30, "max_batch"_a = 0,
-             "eliminate_redundant"_a = false, "trajectory_kernel"_a = "auto")
+             "eliminate_redundant"_a = false, "trajectory_kernel"_a = "auto", "tile_kernel"_a = "default")
         .def("run", &StatevectorSimulator::run, "circuit"_a, "shots"_a = 1024,
              py::call_guard<py::gil_scoped_release>())
         .def("run_statevector", &StatevectorSimulator::run_statevector, "circuit"_a,
@@ -350,18 +352,19 @@ PYBIND11_MODULE(_pylqcs, m) {
 
     py::class_<DensityMatrixSimulator>(m, "DensityMatrixSimulator")
         .def(py::init([](std::optional<std::uint64_t> seed, NoiseModel noise, const std::string& plan,
-                         int tile_qubits, int low_qubits, std::size_t max_qubits) {
+                         int tile_qubits, int low_qubits, std::size_t max_qubits, const std::string& tile_kernel) {
                  DensityMatrixSimulator::Options o;
                  o.seed = seed;
                  o.noise = std::move(noise);
                  o.engine.plan = plan_of(plan);
                  o.engine.tile_qubits = tile_qubits;
                  o.engine.low_qubits = low_qubits;
+                 o.engine.tile_kernel = tile_kernel_of(tile_kernel);
                  o.max_qubits = max_qubits;
                  return DensityMatrixSimulator(std::move(o));
              }),
              "seed"_a = py::none(), "noise"_a = NoiseModel{}, "plan"_a = "auto", "tile_qubits"_a = 0,
-             "low_qubits"_a = 0, "max_qubits"_a = DensityMatrix::kMaxQubits)
+             "low_qubits"_a = 0, "max_qubits"_a = DensityMatrix::kMaxQubits, "tile_kernel"_a = "default")
         .def("run", &DensityMatrixSimulator::run, "circuit"_a, "shots"_a = 1024,
              py::call_guard<py::gil_scoped_release>())
         .def("run_density_matrix", &DensityMatrixSimulator::run_density_matrix, "circuit"_a,

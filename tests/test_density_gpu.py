@@ -155,7 +155,23 @@ def test_density_matrix_matches_oracle(lq, n, seed, spec, plan):
     assert np.max(np.abs(got - want)) <= TOL
 
 
-def test_noisy_ghz12_matches_fixture(lq):
+@pytest.mark.parametrize("n,seed", [(7, 3), (8, 4)])
+@pytest.mark.parametrize("spec", [0, 1])
+def test_density_matrix_on_specialized_passes_matches_oracle(lq, n, seed, spec):
+    """rho's 2n-qubit register on 12-qubit tiles with specialized passes (real
+    4x4 superoperators, ket and bra halves of every gate) against the oracle's
+    density matrix."""
+    c = _random_noisy(n, seed)
+    spec = NOISE_SPECS[spec]
+    want = O.density_matrix(c, _orc_channels(spec))
+    sim = lq.DensityMatrixSimulator(noise=_channels(lq, spec), plan="greedy", tile_qubits=12,
+                                    tile_kernel="specialized")
+    got = sim.run_density_matrix(c.to_product(lq)).to_numpy()
+    assert np.max(np.abs(got - want)) <= TOL
+
+
+@pytest.mark.parametrize("kernel", ["default", "specialized"])
+def test_noisy_ghz12_matches_fixture(lq, kernel):
     """The 12-qubit noisy GHZ workload against the committed oracle fixture."""
     import os
 
@@ -163,7 +179,8 @@ def test_noisy_ghz12_matches_fixture(lq):
 
     want = np.load(os.path.join(os.path.dirname(__file__), "golden", "dm_noisy_ghz12.npy"))
     c, _ = cases.noisy_ghz(12)
-    sim = lq.DensityMatrixSimulator(noise=_channels(lq, [("depolarizing", 1e-3), ("amplitude_damping", 1e-3)]))
+    sim = lq.DensityMatrixSimulator(noise=_channels(lq, [("depolarizing", 1e-3), ("amplitude_damping", 1e-3)]),
+                                    tile_kernel=kernel)
     rho = sim.run_density_matrix(c.to_product(lq))
     assert np.max(np.abs(rho.probabilities() - want)) <= TOL
     assert abs(rho.trace() - 1.0) <= 1e-12
