@@ -525,11 +525,40 @@ def planner_32q(lq, W, seed: int, n: int = 32, depth: int = 40) -> dict:
     choice = lq.last_plan_choice()
     diff = a.max_abs_diff(b)
     log(f"planner at {n} qubits: greedy {t_greedy:.3f} s, learned {t_learned:.3f} s ({choice}), max diff {diff:.2e}")
-    return {"workload": f"random U3/CX/CP layers, {n} qubits, depth {depth}, {qc.num_instructions()} gates",
-            "greedy_s": round(t_greedy, 3), "learned_s": round(t_learned, 3), "learned_pick": choice,
-            "learned_vs_greedy": round(t_learned / t_greedy, 4),
-            "same_plan_as_greedy": choice["plan"] == "greedy" and choice["low_qubits"] == 4,
-            "max_abs_diff_states": diff}
+    out = {"workload": f"random U3/CX/CP layers, {n} qubits, depth {depth}, {qc.num_instructions()} gates",
+           "kernel": "k_rtile", "greedy_s": round(t_greedy, 3), "learned_s": round(t_learned, 3), "learned_pick": choice,
+           "learned_vs_greedy": round(t_learned / t_greedy, 4),
+           "same_plan_as_greedy": choice["plan"] == "greedy" and choice["low_qubits"] == 4,
+           "max_abs_diff_states": diff}
+    # The same on specialized passes (the specialized-pass model picks among
+    # 12- and 13-qubit tiles); first applies compile, then alternate.
+    ks = dict(tile_kernel="specialized")
+    t0 = time.perf_counter()
+    comp.apply(a, plan="greedy", **ks)
+    comp.apply(b, plan="learned", **ks)
+    lq.synchronize()
+    compile_s = time.perf_counter() - t0
+    schoice = lq.last_plan_choice()  # the learned pick (later applies of the same program reuse it)
+    sg = sl = float("inf")
+    for rep in range(2):
+        for plan in (("greedy", "learned") if rep % 2 == 0 else ("learned", "greedy")):
+            reg = a if plan == "greedy" else b
+            reg.set_basis(0)
+            lq.synchronize()
+            t0 = time.perf_counter()
+            comp.apply(reg, plan=plan, **ks)
+            lq.synchronize()
+            dt = time.perf_counter() - t0
+            if plan == "greedy":
+                sg = min(sg, dt)
+            else:
+                sl = min(sl, dt)
+    sdiff = a.max_abs_diff(b)
+    log(f"planner at {n} qubits, specialized: greedy {sg:.3f} s, learned {sl:.3f} s ({schoice}), max diff {sdiff:.2e}")
+    out["specialized"] = {"greedy_s": round(sg, 3), "learned_s": round(sl, 3), "learned_pick": schoice,
+                          "learned_vs_greedy": round(sl / sg, 4), "first_applies_s": round(compile_s, 3),
+                          "max_abs_diff_states": sdiff}
+    return out
 
 
 PEAK_FP64_TFLOPS = 78.6
@@ -769,7 +798,7 @@ def main() -> int:
         # on 12- and 13-qubit tiles (13 runs only on specialized passes).
         spec = {}
         for name, L, plan, low in (("greedy_12q_low4", 12, "greedy", 4), ("gain_12q_low3", 12, "gain", 3),
-                                   ("gain_13q_low3", 13, "gain", 3)):
+                                   ("gain_13q_low3", 13, "gain", 3), ("learned", 0, "learned", 0)):
             kw = dict(plan=plan, tile_qubits=L, low_qubits=low, tile_kernel="specialized")
             t0 = time.perf_counter()
             comp.apply(sv, **kw)  # compiles its passes
@@ -785,7 +814,10 @@ def main() -> int:
                 best = min(best, (time.perf_counter() - t0) * 1e3)
             spec[name] = {"ms_per_step": round(best, 3), "gates_per_s": round(gates / best * 1e3, 1),
                           "passes": lq.last_plan_stats()["passes"], "first_apply_s": round(compile_s, 3)}
+            if plan == "learned":
+                spec[name]["pick"] = lq.last_plan_choice()
             log(f"specialized {name}: {best:.1f} ms/step")
+        spec["learned_vs_greedy"] = round(spec["learned"]["ms_per_step"] / spec["greedy_12q_low4"]["ms_per_step"], 4)
         planner["specialized_plans"] = spec
 
     adjoint = None

@@ -126,8 +126,10 @@ struct CostModel {
     double t_mean = 0.0, t_scale = 1.0;
 };
 std::uint64_t mlp_num_params(std::uint32_t n_features);
-// Installs (or, with empty params, removes) the model Plan::Learned consults.
-void install_cost_model(const CostModel& m);
+// Installs (or, with empty params, removes) the model Plan::Learned consults
+// for passes on the tile kernel EngineOptions::tile_kernel names: the
+// interpreting register tile (any value but 5) or specialized passes (5).
+void install_cost_model(const CostModel& m, int tile_kernel = 0);
 // Reads a model written by litequantumcomputingsimulator_amd.costmodel:
 // "lqcs-costmodel 1 <F> <n_params>" then x_mean, x_scale, t_mean, t_scale
 // and the parameters, whitespace separated.
@@ -136,6 +138,7 @@ struct PlanChoice {
     EngineOptions::Plan plan;
     int low_qubits;
     double predicted_ms;  // -1 when no model decided
+    int tile_qubits;      // tile width of the plan (0: gate by gate)
 };
 PlanChoice last_plan_choice();
 

@@ -229,6 +229,13 @@ lqcs_status lqcs_mlp_train(uint32_t n_features, double* params, const double* x,
  * params NULL uninstalls it. */
 lqcs_status lqcs_costmodel_install(uint32_t n_features, const double* params, const double* x_mean,
                                    const double* x_scale, double t_mean, double t_scale);
+/* The same for the model of one tile kernel: LQCS_TILE_REGISTER (the model
+ * above) or LQCS_TILE_SPECIALIZED, which LQCS_PLAN_LEARNED consults for the
+ * calls that select specialized passes; its candidates then include 13-qubit
+ * tiles and five low qubits. */
+lqcs_status lqcs_costmodel_install_kernel(int32_t tile_kernel, uint32_t n_features, const double* params,
+                                          const double* x_mean, const double* x_scale, double t_mean,
+                                          double t_scale);
 /* Features of every step of the plan lqcs_state_apply would run for these
  * options on a register of n_qubits (batch members per launch), in step
  * order, LQCS_COST_FEATURES values per step; arrays hold max_steps rows. */
@@ -237,6 +244,8 @@ lqcs_status lqcs_plan_features(uint32_t n_qubits, const lqcs_program* prog, cons
 /* Plan kind and low-qubit count the last apply on this thread ran, and the
  * cost model's predicted milliseconds for it (-1 when no model decided). */
 lqcs_status lqcs_last_plan_choice(int32_t* plan, int32_t* low_qubits, double* predicted_ms);
+/* Tile width of that plan (0: gate by gate). */
+lqcs_status lqcs_last_plan_tile_qubits(int32_t* tile_qubits);
 
 /* ---- reductions and measurement ---------------------------------------- */
 /* Statevector::norm (statevector.cpp:27-31) squared, before the sqrt. */

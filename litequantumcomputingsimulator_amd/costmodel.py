@@ -2,8 +2,9 @@
 
 The engine's LEARNED plan asks a small network how long each pass of a
 candidate plan will take. The candidates are greedy and gain tile grouping at
-2, 3 and 4 low qubits. It then runs the candidate with the smallest predicted
-total. The network is a multilayer perceptron COST_FEATURES -> 64 -> 64 -> 1
+2, 3 and 4 low qubits on the interpreting register tile, or at 3, 4 and 5 low
+qubits on 12- and 13-qubit tiles for specialized passes (a model per kernel).
+It then runs the candidate with the smallest predicted total. The network is a multilayer perceptron COST_FEATURES -> 64 -> 64 -> 1
 (ReLU, FP64) on per-pass features: register phases, LDS transposes, op counts
 by kind, lane spread of the load layout, tile qubit positions, register width
 and batch; for stand-alone ops, k, controls and qubit positions. It predicts
@@ -32,8 +33,11 @@ from . import workloads as W
 FORMAT = "lqcs-costmodel"
 
 
-def default_model_path() -> str:
-    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "costmodel.txt")
+def default_model_path(kernel: str = "register") -> str:
+    """The packaged model for passes on the interpreting register tile
+    (costmodel.txt) or on specialized passes (costmodel_specialized.txt)."""
+    name = "costmodel_specialized.txt" if kernel == "specialized" else "costmodel.txt"
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), name)
 
 
 @dataclass
@@ -66,8 +70,9 @@ class Model:
             raise ValueError(f"{path}: truncated")
         return Model(vals[2 * F + 2:], vals[:F], vals[F:2 * F], float(vals[2 * F]), float(vals[2 * F + 1]))
 
-    def install(self, lq) -> None:
-        lq.costmodel_install(list(self.params), list(self.x_mean), list(self.x_scale), self.t_mean, self.t_scale)
+    def install(self, lq, kernel: str = "register") -> None:
+        lq.costmodel_install(list(self.params), list(self.x_mean), list(self.x_scale), self.t_mean, self.t_scale,
+                             tile_kernel=kernel)
 
     def predict_ms(self, lq, features: np.ndarray) -> np.ndarray:
         x = (np.asarray(features, float) - self.x_mean) / self.x_scale
@@ -107,31 +112,39 @@ def random_circuit(lq, n: int, family: str, depth: int, seed: int):
 
 FAMILIES = ("u3cx", "u3cxcp", "pauli", "local", "qft")
 CANDIDATES = tuple((plan, low) for plan in ("greedy", "gain") for low in (2, 3, 4))
+# Specialized passes: (plan, tile width, low qubits), as the engine plans them
+# for registers of 17 qubits and more.
+SPEC_CANDIDATES = tuple((plan, L, low) for L in (12, 13) for plan in ("greedy", "gain") for low in (3, 4, 5))
 # The engine keeps greedy grouping at the default low qubits (4 at these
 # widths) unless another candidate is predicted faster by this fraction.
 MARGIN = 0.02
 GREEDY_DEFAULT = ("greedy", 4)
+SPEC_GREEDY_DEFAULT = ("greedy", 12, 4)
 
 
 def pick(pred: dict) -> tuple:
-    """The engine's LEARNED choice among predicted totals {(plan, low): ms}."""
+    """The engine's LEARNED choice among predicted totals {candidate: ms}
+    (candidates as in CANDIDATES or SPEC_CANDIDATES)."""
     best = min(pred, key=pred.get)
-    if GREEDY_DEFAULT in pred and pred[best] > (1.0 - MARGIN) * pred[GREEDY_DEFAULT]:
-        return GREEDY_DEFAULT
+    base = SPEC_GREEDY_DEFAULT if SPEC_GREEDY_DEFAULT in pred else GREEDY_DEFAULT
+    if base in pred and pred[best] > (1.0 - MARGIN) * pred[base]:
+        return base
     return best
 
 
-def time_plan(lq, comp, sv, plan: str, low: int, reps: int = 1):
+def time_plan(lq, comp, sv, plan: str, low: int, reps: int = 1, L: int = 12, kernel: str = "register"):
     """(features, per-pass ms, total ms) of one plan of a compiled circuit:
-    one untimed apply, then `reps` timed applies (per-pass times averaged)."""
-    feats = comp.features(plan=plan, tile_qubits=12, low_qubits=low)
-    comp.apply(sv, plan=plan, tile_qubits=12, low_qubits=low)
+    one untimed apply (which compiles specialized passes), then `reps` timed
+    applies (per-pass times averaged)."""
+    kw = dict(plan=plan, tile_qubits=L, low_qubits=low, tile_kernel=kernel)
+    feats = comp.features(**kw)
+    comp.apply(sv, **kw)
     lq.synchronize()
     lq.profile_enable(True)
     lq.profile_read_launches()
     acc = None
     for _ in range(reps):
-        comp.apply(sv, plan=plan, tile_qubits=12, low_qubits=low)
+        comp.apply(sv, **kw)
         ms, _ = lq.profile_read_launches()
         if len(ms) != len(feats):
             raise RuntimeError(f"{len(ms)} launches timed for a {len(feats)}-pass plan")
@@ -141,25 +154,38 @@ def time_plan(lq, comp, sv, plan: str, low: int, reps: int = 1):
     return feats, ms, float(ms.sum())
 
 
-def generate(lq, specs, reps: int = 1, log=print):
+def generate(lq, specs, reps: int = 1, log=print, kernel: str = "register"):
     """Times every candidate plan of every (n, family, depth, seed) in specs.
     Returns (X, t, records): per-pass feature rows and t = log(ms), and per
-    circuit a record {"spec", "cands": [(plan, low, features, total_ms)]}."""
+    circuit a record {"spec", "cands": [(candidate, features, total_ms)]},
+    the candidate as in CANDIDATES (register tile) or SPEC_CANDIDATES."""
     X, T, records = [], [], []
+    cands_of = SPEC_CANDIDATES if kernel == "specialized" else CANDIDATES
+    base = SPEC_GREEDY_DEFAULT if kernel == "specialized" else GREEDY_DEFAULT
     for n, family, depth, seed in specs:
         qc = random_circuit(lq, n, family, depth, seed)
         comp = lq.CompiledCircuit(qc, True)
         sv = lq.statevector(n)
         cands = []
-        for plan, low in CANDIDATES:
-            f, ms, tot = time_plan(lq, comp, sv, plan, low, reps)
+        for cand in cands_of:
+            if kernel == "specialized":
+                plan, L, low = cand
+                try:
+                    f, ms, tot = time_plan(lq, comp, sv, plan, low, reps, L=L, kernel=kernel)
+                except ValueError:
+                    if L == 12:
+                        raise
+                    continue  # 13-qubit tiles refuse Pauli rotations wider than 4 qubits; so does the engine's pick
+            else:
+                plan, low = cand
+                f, ms, tot = time_plan(lq, comp, sv, plan, low, reps)
             X.append(f)
             T.append(np.log(np.maximum(ms, 1e-4)))
-            cands.append((plan, low, f, tot))
+            cands.append((cand, f, tot))
         records.append({"spec": (n, family, depth, seed), "cands": cands})
-        best = min(cands, key=lambda c: c[3])
-        log(f"timed n={n} {family} depth={depth} seed={seed}: best {best[0]}/{best[1]} {best[3]:.3f} ms, "
-            f"greedy/4 {[c[3] for c in cands if c[:2] == ('greedy', 4)][0]:.3f} ms")
+        best = min(cands, key=lambda c: c[2])
+        log(f"timed n={n} {family} depth={depth} seed={seed}: best {best[0]} {best[2]:.3f} ms, "
+            f"greedy {[c[2] for c in cands if c[0] == base][0]:.3f} ms")
         del sv
     return np.concatenate(X), np.concatenate(T), records
 
@@ -170,11 +196,12 @@ def evaluate(lq, model: "Model", records):
     model) and of the best candidate in hindsight."""
     out = []
     for rec in records:
-        pred = {(p, lw): float(model.predict_ms(lq, f).sum()) for (p, lw, f, _) in rec["cands"]}
+        pred = {c: float(model.predict_ms(lq, f).sum()) for (c, f, _) in rec["cands"]}
         k = pick(pred)
-        meas = {(p, lw): t for (p, lw, _, t) in rec["cands"]}
+        meas = {c: t for (c, _, t) in rec["cands"]}
+        base = SPEC_GREEDY_DEFAULT if SPEC_GREEDY_DEFAULT in meas else GREEDY_DEFAULT
         out.append({"spec": list(rec["spec"]), "pick": list(k),
-                    "learned_ms": meas[k], "greedy_ms": meas[GREEDY_DEFAULT],
+                    "learned_ms": meas[k], "greedy_ms": meas[base],
                     "best_ms": min(meas.values()), "predicted_ms": pred[k]})
     return out
 
@@ -207,7 +234,11 @@ def train(lq, X: np.ndarray, T: np.ndarray, epochs: int = 300, batch_rows: int =
 
 
 def install_default(lq) -> bool:
-    """Installs the packaged model if present; returns whether it was."""
+    """Installs the packaged models (register tile, specialized passes) that
+    are present; returns whether the register tile's was."""
+    spec = default_model_path("specialized")
+    if os.path.exists(spec):
+        lq.costmodel_load(spec, tile_kernel="specialized")
     path = default_model_path()
     if not os.path.exists(path):
         return False
@@ -241,21 +272,24 @@ def main(argv=None) -> int:
     from . import pylqcs as lq
 
     ap = argparse.ArgumentParser(description="time plans on this GPU, train the cost model on it, evaluate")
-    ap.add_argument("--out", default=default_model_path())
+    ap.add_argument("--kernel", default="register", choices=("register", "specialized"),
+                    help="the tile kernel whose pass times the model learns")
+    ap.add_argument("--out", default=None)
     ap.add_argument("--summary", default=None, help="JSON file for the held-out evaluation")
     ap.add_argument("--widths", default="24,26,28,30,32")
     ap.add_argument("--epochs", type=int, default=400)
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args(argv)
     widths = tuple(int(w) for w in args.widths.split(","))
-    lq.costmodel_uninstall()
+    out = args.out or default_model_path(args.kernel)
+    lq.costmodel_uninstall(tile_kernel=args.kernel)
     t0 = time.perf_counter()
-    X, T, _ = generate(lq, default_specs(args.seed, widths))
+    X, T, _ = generate(lq, default_specs(args.seed, widths), kernel=args.kernel)
     print(f"training rows: {len(X)} ({time.perf_counter() - t0:.1f} s of timing)", flush=True)
     model, loss = train(lq, X, T, epochs=args.epochs, seed=args.seed)
     print(f"trained: final loss {loss:.4f} (normalised log-ms)", flush=True)
-    model.save(args.out)
-    _, _, held = generate(lq, default_specs(args.seed + 1, widths))
+    model.save(out)
+    _, _, held = generate(lq, default_specs(args.seed + 1, widths), kernel=args.kernel)
     ev = evaluate(lq, model, held)
     tot = {k: sum(e[k] for e in ev) for k in ("learned_ms", "greedy_ms", "best_ms")}
     print(json.dumps({"held_out_circuits": len(ev), **{k: round(v, 3) for k, v in tot.items()}}), flush=True)

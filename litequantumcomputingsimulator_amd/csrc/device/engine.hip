@@ -1082,6 +1082,13 @@ lqcs_status lqcs_last_plan_choice(int32_t* plan, int32_t* low_qubits, double* pr
     return LQCS_OK;
 }
 
+lqcs_status lqcs_last_plan_tile_qubits(int32_t* tile_qubits) {
+    int L = 0;
+    last_plan_choice(nullptr, nullptr, nullptr, &L);
+    if (tile_qubits) *tile_qubits = L;
+    return LQCS_OK;
+}
+
 lqcs_status lqcs_last_plan_stats(uint64_t* passes, uint64_t* launches) {
     if (passes) *passes = g_last_passes;
     if (launches) *launches = g_last_launches;

@@ -97,14 +97,15 @@ void apply_learned(amp_t* s, uint32_t nq, const std::vector<DecodedOp>& ops, uin
 // Plan for GATE_BY_GATE / GREEDY / GAIN at a fixed tile width and low count.
 Plan make_plan(int kind, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t low);
 // The plan lqcs_state_apply runs for these options (LEARNED: the cost
-// model's pick among greedy and gain plans at 2..4 low qubits, or greedy
-// without a model); L receives the tile width (0: gate by gate).
+// model's pick among greedy and gain plans at 2..4 low qubits -- 3..5 on 12-
+// and 13-qubit tiles for specialized passes -- or greedy without a model); L
+// receives the tile width (0: gate by gate).
 Plan choose_plan(const lqcs_exec_options* opts, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t batch,
                  uint32_t& L);
 // LQCS_COST_FEATURES values describing one planned step (planner.hip).
 void step_features(const PlanStep& st, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t batch,
                    double* f);
-void last_plan_choice(int* plan, int* low, double* predicted_ms);
+void last_plan_choice(int* plan, int* low, double* predicted_ms, int* tile_qubits = nullptr);
 // Specialized register-tile passes (spec_pass.hip): on for the calls that
 // select them; kernel text of a program on L = 12 or 13 tile qubits (empty:
 // a form the generator does not handle); the kernels of a list of sources,

@@ -228,10 +228,14 @@ struct CostModel {
     std::vector<double> params, x_mean, x_scale;
     double t_mean = 0.0, t_scale = 1.0;
 };
-CostModel& cost_model() {
-    static CostModel m;
-    return m;
+// Two models: one for passes on the interpreting register tile, one for
+// specialized passes (whose costs differ); the planner consults the one of
+// the kernel a call selects.
+CostModel& cost_model_slot(bool specialized) {
+    static CostModel m[2];
+    return m[specialized ? 1 : 0];
 }
+CostModel& cost_model() { return cost_model_slot(g_rtile_spec); }
 
 }  // namespace
 
@@ -385,10 +389,11 @@ lqcs_status lqcs_mlp_train(uint32_t n_features, double* params, const double* x,
     });
 }
 
-lqcs_status lqcs_costmodel_install(uint32_t n_features, const double* params, const double* x_mean,
-                                   const double* x_scale, double t_mean, double t_scale) {
+namespace {
+lqcs_status install_model(bool specialized, uint32_t n_features, const double* params, const double* x_mean,
+                          const double* x_scale, double t_mean, double t_scale) {
     return run_guarded([&] {
-        CostModel& cm = cost_model();
+        CostModel& cm = cost_model_slot(specialized);
         ++cost_model_counter();
         if (!params) {
             cm = CostModel{};
@@ -409,4 +414,20 @@ lqcs_status lqcs_costmodel_install(uint32_t n_features, const double* params, co
         cm.t_mean = t_mean;
         cm.t_scale = t_scale;
     });
+}
+}  // namespace
+
+lqcs_status lqcs_costmodel_install(uint32_t n_features, const double* params, const double* x_mean,
+                                   const double* x_scale, double t_mean, double t_scale) {
+    return install_model(false, n_features, params, x_mean, x_scale, t_mean, t_scale);
+}
+
+lqcs_status lqcs_costmodel_install_kernel(int32_t tile_kernel, uint32_t n_features, const double* params,
+                                          const double* x_mean, const double* x_scale, double t_mean,
+                                          double t_scale) {
+    if (tile_kernel != LQCS_TILE_REGISTER && tile_kernel != LQCS_TILE_SPECIALIZED)
+        return run_guarded([] {
+            throw Error(LQCS_ERR_INVALID_ARGUMENT, "cost models exist for LQCS_TILE_REGISTER and LQCS_TILE_SPECIALIZED");
+        });
+    return install_model(tile_kernel == LQCS_TILE_SPECIALIZED, n_features, params, x_mean, x_scale, t_mean, t_scale);
 }

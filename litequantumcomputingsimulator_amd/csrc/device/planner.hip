@@ -109,6 +109,7 @@ struct Choice {
     int plan = LQCS_PLAN_GREEDY;
     int low = 0;
     double predicted_ms = -1.0;
+    uint32_t L = 0;  // tile width
 };
 thread_local Choice g_last_choice;
 
@@ -141,6 +142,7 @@ uint64_t choice_key(const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t bat
     f.pod(low_req);
     f.pod(cost_model_version());
     f.pod(rtile_enabled());
+    f.pod(g_rtile_spec);
     f.pod(g_dense_mfma_min_targets);
     for (const DecodedOp& d : ops) {
         f.pod(d.kind);
@@ -182,6 +184,7 @@ void step_features(const PlanStep& st, const std::vector<DecodedOp>& ops, uint32
     f[2] = std::log2(double(std::max<uint32_t>(batch, 1)));
     if (st.tiled) {
         f[0] = 1.0;
+        f[18] = double(L) - 12.0;  // 13-qubit tiles (specialized passes only)
         f[15] = __builtin_popcountll(st.smask >> 20);
         f[16] = __builtin_ctzll(~st.smask);
         f[17] = double(st.tile_ops.size());
@@ -233,34 +236,53 @@ Plan choose_plan(const lqcs_exec_options* opts, const std::vector<DecodedOp>& op
     if (kind < LQCS_PLAN_GATE_BY_GATE || kind > LQCS_PLAN_GAIN)
         throw Error(LQCS_ERR_INVALID_ARGUMENT, "unknown plan kind");
     const TileChoice def = default_tile(nq);
-    L = opts && opts->tile_qubits > 0 ? static_cast<uint32_t>(opts->tile_qubits) : def.L;
+    const uint32_t L_req = opts && opts->tile_qubits > 0 ? static_cast<uint32_t>(opts->tile_qubits) : 0;
+    L = L_req ? L_req : def.L;
     const uint32_t low_req = opts && opts->low_qubits > 0 ? static_cast<uint32_t>(opts->low_qubits) : 0;
     const uint32_t low = low_req ? low_req : def.low;
     if (kind != LQCS_PLAN_LEARNED || L == 0 || !cost_model_installed()) {
         // Without a cost model the learned plan falls back to greedy grouping.
         const int k = kind == LQCS_PLAN_LEARNED ? LQCS_PLAN_GREEDY : kind;
-        g_last_choice = Choice{k, static_cast<int>(low), -1.0};
+        g_last_choice = Choice{k, static_cast<int>(low), -1.0, L};
         return make_plan(k, ops, nq, L, low);
     }
     ensure_ready();
-    const uint64_t key = choice_key(ops, nq, batch, L, low_req);
+    const uint64_t key = choice_key(ops, nq, batch, L_req, low_req);
     for (const CachedChoice& c : g_choice_cache) {
         if (c.key != key) continue;
         g_last_choice = c.choice;
+        L = c.choice.L;
         return make_plan(c.choice.plan, ops, nq, L, static_cast<uint32_t>(c.choice.low));
+    }
+    // Candidates: greedy and gain grouping at 2..4 low qubits; for specialized
+    // passes at 3..5 low qubits on 12- and 13-qubit tiles (unless the call
+    // fixes the width), from registers wide enough for 13-qubit tiles.
+    std::vector<uint32_t> widths{L};
+    std::vector<uint32_t> lows{2u, 3u, 4u};
+    if (g_rtile_spec) {
+        lows = {3u, 4u, 5u};
+        // 13-qubit tiles run register programs only: no Pauli rotation may
+        // act densely on more than four qubits (the LDS tile kernel, which
+        // runs those on narrower tiles, stops at 12).
+        bool narrow = true;
+        for (const DecodedOp& d : ops)
+            if (d.kind == LQCS_OP_PAULI_ROTATION && __builtin_popcountll(d.x) > 4) narrow = false;
+        if (!L_req && L == 12 && nq >= 17 && narrow) widths.push_back(13);
     }
     std::vector<Plan> cands;
     std::vector<Choice> meta;
-    for (int k : {LQCS_PLAN_GREEDY, LQCS_PLAN_GAIN}) {
-        for (uint32_t lw : {2u, 3u, 4u}) {
-            if (low_req && lw != low_req) continue;
-            if (lw + 2 > L) continue;
-            cands.push_back(make_plan(k, ops, nq, L, lw));
-            meta.push_back(Choice{k, static_cast<int>(lw), 0.0});
+    for (uint32_t w : widths) {
+        for (int k : {LQCS_PLAN_GREEDY, LQCS_PLAN_GAIN}) {
+            for (uint32_t lw : lows) {
+                if (low_req && lw != low_req) continue;
+                if (lw + 2 > w) continue;
+                cands.push_back(make_plan(k, ops, nq, w, lw));
+                meta.push_back(Choice{k, static_cast<int>(lw), 0.0, w});
+            }
         }
     }
     if (cands.empty()) {
-        g_last_choice = Choice{LQCS_PLAN_GREEDY, static_cast<int>(low), -1.0};
+        g_last_choice = Choice{LQCS_PLAN_GREEDY, static_cast<int>(low), -1.0, L};
         return make_plan(LQCS_PLAN_GREEDY, ops, nq, L, low);
     }
     const uint32_t F = LQCS_COST_FEATURES;
@@ -270,7 +292,7 @@ Plan choose_plan(const lqcs_exec_options* opts, const std::vector<DecodedOp>& op
         first[c] = rows.size() / F;
         for (const PlanStep& st : cands[c].steps) {
             rows.resize(rows.size() + F);
-            step_features(st, ops, nq, L, batch, rows.data() + rows.size() - F);
+            step_features(st, ops, nq, meta[c].L, batch, rows.data() + rows.size() - F);
         }
     }
     first[cands.size()] = rows.size() / F;
@@ -281,12 +303,13 @@ Plan choose_plan(const lqcs_exec_options* opts, const std::vector<DecodedOp>& op
         for (size_t r = first[c]; r < first[c + 1]; ++r) t += ms[r];
         meta[c].predicted_ms = t;
         if (t < meta[best].predicted_ms || c == 0) best = c;
-        if (meta[c].plan == LQCS_PLAN_GREEDY && meta[c].low == static_cast<int>(low)) base = c;
+        if (meta[c].plan == LQCS_PLAN_GREEDY && meta[c].low == static_cast<int>(low) && meta[c].L == L) base = c;
     }
     // The plan greedy grouping would run stays unless another is predicted
     // to be faster by kLearnedMargin: a near tie is within the model's error.
     if (base < cands.size() && meta[best].predicted_ms > (1.0 - kLearnedMargin) * meta[base].predicted_ms) best = base;
     g_last_choice = meta[best];
+    L = meta[best].L;
     if (g_choice_cache.size() >= 16) g_choice_cache.erase(g_choice_cache.begin());
     g_choice_cache.push_back(CachedChoice{key, meta[best]});
     return std::move(cands[best]);
@@ -300,10 +323,11 @@ void apply_learned(amp_t* s, uint32_t nq, const std::vector<DecodedOp>& ops, uin
     execute_plan(s, nq, width, plan, ops, batch);
 }
 
-void last_plan_choice(int* plan, int* low, double* predicted_ms) {
+void last_plan_choice(int* plan, int* low, double* predicted_ms, int* tile_qubits) {
     if (plan) *plan = g_last_choice.plan;
     if (low) *low = g_last_choice.low;
     if (predicted_ms) *predicted_ms = g_last_choice.predicted_ms;
+    if (tile_qubits) *tile_qubits = static_cast<int>(g_last_choice.L);
 }
 
 }  // namespace lqcs_dev

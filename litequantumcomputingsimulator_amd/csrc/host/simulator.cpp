@@ -432,16 +432,17 @@ std::uint64_t mlp_num_params(std::uint32_t n_features) {
     return n;
 }
 
-void install_cost_model(const CostModel& m) {
+void install_cost_model(const CostModel& m, int tile_kernel) {
+    const int32_t slot = tile_kernel == LQCS_TILE_SPECIALIZED ? LQCS_TILE_SPECIALIZED : LQCS_TILE_REGISTER;
     if (m.params.empty()) {
-        check(lqcs_costmodel_install(kCostFeatures, nullptr, nullptr, nullptr, 0.0, 1.0));
+        check(lqcs_costmodel_install_kernel(slot, kCostFeatures, nullptr, nullptr, nullptr, 0.0, 1.0));
         return;
     }
     if (m.params.size() != mlp_num_params(kCostFeatures) || m.x_mean.size() != kCostFeatures ||
         m.x_scale.size() != kCostFeatures)
         throw std::invalid_argument("install_cost_model: sizes do not match the feature count");
-    check(lqcs_costmodel_install(kCostFeatures, m.params.data(), m.x_mean.data(), m.x_scale.data(), m.t_mean,
-                                 m.t_scale));
+    check(lqcs_costmodel_install_kernel(slot, kCostFeatures, m.params.data(), m.x_mean.data(), m.x_scale.data(),
+                                        m.t_mean, m.t_scale));
 }
 
 CostModel load_cost_model(const std::string& path) {
@@ -468,10 +469,11 @@ CostModel load_cost_model(const std::string& path) {
 }
 
 PlanChoice last_plan_choice() {
-    int32_t plan = 0, low = 0;
+    int32_t plan = 0, low = 0, L = 0;
     double ms = 0.0;
     check(lqcs_last_plan_choice(&plan, &low, &ms));
-    return {static_cast<EngineOptions::Plan>(plan), low, ms};
+    check(lqcs_last_plan_tile_qubits(&L));
+    return {static_cast<EngineOptions::Plan>(plan), low, ms, L};
 }
 
 void profile_enable(bool on) { check(lqcs_profile_enable(on ? 1 : 0)); }

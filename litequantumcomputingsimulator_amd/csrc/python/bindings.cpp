@@ -565,11 +565,12 @@ PYBIND11_MODULE(_pylqcs, m) {
              "kernel text and op words of the specialized kernel of one planned pass (no device needed)")
         .def("features",
              [](const CompiledCircuit& c, const std::string& plan, int tile_qubits, int low_qubits,
-                std::uint32_t batch) {
+                std::uint32_t batch, const std::string& tile_kernel) {
                  EngineOptions e;
                  e.plan = plan_of(plan);
                  e.tile_qubits = tile_qubits;
                  e.low_qubits = low_qubits;
+                 e.tile_kernel = tile_kernel_of(tile_kernel);
                  std::vector<double> f;
                  {
                      py::gil_scoped_release nogil;
@@ -580,7 +581,7 @@ PYBIND11_MODULE(_pylqcs, m) {
                  std::memcpy(out.mutable_data(), f.data(), f.size() * sizeof(double));
                  return out;
              },
-             "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0, "batch"_a = 1,
+             "plan"_a = "auto", "tile_qubits"_a = 0, "low_qubits"_a = 0, "batch"_a = 1, "tile_kernel"_a = "default",
              "cost-model features of every pass of the plan (rows x COST_FEATURES)");
     py::class_<CircuitGraph>(m, "CircuitGraph")
         .def(py::init([](const CompiledCircuit& c, Statevector& sv, const std::string& plan, int tile_qubits,
@@ -704,16 +705,25 @@ PYBIND11_MODULE(_pylqcs, m) {
     m.def(
         "costmodel_install",
         [](std::vector<double> params, std::vector<double> x_mean, std::vector<double> x_scale, double t_mean,
-           double t_scale) { install_cost_model(CostModel{params, x_mean, x_scale, t_mean, t_scale}); },
-        "params"_a, "x_mean"_a, "x_scale"_a, "t_mean"_a, "t_scale"_a);
-    m.def("costmodel_uninstall", [] { install_cost_model(CostModel{}); });
-    m.def("costmodel_load", [](const std::string& path) { install_cost_model(load_cost_model(path)); }, "path"_a);
+           double t_scale, const std::string& tile_kernel) {
+            install_cost_model(CostModel{params, x_mean, x_scale, t_mean, t_scale}, tile_kernel_of(tile_kernel));
+        },
+        "params"_a, "x_mean"_a, "x_scale"_a, "t_mean"_a, "t_scale"_a, "tile_kernel"_a = "register",
+        "install the model plan='learned' consults for passes on this tile kernel (register or specialized)");
+    m.def("costmodel_uninstall",
+          [](const std::string& tile_kernel) { install_cost_model(CostModel{}, tile_kernel_of(tile_kernel)); },
+          "tile_kernel"_a = "register");
+    m.def("costmodel_load",
+          [](const std::string& path, const std::string& tile_kernel) {
+              install_cost_model(load_cost_model(path), tile_kernel_of(tile_kernel));
+          },
+          "path"_a, "tile_kernel"_a = "register");
     m.def("last_plan_choice", [] {
         const PlanChoice c = last_plan_choice();
         static const char* names[] = {"gate_by_gate", "greedy", "learned", "auto", "gain"};
         const int k = static_cast<int>(c.plan);
         return py::dict("plan"_a = std::string(k >= 0 && k <= 4 ? names[k] : "?"), "low_qubits"_a = c.low_qubits,
-                        "predicted_ms"_a = c.predicted_ms);
+                        "predicted_ms"_a = c.predicted_ms, "tile_qubits"_a = c.tile_qubits);
     });
 
     m.def("device_info", [] {

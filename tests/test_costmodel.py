@@ -162,6 +162,39 @@ def test_learned_plan_gives_the_greedy_state(lq):
     assert np.max(np.abs(a.to_numpy() - b.to_numpy())) <= 1e-12
 
 
+@pytest.mark.gpu
+def test_learned_plan_on_specialized_passes_gives_the_greedy_state(lq):
+    """plan='learned' with specialized passes consults the specialized-pass
+    model (a random one when none is packaged) among 12- and 13-qubit tiles
+    at 3..5 low qubits; the state equals greedy grouping's on the
+    interpreting kernel."""
+    from litequantumcomputingsimulator_amd import costmodel as CM
+    from litequantumcomputingsimulator_amd import workloads as W
+
+    n = 20
+    qc = lq.QuantumCircuit(n)
+    W.random_u3_cx_cp(qc, n, 12, seed=5)
+    comp = lq.CompiledCircuit(qc, True)
+    a, b = lq.statevector(n), lq.statevector(n)
+    comp.apply(a, plan="greedy", tile_qubits=12, low_qubits=4, tile_kernel="register")
+    packaged = os.path.exists(CM.default_model_path("specialized"))
+    if packaged:
+        lq.costmodel_load(CM.default_model_path("specialized"), tile_kernel="specialized")
+    else:
+        F = lq.COST_FEATURES
+        lq.costmodel_install(list(CM.init_params(lq, F, 4)), [0.0] * F, [1.0] * F, 0.0, 1.0,
+                             tile_kernel="specialized")
+    try:
+        comp.apply(b, plan="learned", tile_kernel="specialized")
+        choice = lq.last_plan_choice()
+        assert choice["plan"] in ("greedy", "gain") and choice["low_qubits"] in (3, 4, 5)
+        assert choice["tile_qubits"] in (12, 13) and choice["predicted_ms"] > 0
+    finally:
+        if not packaged:
+            lq.costmodel_uninstall(tile_kernel="specialized")
+    assert a.max_abs_diff(b) <= 1e-12
+
+
 def _gain_circuits(lq):
     from litequantumcomputingsimulator_amd import costmodel as CM
 
