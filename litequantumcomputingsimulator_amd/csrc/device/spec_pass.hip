@@ -463,6 +463,7 @@ std::string spec_source(const RTileProgram& rp, uint32_t nq, uint32_t L, uint64_
         if (t != static_cast<int>(L)) return {};
     }
     const uint32_t threads = 1u << (L - 4);
+
     // Tile number -> base index: its bits on the qubits outside the tile set.
     std::vector<int> outside_dst;
     for (uint32_t q = 0; q < nq; ++q)

@@ -394,7 +394,7 @@ def test_errors_keep_reference_types(lq):
         _ = sv[4]
 
 
-@pytest.mark.parametrize("kernel", ["register", "stream", "land", "lds"])
+@pytest.mark.parametrize("kernel", ["register", "stream", "land", "lds", "specialized"])
 @pytest.mark.parametrize("plan,low", [("greedy", 4), ("learned", 3), ("learned", 2), ("greedy", 5)])
 def test_tile_kernels_and_planners_match_oracle(lq, kernel, plan, low):
     n = 17
@@ -428,6 +428,25 @@ def test_noisy_trajectories_12q_match_oracle_counts(lq, kernel):
     want = O.run_counts(c, seed=11, shots=256, channels=chans)
     sim = lq.StatevectorSimulator(seed=11, noise=_noise_model(lq), trajectory_kernel=kernel)
     assert dict(sim.run(c.to_product(lq), 256).counts()) == want
+
+
+def test_noisy_trajectories_on_specialized_passes_match_oracle_counts(lq):
+    """Shared trajectory runs with every tiled pass on specialized kernels:
+    batches of registers (grid rows), state-independent branches as
+    single-register ops and default branches on all but one register (batch
+    restrictions in the generated code); counts equal the oracle's."""
+    from tests import cases
+
+    c, chans = cases.noisy_ghz(12, p=0.02, gamma=0.05)
+    want = O.run_counts(c, seed=13, shots=128, channels=chans)
+    noise = _noise_model(lq, 0.02, 0.05)
+    sim = lq.StatevectorSimulator(seed=13, noise=noise, trajectory_kernel="shared")
+    lq.set_tile_kernel("specialized")
+    try:
+        got = dict(sim.run(c.to_product(lq), 128).counts())
+    finally:
+        lq.set_tile_kernel("register")
+    assert got == want
 
 
 @pytest.mark.parametrize("kernel", ["shared", "persistent"])
