@@ -55,8 +55,9 @@ public:
 
     // Autotuner: times every candidate tile shape on `scratch` (a register of
     // the circuit's width, overwritten) -- greedy and gain grouping at 2, 3
-    // and 4 low qubits on the register tile, and the other tile kernels at
-    // the default shape -- keeping the fastest; apply() with Plan::Auto then
+    // and 4 low qubits on the register tile, the other tile kernels at the
+    // default shape, and dense blocks of 4, 5 and 6 qubits (up to the
+    // register's width) -- keeping the fastest; apply() with Plan::Auto then
     // uses it. Each candidate: one untimed apply, then the minimum of `reps`
     // timed ones (kernel time from HIP events).
     struct TuneTrial {
@@ -138,7 +139,7 @@ struct PlanChoice {
     EngineOptions::Plan plan;
     int low_qubits;
     double predicted_ms;  // -1 when no model decided
-    int tile_qubits;      // tile width of the plan (0: gate by gate)
+    int tile_qubits;      // tile width of the plan, or dense block width (0: gate by gate)
 };
 PlanChoice last_plan_choice();
 

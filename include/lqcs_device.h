@@ -106,7 +106,10 @@ enum {
     LQCS_PLAN_LEARNED = 2,       /* the installed cost model's pick among GREEDY and GAIN
                                     plans with 2..4 low qubits; GREEDY when none is installed */
     LQCS_PLAN_AUTO = 3,          /* engine default (GREEDY) */
-    LQCS_PLAN_GAIN = 4           /* tiled passes, tile sets grown by marginal gain */
+    LQCS_PLAN_GAIN = 4,          /* tiled passes, tile sets grown by marginal gain */
+    LQCS_PLAN_DENSE = 5          /* ops grouped into blocks of tile_qubits qubits (2..6, default
+                                    5), each multiplied on the host into one unitary and applied
+                                    in one pass (a complex GEMM on FP64 MFMA for 4..6 qubits) */
 };
 
 /* Kernel for tiled passes, per call (lqcs_exec_options.tile_kernel). */
@@ -153,9 +156,10 @@ lqcs_status lqcs_graph_destroy(lqcs_graph* g);
 lqcs_status lqcs_last_plan_stats(uint64_t* passes, uint64_t* launches);
 
 /* Plan of lqcs_state_apply without running it (no device needed): number of
- * steps (HBM passes), ops per step, each step's tile qubit mask (0 for a
- * stand-alone gate-by-gate step) and its register-tile phase count (0 when
- * the step runs on another kernel); arrays hold max_steps entries. */
+ * steps (HBM passes), ops per step, each step's qubit mask (a tile's qubits, a
+ * dense block's qubits, 0 for a stand-alone gate-by-gate step) and its
+ * register-tile phase count (0 when the step runs on another kernel); arrays
+ * hold max_steps entries. */
 lqcs_status lqcs_plan_describe(uint32_t n_qubits, const lqcs_program* prog, const lqcs_exec_options* opts,
                                uint64_t* n_steps, uint64_t* ops_per_step, uint64_t* tile_masks,
                                uint64_t* phases_per_step, uint64_t max_steps);

@@ -540,7 +540,7 @@ void launch_staged(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan, const st
         } else if (w.kind == 1) {
             launch_tile_pass(s, nq, L, st.smask, d_ops + w.first, static_cast<uint32_t>(w.count), batch);
         } else {
-            run_single_op(s, nq, ops[st.op], batch);
+            run_single_op(s, nq, step_op(plan, ops, st), batch);
         }
         if (timed) {
             LQCS_HIP(hipEventRecord(e1, stream()));
@@ -919,8 +919,8 @@ lqcs_status lqcs_graph_create(lqcs_dev_state* s, const lqcs_program* prog, const
         uint32_t L = 0;
         const Plan plan = choose_plan(opts, ops, s->nq, 1, L);
         for (const PlanStep& st : plan.steps)
-            if (!st.tiled && (ops[st.op].kind == LQCS_OP_CONTROLLED_UNITARY ||
-                              ops[st.op].kind == LQCS_OP_CONTROLLED_PERMUTATION))
+            if (!st.tiled && (step_op(plan, ops, st).kind == LQCS_OP_CONTROLLED_UNITARY ||
+                              step_op(plan, ops, st).kind == LQCS_OP_CONTROLLED_PERMUTATION))
                 throw Error(LQCS_ERR_INVALID_ARGUMENT,
                             "graph capture: a stand-alone controlled-unitary or permutation step stages tables per call");
         const StagedPlan sp = stage_plan(plan, s->nq, L);
@@ -999,8 +999,9 @@ lqcs_status lqcs_plan_describe(uint32_t n_qubits, const lqcs_program* prog, cons
         const Plan plan = choose_plan(opts, ops, n_qubits, 1, L);
         *n_steps = plan.steps.size();
         for (uint64_t i = 0; i < plan.steps.size() && i < max_steps; ++i) {
-            if (ops_per_step) ops_per_step[i] = plan.steps[i].tiled ? plan.steps[i].tile_ops.size() : 1;
-            if (tile_masks) tile_masks[i] = plan.steps[i].tiled ? plan.steps[i].smask : 0;
+            const PlanStep& st = plan.steps[i];
+            if (ops_per_step) ops_per_step[i] = st.tiled ? st.tile_ops.size() : st.fused_ops ? st.fused_ops : 1;
+            if (tile_masks) tile_masks[i] = st.tiled ? st.smask : st.fused_ops ? step_op(plan, ops, st).touched : 0;
             if (phases_per_step) {
                 RTileProgram rp;
                 phases_per_step[i] = (plan.steps[i].tiled && g_use_rtile &&
@@ -1067,7 +1068,7 @@ lqcs_status lqcs_plan_features(uint32_t n_qubits, const lqcs_program* prog, cons
         *n_steps = plan.steps.size();
         if (!features) return;
         for (uint64_t i = 0; i < plan.steps.size() && i < max_steps; ++i)
-            step_features(plan.steps[i], ops, n_qubits, L, std::max<uint32_t>(batch, 1),
+            step_features(plan, plan.steps[i], ops, n_qubits, L, std::max<uint32_t>(batch, 1),
                           features + i * LQCS_COST_FEATURES);
     });
 }

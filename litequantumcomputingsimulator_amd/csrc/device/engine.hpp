@@ -47,11 +47,17 @@ struct PlanStep {
     bool tiled = false;
     uint64_t smask = 0;            // tile qubits (tiled)
     std::vector<TileOp> tile_ops;  // tiled
-    uint32_t op = 0;               // stand-alone op index
+    uint32_t op = 0;               // stand-alone op index; from ops.size() on, Plan::fused
+    uint32_t fused_ops = 0;        // ops multiplied into that fused op (dense blocks)
 };
 struct Plan {
     std::vector<PlanStep> steps;
+    std::vector<DecodedOp> fused;  // ops the plan made (dense blocks)
 };
+// The op a stand-alone step runs.
+inline const DecodedOp& step_op(const Plan& p, const std::vector<DecodedOp>& ops, const PlanStep& st) {
+    return st.op < ops.size() ? ops[st.op] : p.fused[st.op - ops.size()];
+}
 
 struct TileChoice {
     uint32_t L = 0;    // 0: no tiling
@@ -64,6 +70,7 @@ uint64_t tile_requirement(const DecodedOp& d);
 TileOp make_tile_op(const DecodedOp& d, uint64_t smask);
 Plan plan_gain(const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t low);
 Plan plan_greedy(const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t low);
+Plan plan_dense(const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t k);
 void execute_plan(amp_t* s, uint32_t nq, uint32_t L, const Plan& plan,
                   const std::vector<DecodedOp>& ops, uint32_t batch);
 // A plan's tile programs, ready to copy to the device: LDS-tile ops, and
@@ -103,7 +110,7 @@ Plan make_plan(int kind, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_
 Plan choose_plan(const lqcs_exec_options* opts, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t batch,
                  uint32_t& L);
 // LQCS_COST_FEATURES values describing one planned step (planner.hip).
-void step_features(const PlanStep& st, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t batch,
+void step_features(const Plan& plan, const PlanStep& st, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t batch,
                    double* f);
 void last_plan_choice(int* plan, int* low, double* predicted_ms, int* tile_qubits = nullptr);
 // Specialized register-tile passes (spec_pass.hip): on for the calls that

@@ -177,7 +177,7 @@ uint32_t lowest_bits_qubits_at_least(uint64_t smask, uint32_t tmask, uint32_t la
 
 }  // namespace
 
-void step_features(const PlanStep& st, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t batch,
+void step_features(const Plan& plan, const PlanStep& st, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t batch,
                    double* f) {
     std::fill(f, f + LQCS_COST_FEATURES, 0.0);
     f[1] = nq;
@@ -214,7 +214,7 @@ void step_features(const PlanStep& st, const std::vector<DecodedOp>& ops, uint32
             f[14] = lowest_bits_qubits_at_least(st.smask, rp.phases.front().tmask, 6, 8);
         }
     } else {
-        const DecodedOp& d = ops[st.op];
+        const DecodedOp& d = step_op(plan, ops, st);
         f[18] = d.kind;
         f[19] = double(d.targets.size());
         f[20] = double(d.controls.size());
@@ -233,10 +233,15 @@ Plan choose_plan(const lqcs_exec_options* opts, const std::vector<DecodedOp>& op
                  uint32_t& L) {
     int kind = opts ? opts->plan : LQCS_PLAN_AUTO;
     if (kind == LQCS_PLAN_AUTO) kind = LQCS_PLAN_GREEDY;
-    if (kind < LQCS_PLAN_GATE_BY_GATE || kind > LQCS_PLAN_GAIN)
+    if (kind < LQCS_PLAN_GATE_BY_GATE || kind > LQCS_PLAN_DENSE)
         throw Error(LQCS_ERR_INVALID_ARGUMENT, "unknown plan kind");
     const TileChoice def = default_tile(nq);
     const uint32_t L_req = opts && opts->tile_qubits > 0 ? static_cast<uint32_t>(opts->tile_qubits) : 0;
+    if (kind == LQCS_PLAN_DENSE) {  // tile_qubits is the block width
+        L = L_req ? L_req : std::min<uint32_t>(5, nq);
+        g_last_choice = Choice{kind, 0, -1.0, L};
+        return plan_dense(ops, nq, L);
+    }
     L = L_req ? L_req : def.L;
     const uint32_t low_req = opts && opts->low_qubits > 0 ? static_cast<uint32_t>(opts->low_qubits) : 0;
     const uint32_t low = low_req ? low_req : def.low;
@@ -292,7 +297,7 @@ Plan choose_plan(const lqcs_exec_options* opts, const std::vector<DecodedOp>& op
         first[c] = rows.size() / F;
         for (const PlanStep& st : cands[c].steps) {
             rows.resize(rows.size() + F);
-            step_features(st, ops, nq, meta[c].L, batch, rows.data() + rows.size() - F);
+            step_features(cands[c], st, ops, nq, meta[c].L, batch, rows.data() + rows.size() - F);
         }
     }
     first[cands.size()] = rows.size() / F;

@@ -318,16 +318,19 @@ CompiledCircuit::TuneResult CompiledCircuit::autotune(Statevector& scratch, int 
     if (scratch.num_qubits() != nq_) throw std::invalid_argument("autotune: register width differs from the circuit");
     reps = std::max(1, reps);
     std::vector<EngineOptions> cands;
-    auto add = [&](EngineOptions::Plan plan, int low, int kernel) {
+    auto add = [&](EngineOptions::Plan plan, int low, int kernel, int width = 0) {
         EngineOptions e;
         e.plan = plan;
         e.low_qubits = low;
         e.tile_kernel = kernel;
+        e.tile_qubits = width;
         cands.push_back(e);
     };
     for (auto plan : {EngineOptions::Plan::Greedy, EngineOptions::Plan::Gain})
         for (int low : {2, 3, 4}) add(plan, low, 2);
     for (int kernel : {1, 3, 4}) add(EngineOptions::Plan::Greedy, 0, kernel);
+    for (int k : {4, 5, 6})
+        if (static_cast<std::size_t>(k) <= nq_) add(EngineOptions::Plan::Dense, 0, 0, k);
     TuneResult r;
     for (const EngineOptions& e : cands) {
         apply(scratch, e);  // untimed

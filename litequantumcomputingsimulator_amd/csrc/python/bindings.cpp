@@ -41,7 +41,8 @@ EngineOptions::Plan plan_of(const std::string& s) {
     if (s == "learned") return EngineOptions::Plan::Learned;
     if (s == "auto") return EngineOptions::Plan::Auto;
     if (s == "gain") return EngineOptions::Plan::Gain;
-    throw std::invalid_argument("plan must be gate_by_gate, greedy, gain, learned or auto");
+    if (s == "dense") return EngineOptions::Plan::Dense;
+    throw std::invalid_argument("plan must be gate_by_gate, greedy, gain, dense, learned or auto");
 }
 
 int trajectory_kernel_of(const std::string& s) {
@@ -62,10 +63,10 @@ int tile_kernel_of(const std::string& s) {
 }
 
 py::dict options_dict(const EngineOptions& e) {
-    static const char* plans[] = {"gate_by_gate", "greedy", "learned", "auto", "gain"};
+    static const char* plans[] = {"gate_by_gate", "greedy", "learned", "auto", "gain", "dense"};
     static const char* kernels[] = {"default", "lds", "register", "stream", "land", "specialized"};
     const int p = static_cast<int>(e.plan);
-    return py::dict("plan"_a = std::string(p >= 0 && p <= 4 ? plans[p] : "?"), "tile_qubits"_a = e.tile_qubits,
+    return py::dict("plan"_a = std::string(p >= 0 && p <= 5 ? plans[p] : "?"), "tile_qubits"_a = e.tile_qubits,
                     "low_qubits"_a = e.low_qubits,
                     "tile_kernel"_a = std::string(e.tile_kernel >= 0 && e.tile_kernel <= 5 ? kernels[e.tile_kernel] : "?"));
 }
@@ -720,9 +721,9 @@ PYBIND11_MODULE(_pylqcs, m) {
           "path"_a, "tile_kernel"_a = "register");
     m.def("last_plan_choice", [] {
         const PlanChoice c = last_plan_choice();
-        static const char* names[] = {"gate_by_gate", "greedy", "learned", "auto", "gain"};
+        static const char* names[] = {"gate_by_gate", "greedy", "learned", "auto", "gain", "dense"};
         const int k = static_cast<int>(c.plan);
-        return py::dict("plan"_a = std::string(k >= 0 && k <= 4 ? names[k] : "?"), "low_qubits"_a = c.low_qubits,
+        return py::dict("plan"_a = std::string(k >= 0 && k <= 5 ? names[k] : "?"), "low_qubits"_a = c.low_qubits,
                         "predicted_ms"_a = c.predicted_ms, "tile_qubits"_a = c.tile_qubits);
     });
 

@@ -29,13 +29,38 @@ def test_tile_kernel_names_are_checked(lq):
         lq.set_tile_kernel("warp")
 
 
+@pytest.mark.parametrize("k", [3, 5, 6])
+def test_dense_plan_blocks_cover_every_op_within_k_qubits(lq, k):
+    """Dense plans (planned on the host, no device): every block spans
+    exactly k qubits (padded up), ops wider than k stand alone, the steps hold
+    every op once, and a deep circuit over few qubits takes far fewer passes
+    than ops."""
+    n = 12
+    qc = _circuit(lq, n, 8, 4)
+    qc.mcx([0, 1, 2, 3, 4, 5, 6], 11)  # 8 qubits: wider than any block
+    comp = lq.CompiledCircuit(qc, False)
+    steps = comp.describe(plan="dense", tile_qubits=k)
+    assert sum(s["ops"] for s in steps) == comp.num_ops()
+    for s in steps:
+        assert s["phases"] == 0
+        assert s["tile_mask"] == 0 or bin(s["tile_mask"]).count("1") == k
+    assert any(s["tile_mask"] == 0 for s in steps)  # the 8-qubit MCX
+    deep = lq.QuantumCircuit(n)
+    W.random_u3_cx_cp(deep, 4, 40, 2)  # 4 of the 12 qubits only
+    dsteps = lq.CompiledCircuit(deep, False).describe(plan="dense", tile_qubits=k)
+    if k >= 4:
+        assert len(dsteps) == 1
+    with pytest.raises(ValueError):
+        comp.describe(plan="dense", tile_qubits=7)
+
+
 @pytest.mark.gpu
 def test_autotuner_picks_the_fastest_candidate_and_keeps_the_state(lq):
     n = 24
     comp = lq.CompiledCircuit(_circuit(lq, n, 12, 5), True)
     scratch = lq.statevector(n)
     r = comp.autotune(scratch, reps=2)
-    assert len(r["trials"]) == 9
+    assert len(r["trials"]) == 12  # 6 register-tile shapes, 3 other kernels, 3 dense widths
     assert r["best_ms"] == min(ms for _, ms in r["trials"])
     assert comp.tuned() == r["best"]
     want, got = lq.statevector(n), lq.statevector(n)

@@ -55,10 +55,12 @@ def product_state(lq, circ, fuse=True, **eng):
 
 
 PLANS = [dict(plan="gate_by_gate"), dict(plan="greedy"), dict(plan="greedy", tile_qubits=6, low_qubits=2)]
+# Dense blocks of 3 (one block per 8x8 GEMM on the generic kernel) to 6 qubits.
+DENSE_PLANS = [dict(plan="dense", tile_qubits=k) for k in (3, 4, 5, 6)]
 
 
 @pytest.mark.parametrize("seed", [11, 22, 33])
-@pytest.mark.parametrize("eng", PLANS)
+@pytest.mark.parametrize("eng", PLANS + DENSE_PLANS)
 def test_random_circuits_match_oracle(lq, seed, eng):
     n = 7
     c = random_circuit(n, 60, seed)
@@ -84,7 +86,7 @@ def test_extra_gates_match_oracle(lq, n):
     c.controlled_permutation([1, 2, 3, 0], [n - 1], [2, 3])
     c.permutation([int(v) for v in rng.permutation(8)], [1, n - 1, 3])
     want = O.run_statevector(c, fuse=False)
-    for eng in PLANS[:2]:
+    for eng in PLANS[:2] + DENSE_PLANS:
         got = product_state(lq, c, False, **eng)
         assert np.max(np.abs(got - want)) <= TOL, eng
 
@@ -110,6 +112,22 @@ def test_random_layers_greedy_vs_oracle(lq, n, layers):
                 dict(plan="gate_by_gate")):
         got = product_state(lq, c, True, **eng)
         assert np.max(np.abs(got - want)) <= TOL, eng
+
+
+@pytest.mark.parametrize("k", [4, 5, 6])
+def test_dense_blocks_on_mfma_match_oracle(lq, k):
+    """Dense blocks of 4..6 qubits at 18 qubits, where they run on the FP64
+    MFMA kernel: random U3/CX/CP layers plus Pauli rotations (an X part and
+    Z-only), a Toffoli and a swap, against the oracle; each block holds at
+    most k qubits and the blocks hold every op once."""
+    n = 18
+    c = O.Circuit(n)
+    W.random_u3_cx_cp(c, n, 6, seed=k)
+    c.rp(0.3, "XZY", [0, 9, 16]).rp(0.9, "ZIZ", [2, 11, 4]).ccx(3, 15, 7).swap(1, 14).ryy(0.4, 5, 12)
+    c.rp(-0.7, "YY", [17, 16]).rp(1.3, "Z", [8])
+    want = O.run_statevector(c, fuse=False)
+    got = product_state(lq, c, False, plan="dense", tile_qubits=k)
+    assert np.max(np.abs(got - want)) <= TOL
 
 
 WAVE_PLANS = [dict(plan="greedy", tile_qubits=10, low_qubits=3), dict(plan="gain", tile_qubits=10, low_qubits=2),
