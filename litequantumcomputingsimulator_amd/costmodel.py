@@ -3,7 +3,8 @@
 The engine's LEARNED plan asks a small network how long each pass of a
 candidate plan will take. The candidates are greedy and gain tile grouping at
 2, 3 and 4 low qubits on the interpreting register tile, or at 3, 4 and 5 low
-qubits on 12- and 13-qubit tiles for specialized passes (a model per kernel).
+qubits on 12- and 13-qubit tiles plus dense blocks of 4, 5 and 6 qubits for
+specialized passes (a model per kernel).
 It then runs the candidate with the smallest predicted total. The network is a multilayer perceptron COST_FEATURES -> 64 -> 64 -> 1
 (ReLU, FP64) on per-pass features: register phases, LDS transposes, op counts
 by kind, lane spread of the load layout, tile qubit positions, register width
@@ -101,6 +102,14 @@ def random_circuit(lq, n: int, family: str, depth: int, seed: int):
                 qc.u(*(float(v) for v in rng.uniform(0, 2 * math.pi, 3)), a)
             else:
                 qc.cx(a, b)
+    elif family == "deep_local":  # many gates on a random window of 5 qubits: dense blocks' case
+        lo = int(rng.integers(0, n - 5))
+        for _ in range(depth * 40):
+            a, b = (int(x) for x in rng.choice(np.arange(lo, lo + 5), 2, replace=False))
+            if rng.random() < 0.6:
+                qc.u(*(float(v) for v in rng.uniform(0, 2 * math.pi, 3)), a)
+            else:
+                qc.cx(a, b)
     elif family == "qft":
         x = int(rng.integers(0, 1 << n))
         W.basis_prep(qc, n, x)
@@ -115,6 +124,9 @@ CANDIDATES = tuple((plan, low) for plan in ("greedy", "gain") for low in (2, 3, 
 # Specialized passes: (plan, tile width, low qubits), as the engine plans them
 # for registers of 17 qubits and more.
 SPEC_CANDIDATES = tuple((plan, L, low) for L in (12, 13) for plan in ("greedy", "gain") for low in (3, 4, 5))
+# Dense blocks of k = 4..6 qubits (plan, k, 0): candidates of the specialized
+# passes' learned plan from 12 qubits up.
+DENSE_CANDIDATES = tuple(("dense", k, 0) for k in (4, 5, 6))
 # The engine keeps greedy grouping at the default low qubits (4 at these
 # widths) unless another candidate is predicted faster by this fraction.
 MARGIN = 0.02
@@ -124,7 +136,7 @@ SPEC_GREEDY_DEFAULT = ("greedy", 12, 4)
 
 def pick(pred: dict) -> tuple:
     """The engine's LEARNED choice among predicted totals {candidate: ms}
-    (candidates as in CANDIDATES or SPEC_CANDIDATES)."""
+    (candidates as in CANDIDATES, or SPEC_CANDIDATES + DENSE_CANDIDATES)."""
     best = min(pred, key=pred.get)
     base = SPEC_GREEDY_DEFAULT if SPEC_GREEDY_DEFAULT in pred else GREEDY_DEFAULT
     if base in pred and pred[best] > (1.0 - MARGIN) * pred[base]:
@@ -154,13 +166,16 @@ def time_plan(lq, comp, sv, plan: str, low: int, reps: int = 1, L: int = 12, ker
     return feats, ms, float(ms.sum())
 
 
-def generate(lq, specs, reps: int = 1, log=print, kernel: str = "register"):
-    """Times every candidate plan of every (n, family, depth, seed) in specs.
-    Returns (X, t, records): per-pass feature rows and t = log(ms), and per
-    circuit a record {"spec", "cands": [(candidate, features, total_ms)]},
-    the candidate as in CANDIDATES (register tile) or SPEC_CANDIDATES."""
+def generate(lq, specs, reps: int = 1, log=print, kernel: str = "register", only=None):
+    """Times every candidate plan of every (n, family, depth, seed) in specs
+    (or the candidates in `only`). Returns (X, t, records): per-pass feature
+    rows and t = log(ms), and per circuit a record {"spec", "cands":
+    [(candidate, features, total_ms)]}, the candidate as in CANDIDATES
+    (register tile) or SPEC_CANDIDATES + DENSE_CANDIDATES."""
     X, T, records = [], [], []
-    cands_of = SPEC_CANDIDATES if kernel == "specialized" else CANDIDATES
+    cands_of = SPEC_CANDIDATES + DENSE_CANDIDATES if kernel == "specialized" else CANDIDATES
+    if only is not None:
+        cands_of = tuple(c for c in cands_of if c in only)
     base = SPEC_GREEDY_DEFAULT if kernel == "specialized" else GREEDY_DEFAULT
     for n, family, depth, seed in specs:
         qc = random_circuit(lq, n, family, depth, seed)
@@ -168,6 +183,8 @@ def generate(lq, specs, reps: int = 1, log=print, kernel: str = "register"):
         sv = lq.statevector(n)
         cands = []
         for cand in cands_of:
+            if cand[0] == "dense" and n < 12:
+                continue
             if kernel == "specialized":
                 plan, L, low = cand
                 try:
@@ -184,8 +201,9 @@ def generate(lq, specs, reps: int = 1, log=print, kernel: str = "register"):
             cands.append((cand, f, tot))
         records.append({"spec": (n, family, depth, seed), "cands": cands})
         best = min(cands, key=lambda c: c[2])
-        log(f"timed n={n} {family} depth={depth} seed={seed}: best {best[0]} {best[2]:.3f} ms, "
-            f"greedy {[c[2] for c in cands if c[0] == base][0]:.3f} ms")
+        greedy = [c[2] for c in cands if c[0] == base]
+        log(f"timed n={n} {family} depth={depth} seed={seed}: best {best[0]} {best[2]:.3f} ms"
+            + (f", greedy {greedy[0]:.3f} ms" if greedy else ""))
         del sv
     return np.concatenate(X), np.concatenate(T), records
 
@@ -264,6 +282,13 @@ def default_specs(seed: int = 0, widths=(24, 26, 28, 30)):
     return specs
 
 
+def deep_specs(seed: int = 0, widths=(24, 26, 28, 30)):
+    """Deep circuits on a window of 5 qubits (one per width), where dense
+    blocks win: part of the specialized passes' training set."""
+    rng = np.random.default_rng(seed + 1000)
+    return [(n, "deep_local", int(rng.integers(3, 7)), int(rng.integers(1 << 30))) for n in widths]
+
+
 def main(argv=None) -> int:
     import argparse
     import json
@@ -284,12 +309,18 @@ def main(argv=None) -> int:
     out = args.out or default_model_path(args.kernel)
     lq.costmodel_uninstall(tile_kernel=args.kernel)
     t0 = time.perf_counter()
-    X, T, _ = generate(lq, default_specs(args.seed, widths), kernel=args.kernel)
+    specs = default_specs(args.seed, widths)
+    if args.kernel == "specialized":
+        specs += deep_specs(args.seed, widths)
+    X, T, _ = generate(lq, specs, kernel=args.kernel)
     print(f"training rows: {len(X)} ({time.perf_counter() - t0:.1f} s of timing)", flush=True)
     model, loss = train(lq, X, T, epochs=args.epochs, seed=args.seed)
     print(f"trained: final loss {loss:.4f} (normalised log-ms)", flush=True)
     model.save(out)
-    _, _, held = generate(lq, default_specs(args.seed + 1, widths), kernel=args.kernel)
+    held_specs = default_specs(args.seed + 1, widths)
+    if args.kernel == "specialized":
+        held_specs += deep_specs(args.seed + 1, widths)
+    _, _, held = generate(lq, held_specs, kernel=args.kernel)
     ev = evaluate(lq, model, held)
     tot = {k: sum(e[k] for e in ev) for k in ("learned_ms", "greedy_ms", "best_ms")}
     print(json.dumps({"held_out_circuits": len(ev), **{k: round(v, 3) for k, v in tot.items()}}), flush=True)

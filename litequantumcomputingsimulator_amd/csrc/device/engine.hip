@@ -627,6 +627,7 @@ private:
 };
 
 Plan make_plan(int kind, const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t L, uint32_t low) {
+    if (kind == LQCS_PLAN_DENSE) return plan_dense(ops, nq, L);  // L: the block width
     Plan plan;
     if (kind == LQCS_PLAN_GATE_BY_GATE || L == 0) {
         for (uint32_t i = 0; i < ops.size(); ++i) {

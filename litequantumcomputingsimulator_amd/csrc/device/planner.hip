@@ -261,7 +261,8 @@ Plan choose_plan(const lqcs_exec_options* opts, const std::vector<DecodedOp>& op
     }
     // Candidates: greedy and gain grouping at 2..4 low qubits; for specialized
     // passes at 3..5 low qubits on 12- and 13-qubit tiles (unless the call
-    // fixes the width), from registers wide enough for 13-qubit tiles.
+    // fixes the width), from registers wide enough for 13-qubit tiles, and
+    // dense blocks (below).
     std::vector<uint32_t> widths{L};
     std::vector<uint32_t> lows{2u, 3u, 4u};
     if (g_rtile_spec) {
@@ -286,6 +287,15 @@ Plan choose_plan(const lqcs_exec_options* opts, const std::vector<DecodedOp>& op
             }
         }
     }
+    // Specialized passes' model is also trained on dense 4..6-qubit blocks
+    // (FP64 MFMA; 12 qubits and up, so that the blocks take the MFMA kernel),
+    // which win on deep circuits over few qubits: candidates too, unless the
+    // call fixes the tile shape.
+    if (g_rtile_spec && !L_req && !low_req && nq >= 12)
+        for (uint32_t k : {4u, 5u, 6u}) {
+            cands.push_back(plan_dense(ops, nq, k));
+            meta.push_back(Choice{LQCS_PLAN_DENSE, 0, 0.0, k});
+        }
     if (cands.empty()) {
         g_last_choice = Choice{LQCS_PLAN_GREEDY, static_cast<int>(low), -1.0, L};
         return make_plan(LQCS_PLAN_GREEDY, ops, nq, L, low);

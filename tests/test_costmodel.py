@@ -166,8 +166,8 @@ def test_learned_plan_gives_the_greedy_state(lq):
 def test_learned_plan_on_specialized_passes_gives_the_greedy_state(lq):
     """plan='learned' with specialized passes consults the specialized-pass
     model (a random one when none is packaged) among 12- and 13-qubit tiles
-    at 3..5 low qubits; the state equals greedy grouping's on the
-    interpreting kernel."""
+    at 3..5 low qubits and dense blocks of 4..6 qubits; the state equals
+    greedy grouping's on the interpreting kernel."""
     from litequantumcomputingsimulator_amd import costmodel as CM
     from litequantumcomputingsimulator_amd import workloads as W
 
@@ -187,8 +187,12 @@ def test_learned_plan_on_specialized_passes_gives_the_greedy_state(lq):
     try:
         comp.apply(b, plan="learned", tile_kernel="specialized")
         choice = lq.last_plan_choice()
-        assert choice["plan"] in ("greedy", "gain") and choice["low_qubits"] in (3, 4, 5)
-        assert choice["tile_qubits"] in (12, 13) and choice["predicted_ms"] > 0
+        assert choice["predicted_ms"] > 0
+        if choice["plan"] == "dense":
+            assert choice["tile_qubits"] in (4, 5, 6)
+        else:
+            assert choice["plan"] in ("greedy", "gain") and choice["low_qubits"] in (3, 4, 5)
+            assert choice["tile_qubits"] in (12, 13)
     finally:
         if not packaged:
             lq.costmodel_uninstall(tile_kernel="specialized")
@@ -294,4 +298,37 @@ def test_learned_plan_matches_oracle_on_planner_workload(lq):
     comp.apply(sv, plan="learned")
     choice = lq.last_plan_choice()
     assert choice["plan"] in ("greedy", "gain")
+    assert np.max(np.abs(np.asarray(sv.to_numpy()) - want)) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_learned_plan_picks_dense_blocks_on_a_deep_local_circuit(lq):
+    """Specialized passes' learned plan on a deep circuit over 5 of 24 qubits
+    (160 U3/CX): the packaged model picks dense blocks (one 5-qubit block
+    holds the whole circuit, where a tiled pass pays for every gate), and the
+    state equals the oracle's (the reference's fused path) at 1e-12."""
+    import math
+
+    from litequantumcomputingsimulator_amd import costmodel as CM
+    from oracle import oracle as O
+
+    if not os.path.exists(CM.default_model_path("specialized")):
+        pytest.skip("no trained specialized-pass model packaged")
+    n = 24
+    rng = np.random.default_rng(77)
+    c = O.Circuit(n)
+    lo = int(rng.integers(0, n - 5))
+    for _ in range(160):
+        a, b = (int(x) for x in rng.choice(np.arange(lo, lo + 5), 2, replace=False))
+        if rng.random() < 0.6:
+            c.u(*(float(v) for v in rng.uniform(0, 2 * math.pi, 3)), a)
+        else:
+            c.cx(a, b)
+    want = O.run_statevector(c, fuse=True)
+    lq.costmodel_load(CM.default_model_path("specialized"), tile_kernel="specialized")
+    comp = lq.CompiledCircuit(c.to_product(lq), True)
+    sv = lq.statevector(n)
+    comp.apply(sv, plan="learned", tile_kernel="specialized")
+    choice = lq.last_plan_choice()
+    assert choice["plan"] == "dense" and choice["tile_qubits"] in (5, 6), choice
     assert np.max(np.abs(np.asarray(sv.to_numpy()) - want)) <= 1e-12
