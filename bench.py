@@ -18,8 +18,9 @@ rotation ansatze (16-24 qubits, 400 rotations, 200-term Hamiltonians) against
 parameter-shift gradients on the same GPU (timed on a bounded sample). Two
 more objects report the noisy-trajectory workload (24 qubits, 2048 shots,
 trajectories/s) and the learned planner (ms per step of the cost model's plan
-next to greedy's, on the headline workload and on random U3/CX/CP layers of
-depth 40 at 32 qubits, with the two final states compared). Further objects
+next to greedy's, on the headline workload, on a deep circuit over 5 of the
+30 qubits where dense blocks win, and on random U3/CX/CP layers of depth 40
+at 32 qubits, with the final states compared). Further objects
 time fused dense blocks of 5 and 6 qubits (complex GEMMs on FP64 MFMA) at
 30 qubits against the FP64 peak, and the noisy workload's density matrix at
 12, 14 and 16 qubits, each with the reference beside it.
@@ -491,6 +492,42 @@ def density_matrix_bench(lq, W, seed: int, widths=(12, 14, 16), cpu: bool = True
     return out
 
 
+def deep_local_bench(lq, sv, n: int, seed: int) -> dict:
+    """A deep circuit over a window of 5 of the n qubits (the cost model's
+    deep_local family): greedy tiled passes, dense 5-qubit blocks and the
+    learned plan, all on specialized passes, each the fastest of 3 applies;
+    the learned plan's state against greedy's."""
+    from litequantumcomputingsimulator_amd import costmodel as CM
+
+    qc = CM.random_circuit(lq, n, "deep_local", 6, seed)
+    comp = lq.CompiledCircuit(qc, True)
+    out = {"workload": f"{qc.num_instructions()} U3/CX on a window of 5 of {n} qubits, greedy 1q fusion"}
+    for name, kw in (("greedy_12q_low4", dict(plan="greedy", tile_qubits=12, low_qubits=4)),
+                     ("dense_5q", dict(plan="dense", tile_qubits=5)), ("learned", dict(plan="learned"))):
+        kw["tile_kernel"] = "specialized"
+        comp.apply(sv, **kw)  # compiles its passes
+        lq.synchronize()
+        best = float("inf")
+        for _ in range(3):
+            sv.set_basis(0)
+            lq.synchronize()
+            t0 = time.perf_counter()
+            comp.apply(sv, **kw)
+            lq.synchronize()
+            best = min(best, (time.perf_counter() - t0) * 1e3)
+        out[name] = {"ms": round(best, 3), "passes": lq.last_plan_stats()["passes"]}
+        if name == "learned":
+            out[name]["pick"] = lq.last_plan_choice()
+    ref = lq.statevector(n)
+    comp.apply(ref, plan="greedy", tile_qubits=12, low_qubits=4, tile_kernel="specialized")
+    out["max_abs_diff_learned_vs_greedy"] = ref.max_abs_diff(sv)
+    del ref
+    out["learned_vs_greedy"] = round(out["learned"]["ms"] / out["greedy_12q_low4"]["ms"], 4)
+    log(f"deep local: greedy {out['greedy_12q_low4']['ms']:.2f} ms, dense {out['dense_5q']['ms']:.2f} ms, "
+        f"learned {out['learned']['ms']:.2f} ms ({out['learned']['pick']})")
+    return out
+
+
 def planner_32q(lq, W, seed: int, n: int = 32, depth: int = 40) -> dict:
     """The learned-planner workload: random U3/CX/CP layers of depth 40 at 32
     qubits (64 GiB per register); applies with greedy grouping and with the
@@ -819,6 +856,7 @@ def main() -> int:
             log(f"specialized {name}: {best:.1f} ms/step")
         spec["learned_vs_greedy"] = round(spec["learned"]["ms_per_step"] / spec["greedy_12q_low4"]["ms_per_step"], 4)
         planner["specialized_plans"] = spec
+        planner["deep_local"] = deep_local_bench(lq, sv, n, args.seed)
 
     adjoint = None
     trajectories = None
