@@ -144,13 +144,15 @@ NOISE_SPECS = [
 
 @pytest.mark.parametrize("n,seed", [(3, 1), (5, 2), (7, 3)])
 @pytest.mark.parametrize("spec", range(len(NOISE_SPECS)))
-@pytest.mark.parametrize("plan", ["gate_by_gate", "greedy"])
+@pytest.mark.parametrize("plan", ["gate_by_gate", "greedy", "dense"])
 def test_density_matrix_matches_oracle(lq, n, seed, spec, plan):
+    """Greedy on 6-qubit tiles; dense: ket, bra and superoperator ops of rho's
+    2n-qubit register multiplied into 4-qubit blocks."""
     c = _random_noisy(n, seed)
     spec = NOISE_SPECS[spec]
     want = O.density_matrix(c, _orc_channels(spec))
     sim = lq.DensityMatrixSimulator(noise=_channels(lq, spec), plan=plan,
-                                    tile_qubits=6 if plan == "greedy" else 0)
+                                    tile_qubits={"greedy": 6, "dense": 4}.get(plan, 0))
     got = sim.run_density_matrix(c.to_product(lq)).to_numpy()
     assert np.max(np.abs(got - want)) <= TOL
 
