@@ -11,7 +11,7 @@ struct EngineOptions {
     // Greedy: first-fit tile sets in program order. Gain: tile sets grown by
     // marginal gain. Learned: the installed cost model's pick among Greedy
     // and Gain plans at 2..4 low qubits (Greedy when no model is installed).
-    // Dense: ops multiplied into unitaries over tile_qubits (2..6) qubits, one pass each.
+    // Dense: ops multiplied into unitaries over tile_qubits (1..6) qubits, one pass each.
     enum class Plan : int { GateByGate = 0, Greedy = 1, Learned = 2, Auto = 3, Gain = 4, Dense = 5 };
     Plan plan = Plan::Auto;
     int tile_qubits = 0;            // LDS tile width L (0 = engine default)

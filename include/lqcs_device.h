@@ -109,9 +109,10 @@ enum {
                                     qubits); GREEDY when none is installed */
     LQCS_PLAN_AUTO = 3,          /* engine default (GREEDY) */
     LQCS_PLAN_GAIN = 4,          /* tiled passes, tile sets grown by marginal gain */
-    LQCS_PLAN_DENSE = 5          /* ops grouped into blocks of tile_qubits qubits (2..6, default
-                                    5), each multiplied on the host into one unitary and applied
-                                    in one pass (a complex GEMM on FP64 MFMA for 4..6 qubits) */
+    LQCS_PLAN_DENSE = 5          /* ops grouped into blocks of tile_qubits qubits (1..6; default
+                                    5, or the register's width if smaller), each multiplied on the
+                                    host into one unitary and applied in one pass (a complex GEMM
+                                    on FP64 MFMA for 4..6 qubits) */
 };
 
 /* Kernel for tiled passes, per call (lqcs_exec_options.tile_kernel). */

@@ -132,7 +132,7 @@ std::vector<amp_t> block_matrix(const std::vector<DecodedOp>& ops, const std::ve
 }  // namespace
 
 Plan plan_dense(const std::vector<DecodedOp>& ops, uint32_t nq, uint32_t k) {
-    if (k < 2 || k > 6 || k > nq) throw Error(LQCS_ERR_INVALID_ARGUMENT, "dense blocks must have 2..6 qubits and <= n");
+    if (k < 1 || k > 6 || k > nq) throw Error(LQCS_ERR_INVALID_ARGUMENT, "dense blocks must have 1..6 qubits and <= n");
     Plan plan;
     std::vector<uint32_t> pending(ops.size());
     for (uint32_t i = 0; i < ops.size(); ++i) pending[i] = i;

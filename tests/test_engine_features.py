@@ -29,7 +29,7 @@ def test_tile_kernel_names_are_checked(lq):
         lq.set_tile_kernel("warp")
 
 
-@pytest.mark.parametrize("k", [3, 5, 6])
+@pytest.mark.parametrize("k", [1, 3, 5, 6])
 def test_dense_plan_blocks_cover_every_op_within_k_qubits(lq, k):
     """Dense plans (planned on the host, no device): every block spans
     exactly k qubits (padded up), ops wider than k stand alone, the steps hold
@@ -52,6 +52,9 @@ def test_dense_plan_blocks_cover_every_op_within_k_qubits(lq, k):
         assert len(dsteps) == 1
     with pytest.raises(ValueError):
         comp.describe(plan="dense", tile_qubits=7)
+    tiny = lq.QuantumCircuit(1)
+    tiny.h(0).t(0).h(0)
+    assert [s["ops"] for s in lq.CompiledCircuit(tiny, False).describe(plan="dense")] == [3]
 
 
 @pytest.mark.gpu

@@ -55,8 +55,9 @@ def product_state(lq, circ, fuse=True, **eng):
 
 
 PLANS = [dict(plan="gate_by_gate"), dict(plan="greedy"), dict(plan="greedy", tile_qubits=6, low_qubits=2)]
-# Dense blocks of 3 (one block per 8x8 GEMM on the generic kernel) to 6 qubits.
-DENSE_PLANS = [dict(plan="dense", tile_qubits=k) for k in (3, 4, 5, 6)]
+# Dense blocks of 1 to 6 qubits (up to 3 on the generic kernel, 4..6 on FP64
+# MFMA where the register has enough groups).
+DENSE_PLANS = [dict(plan="dense", tile_qubits=k) for k in (1, 2, 3, 4, 5, 6)]
 
 
 @pytest.mark.parametrize("seed", [11, 22, 33])
