@@ -56,8 +56,10 @@ public:
     // Autotuner: times every candidate tile shape on `scratch` (a register of
     // the circuit's width, overwritten) -- greedy and gain grouping at 2, 3
     // and 4 low qubits on the register tile, the other tile kernels at the
-    // default shape, and dense blocks of 4, 5 and 6 qubits (up to the
-    // register's width) -- keeping the fastest; apply() with Plan::Auto then
+    // default shape, dense blocks of 4, 5 and 6 qubits (up to the register's
+    // width), and from 12 qubits specialized passes (greedy at 4 low qubits,
+    // gain at 3 on 12- and, from 17 qubits, 13-qubit tiles; a 13-qubit
+    // candidate the circuit's ops refuse is skipped) -- keeping the fastest; apply() with Plan::Auto then
     // uses it. Each candidate: one untimed apply, then the minimum of `reps`
     // timed ones (kernel time from HIP events).
     struct TuneTrial {

@@ -331,9 +331,20 @@ CompiledCircuit::TuneResult CompiledCircuit::autotune(Statevector& scratch, int 
     for (int kernel : {1, 3, 4}) add(EngineOptions::Plan::Greedy, 0, kernel);
     for (int k : {4, 5, 6})
         if (static_cast<std::size_t>(k) <= nq_) add(EngineOptions::Plan::Dense, 0, 0, k);
+    // Specialized passes (their kernels compile during the untimed apply).
+    if (nq_ >= 12) {
+        add(EngineOptions::Plan::Greedy, 4, 5, 12);
+        add(EngineOptions::Plan::Gain, 3, 5, 12);
+        if (nq_ >= 17) add(EngineOptions::Plan::Gain, 3, 5, 13);
+    }
     TuneResult r;
     for (const EngineOptions& e : cands) {
-        apply(scratch, e);  // untimed
+        try {
+            apply(scratch, e);  // untimed
+        } catch (const std::invalid_argument&) {
+            if (e.tile_qubits == 13) continue;  // 13-qubit tiles refuse Pauli rotations wider than 4 qubits
+            throw;
+        }
         double best = -1.0;
         for (int k = 0; k < reps; ++k) {
             synchronize();
