@@ -21,8 +21,9 @@ trajectories/s) and the learned planner (ms per step of the cost model's plan
 next to greedy's, on the headline workload, on a deep circuit over 5 of the
 30 qubits where dense blocks win, and on random U3/CX/CP layers of depth 40
 at 32 qubits, with the final states compared). Further objects
-time fused dense blocks of 5 and 6 qubits (complex GEMMs on FP64 MFMA) at
-30 qubits against the FP64 peak, and the noisy workload's density matrix at
+time the QFT workload (20 qubits, 1024 shots, one run per unit), fused dense
+blocks of 5 and 6 qubits (complex GEMMs on FP64 MFMA) at 30 qubits against
+the FP64 peak, and the noisy workload's density matrix at
 12, 14 and 16 qubits, each with the reference beside it.
 
 `traffic` (HBM bytes per tiled pass) comes from the rocprofv3 counter summary
@@ -492,6 +493,56 @@ def density_matrix_bench(lq, W, seed: int, widths=(12, 14, 16), cpu: bool = True
     return out
 
 
+def qft_bench(lq, W, seed: int, n: int = 20, shots: int = 1024, reps: int = 20, cpu: bool = True) -> dict:
+    """The QFT workload: X on the bits of a seeded x, then algorithms::qft(n)
+    (220 gates at 20 qubits), measure_all, fusion off, `shots` shots. One run
+    (op upload, evolve, sampling, counts) is the unit; the fastest of `reps`.
+    The counts are compared with the reference's run on the same seed (or the
+    oracle's) and the reference's run is timed beside it."""
+    import numpy as np
+
+    x = int(np.random.default_rng(seed).integers(0, 1 << n))
+    qc = lq.QuantumCircuit(n, n)
+    W.basis_prep(qc, n, x)
+    qc.compose(lq.algorithms.qft(n))
+    gates = qc.num_instructions()
+    qc.measure_all()
+    sim = lq.StatevectorSimulator(seed=seed, fuse_gates=False)
+    counts = dict(sim.run(qc, shots).counts())  # warm-up
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        sim.run(qc, shots)
+        best = min(best, time.perf_counter() - t0)
+    out = {"workload": f"QFT from a basis state, {n} qubits, {gates} gates, measure_all, {shots} shots, fusion off",
+           "run_ms": round(best * 1e3, 3), "runs_per_s": round(1.0 / best, 1),
+           "gates_per_s": round(gates / best, 1)}
+    try:
+        from oracle import oracle as O
+    except Exception as e:  # noqa: BLE001
+        log(f"qft reference unavailable: {e}")
+        return out
+    ref = O.Circuit(n)
+    W.basis_prep(ref, n, x)
+    W.qft_into(ref, n)
+    ref.measure_all()
+    if O.ref_available():
+        cores = host_cpu()["threads"]
+        O.ref_set_threads(cores)
+        t0 = time.perf_counter()
+        want = O.ref_run_counts(ref, seed, shots, fuse=False)
+        t_ref = time.perf_counter() - t0
+        out["counts_equal_reference"] = counts == want
+        if cpu:
+            out["cpu_baseline"] = {"value": round(1.0 / t_ref, 3), "unit": "runs/s", **cpu_fields(cores),
+                                   "kind": "reference", "sample": f"one run() of the same circuit and seed, "
+                                                                  f"OMP threads={cores}"}
+    else:
+        out["counts_equal_oracle"] = counts == O.run_counts(ref, seed, shots, fuse=False)
+    log(f"qft{n}: {best * 1e3:.2f} ms per run of {shots} shots")
+    return out
+
+
 def deep_local_bench(lq, sv, n: int, seed: int) -> dict:
     """A deep circuit over a window of 5 of the n qubits (the cost model's
     deep_local family): greedy tiled passes, dense 5-qubit blocks and the
@@ -711,6 +762,7 @@ def main() -> int:
     ap.add_argument("--no-trajectories", action="store_true")
     ap.add_argument("--no-density-matrix", action="store_true",
                     help="skip the density-matrix workload (12, 14 and 16 qubits)")
+    ap.add_argument("--no-qft", action="store_true", help="skip the QFT workload (20 qubits, 1024 shots)")
     ap.add_argument("--no-dense", action="store_true",
                     help="skip the dense-block workload (k = 5, 6 targets on FP64 MFMA)")
     ap.add_argument("--no-planner", action="store_true",
@@ -873,6 +925,7 @@ def main() -> int:
     dm = None
     if not args.no_density_matrix:
         dm = density_matrix_bench(lq, W, args.seed, cpu=not args.no_cpu_baseline, tile_kernel=args.tile_kernel)
+    qft = None if args.no_qft else qft_bench(lq, W, args.seed, cpu=not args.no_cpu_baseline)
     dense = None
     if not args.no_dense:
         dense = dense_blocks_bench(lq, args.seed, cpu=not args.no_cpu_baseline)
@@ -904,6 +957,7 @@ def main() -> int:
         "adjoint": adjoint,
         "trajectories": trajectories,
         "planner": planner,
+        "qft": qft,
         "dense_blocks": dense,
         "density_matrix": dm,
     }
