@@ -2,6 +2,7 @@
 
 Products (all under litequantumcomputingsimulator_amd/lib/):
   liblqcs_device.so  HIP kernels for gfx950 + the thin C-ABI (include/lqcs_device.h)
+  lqcs_rtc_worker    compile worker the device library starts for specialized passes
   liblqcs.so         C++20 host library: the lqcs:: drop-in API over the C-ABI
   _pylqcs*.so        pybind11 module (the pylqcs Python face), in the package dir
 
@@ -85,6 +86,18 @@ def build_device() -> str:
     return out
 
 
+def build_worker() -> str:
+    """lqcs_rtc_worker: the specialized passes' compile worker (host code and
+    hipRTC only), beside liblqcs_device.so, which starts it."""
+    embed_spec_header()
+    out = os.path.join(LIB, "lqcs_rtc_worker")
+    _run([CXX, "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", f"-I{GEN}",
+          os.path.join(CSRC, "device", "rtc_worker.cpp"), "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib",
+          "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_host() -> str:
     srcs = sorted(glob.glob(os.path.join(CSRC, "host", "*.cpp")))
     objs, jobs = [], []
@@ -122,6 +135,7 @@ def build_oracle() -> None:
 
 def build(oracle: bool = True) -> None:
     build_device()
+    build_worker()
     build_host()
     build_python()
     if oracle:

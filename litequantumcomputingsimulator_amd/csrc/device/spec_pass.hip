@@ -29,15 +29,25 @@
 // The generated kernel reads its matrix entries from the pass's op words (the
 // same words the interpreting kernel reads), so passes that differ only in
 // angles share one kernel. Kernels are cached by source text for the life of
-// the process; distinct sources of one plan compile in parallel host threads.
+// the process; distinct sources of one plan compile in parallel, in helper
+// processes (lqcs_rtc_worker) where it is installed beside the library.
 #include "common.hpp"
 #include "engine.hpp"
 #include "spec_pass.hpp"
 #include "tile.hpp"
 
-#include <hip/hiprtc.h>
+#include "spec_compile.hpp"
+
+#include <dlfcn.h>
+#include <fcntl.h>
+#include <pthread.h>
+#include <spawn.h>
+#include <sys/wait.h>
+#include <unistd.h>
 
 #include <algorithm>
+#include <cerrno>
+#include <csignal>
 #include <array>
 #include <atomic>
 #include <cstdlib>
@@ -48,6 +58,8 @@
 #include <thread>
 #include <unordered_map>
 #include <vector>
+
+extern char** environ;
 
 namespace lqcs_dev {
 
@@ -547,32 +559,94 @@ SpecCache& cache() {
     return c;
 }
 
-// hipRTC compile of one pass; the code object, or an exception with the log.
+// hipRTC compile of one pass in this process; the code object, or an
+// exception with the log.
 std::vector<char> compile(const std::string& body) {
-    const std::string src = std::string(kSpecHeader) + "\n" + body;
-    hiprtcProgram prog = nullptr;
-    if (hiprtcCreateProgram(&prog, src.c_str(), "lqcs_pass.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
-        throw Error(LQCS_ERR_DEVICE, "hipRTC: cannot create a program");
-    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17",
-                          // the same code-generation switch as every HIP source of the engine
-                          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
-    const hiprtcResult r = hiprtcCompileProgram(prog, static_cast<int>(sizeof opts / sizeof opts[0]), opts);
     std::vector<char> code;
-    if (r == HIPRTC_SUCCESS) {
-        size_t n = 0;
-        hiprtcGetCodeSize(prog, &n);
-        code.resize(n);
-        hiprtcGetCode(prog, code.data());
-    } else {
-        size_t n = 0;
-        hiprtcGetProgramLogSize(prog, &n);
-        std::string log(n, '\0');
-        if (n) hiprtcGetProgramLog(prog, &log[0]);
-        hiprtcDestroyProgram(&prog);
+    std::string log;
+    if (!lqcs_rtc::compile_pass(kSpecHeader, body, code, log))
         throw Error(LQCS_ERR_DEVICE, "hipRTC compile of a specialized pass failed: " + log.substr(0, 2000));
-    }
-    hiprtcDestroyProgram(&prog);
     return code;
+}
+
+// hipRTC serialises the compiles of one process, so the passes of a plan
+// compile in parallel in helper processes: lqcs_rtc_worker, next to this
+// library, reads one kernel text on stdin and writes its code object on stdout.
+// Empty when the worker is missing.
+const std::string& worker_path() {
+    static const std::string path = [] {
+        Dl_info info{};
+        if (!dladdr(reinterpret_cast<void*>(&spec_cache_size), &info) || !info.dli_fname) return std::string();
+        const std::string lib = info.dli_fname;
+        const size_t slash = lib.rfind('/');
+        const std::string w = (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/lqcs_rtc_worker";
+        return access(w.c_str(), X_OK) == 0 ? w : std::string();
+    }();
+    return path;
+}
+
+// One pass compiled by a worker process; false (code empty) when the worker
+// cannot be started or fails, so that the caller compiles in process.
+bool compile_in_worker(const std::string& body, std::vector<char>& code) {
+    const std::string& path = worker_path();
+    if (path.empty()) return false;
+    int in[2], out[2];
+    if (pipe2(in, O_CLOEXEC) != 0) return false;
+    if (pipe2(out, O_CLOEXEC) != 0) {
+        close(in[0]);
+        close(in[1]);
+        return false;
+    }
+    posix_spawn_file_actions_t fa;
+    posix_spawn_file_actions_init(&fa);
+    posix_spawn_file_actions_adddup2(&fa, in[0], 0);
+    posix_spawn_file_actions_adddup2(&fa, out[1], 1);
+    // The child keeps stdin, stdout and stderr only (no device or other
+    // descriptors of this process).
+    posix_spawn_file_actions_addclosefrom_np(&fa, 3);
+    char* argv[] = {const_cast<char*>(path.c_str()), nullptr};
+    pid_t pid = -1;
+    const int rc = posix_spawn(&pid, path.c_str(), &fa, nullptr, argv, environ);
+    posix_spawn_file_actions_destroy(&fa);
+    close(in[0]);
+    close(out[1]);
+    if (rc != 0) {
+        close(in[1]);
+        close(out[0]);
+        return false;
+    }
+    // A worker that exits early must not raise SIGPIPE in this process: block
+    // it in this thread and take any pending one before unblocking.
+    sigset_t pipe_set, old;
+    sigemptyset(&pipe_set);
+    sigaddset(&pipe_set, SIGPIPE);
+    pthread_sigmask(SIG_BLOCK, &pipe_set, &old);
+    bool ok = true;
+    for (size_t done = 0; ok && done < body.size();) {
+        const ssize_t n = write(in[1], body.data() + done, body.size() - done);
+        if (n > 0) done += static_cast<size_t>(n);
+        else if (n < 0 && errno == EINTR) continue;
+        else ok = false;
+    }
+    close(in[1]);
+    char buf[65536];
+    for (;;) {
+        const ssize_t n = read(out[0], buf, sizeof buf);
+        if (n > 0) code.insert(code.end(), buf, buf + n);
+        else if (n < 0 && errno == EINTR) continue;
+        else break;
+    }
+    close(out[0]);
+    int status = 0;
+    while (waitpid(pid, &status, 0) < 0 && errno == EINTR) {
+    }
+    const timespec zero{0, 0};
+    while (sigtimedwait(&pipe_set, nullptr, &zero) > 0) {
+    }
+    pthread_sigmask(SIG_SETMASK, &old, nullptr);
+    ok = ok && WIFEXITED(status) && WEXITSTATUS(status) == 0 && code.size() > 4 && std::memcmp(code.data(), "\x7f" "ELF", 4) == 0;
+    if (!ok) code.clear();
+    return ok;
 }
 
 }  // namespace
@@ -592,7 +666,7 @@ std::vector<hipFunction_t> spec_kernels(const std::vector<std::string>& sources)
             pool.emplace_back([&] {
                 for (size_t i; (i = next.fetch_add(1)) < todo.size();) {
                     try {
-                        code[i] = compile(todo[i]);
+                        if (todo.size() < 2 || !compile_in_worker(todo[i], code[i])) code[i] = compile(todo[i]);
                     } catch (const std::exception& e) {
                         err[i] = e.what();
                     }

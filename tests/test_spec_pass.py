@@ -166,3 +166,24 @@ def test_benchmark_passes_fit_four_waves_per_simd(tmp_path):
         vgpr = int(notes.split(".vgpr_count:")[1].split()[0])
         spill = int(notes.split(".vgpr_spill_count:")[1].split()[0])
         assert vgpr <= 128 and spill == 0, (step, vgpr, spill)
+
+
+def test_compile_worker_builds_the_pass_code_object(tmp_path):
+    """The compile worker the engine starts (one process per pass, since
+    hipRTC serialises the compiles of one process): a generated pass's text on
+    stdin gives a gfx950 code object holding lqcs_pass on stdout; text that
+    does not compile gives exit status 1 and the compiler's log."""
+    worker = os.path.join(ROOT, "litequantumcomputingsimulator_amd", "lib", "lqcs_rtc_worker")
+    if not os.path.exists(worker):
+        pytest.skip("compile worker not built")
+    circ = _u3cx(14, 2, 5)
+    text, _ = _passes(circ, 14)[0]
+    r = subprocess.run([worker], input=text.encode(), capture_output=True, timeout=600)
+    assert r.returncode == 0, r.stderr.decode()[:2000]
+    assert r.stdout[:4] == b"\x7fELF"
+    co = tmp_path / "pass.co"
+    co.write_bytes(r.stdout)
+    syms = subprocess.run([READELF, "--syms", str(co)], check=True, capture_output=True, text=True).stdout
+    assert "lqcs_pass" in syms
+    bad = subprocess.run([worker], input=b"this is not a kernel", capture_output=True, timeout=600)
+    assert bad.returncode == 1 and b"error" in bad.stderr
