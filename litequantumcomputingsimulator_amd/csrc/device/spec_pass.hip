@@ -40,6 +40,7 @@
 
 #include <dlfcn.h>
 #include <fcntl.h>
+#include <poll.h>
 #include <pthread.h>
 #include <spawn.h>
 #include <sys/wait.h>
@@ -621,8 +622,18 @@ bool compile_in_worker(const std::string& body, std::vector<char>& code) {
     sigemptyset(&pipe_set);
     sigaddset(&pipe_set, SIGPIPE);
     pthread_sigmask(SIG_BLOCK, &pipe_set, &old);
+    // A worker that takes or sends nothing for 10 minutes is stopped (the pass
+    // then compiles in process); a compile takes about a second.
+    constexpr int kQuietMs = 600 * 1000;
     bool ok = true;
     for (size_t done = 0; ok && done < body.size();) {
+        pollfd pf{in[1], POLLOUT, 0};
+        const int ready = poll(&pf, 1, kQuietMs);
+        if (ready < 0 && errno == EINTR) continue;
+        if (ready <= 0) {
+            ok = false;
+            break;
+        }
         const ssize_t n = write(in[1], body.data() + done, body.size() - done);
         if (n > 0) done += static_cast<size_t>(n);
         else if (n < 0 && errno == EINTR) continue;
@@ -631,6 +642,17 @@ bool compile_in_worker(const std::string& body, std::vector<char>& code) {
     close(in[1]);
     char buf[65536];
     for (;;) {
+        if (!ok) {
+            kill(pid, SIGKILL);
+            break;
+        }
+        pollfd pf{out[0], POLLIN, 0};
+        const int ready = poll(&pf, 1, kQuietMs);
+        if (ready < 0 && errno == EINTR) continue;
+        if (ready <= 0) {
+            ok = false;
+            continue;
+        }
         const ssize_t n = read(out[0], buf, sizeof buf);
         if (n > 0) code.insert(code.end(), buf, buf + n);
         else if (n < 0 && errno == EINTR) continue;
