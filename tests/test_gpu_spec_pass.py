@@ -108,6 +108,28 @@ def test_specialized_passes_on_a_4gib_register(lq):
     assert abs(a.norm() - 1.0) <= 1e-10
 
 
+def test_benchmark_circuit_at_full_size_agrees_across_kernels_and_plans(lq):
+    """The 30-qubit benchmark circuit (20 U3/CX layers, seed 1234, 16 GiB per
+    register) at full size, where the oracle does not reach: the bench's plan
+    on specialized passes, the same plan on the interpreting kernel, and the
+    learned plan on specialized passes give the same state within 1e-12, of
+    norm 1."""
+    n = 30
+    print(f"specialized passes: benchmark circuit n={n}", flush=True)
+    qc = lq.QuantumCircuit(n)
+    W.random_u3_cx_layers(qc, n, 20, seed=1234)
+    comp = lq.CompiledCircuit(qc, True)
+    a = lq.statevector(n)
+    comp.apply(a, plan="greedy", tile_qubits=12, low_qubits=4, tile_kernel="specialized")
+    b = lq.statevector(n)
+    comp.apply(b, plan="greedy", tile_qubits=12, low_qubits=4, tile_kernel="register")
+    assert a.max_abs_diff(b) <= 1e-12
+    b.set_basis(0)
+    comp.apply(b, plan="learned", tile_kernel="specialized")
+    assert a.max_abs_diff(b) <= 1e-12
+    assert abs(a.norm() - 1.0) <= 1e-10
+
+
 def test_13_qubit_tiles_on_a_4gib_register(lq):
     """28 qubits on 13-qubit tiles (512 threads, 64 KiB of LDS per tile) with
     five contiguous low qubits, against the interpreting kernel on 12-qubit
