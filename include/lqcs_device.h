@@ -105,8 +105,8 @@ enum {
     LQCS_PLAN_GREEDY = 1,        /* tiled passes, first-fit grouping in program order */
     LQCS_PLAN_LEARNED = 2,       /* the installed cost model's pick among GREEDY and GAIN
                                     plans with 2..4 low qubits (specialized passes: 3..5 low
-                                    qubits on 12- and 13-qubit tiles, and DENSE blocks of 4..6
-                                    qubits); GREEDY when none is installed */
+                                    qubits on 12- and 13-qubit tiles) and, from 12 qubits, DENSE
+                                    blocks of 4..6 qubits; GREEDY when none is installed */
     LQCS_PLAN_AUTO = 3,          /* engine default (GREEDY) */
     LQCS_PLAN_GAIN = 4,          /* tiled passes, tile sets grown by marginal gain */
     LQCS_PLAN_DENSE = 5          /* ops grouped into blocks of tile_qubits qubits (1..6; default

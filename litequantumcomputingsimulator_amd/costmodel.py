@@ -3,8 +3,8 @@
 The engine's LEARNED plan asks a small network how long each pass of a
 candidate plan will take. The candidates are greedy and gain tile grouping at
 2, 3 and 4 low qubits on the interpreting register tile, or at 3, 4 and 5 low
-qubits on 12- and 13-qubit tiles plus dense blocks of 4, 5 and 6 qubits for
-specialized passes (a model per kernel).
+qubits on 12- and 13-qubit tiles for specialized passes (a model per kernel),
+and from 12 qubits dense blocks of 4, 5 and 6 qubits.
 It then runs the candidate with the smallest predicted total. The network is a multilayer perceptron COST_FEATURES -> 64 -> 64 -> 1
 (ReLU, FP64) on per-pass features: register phases, LDS transposes, op counts
 by kind, lane spread of the load layout, tile qubit positions, register width
@@ -120,7 +120,8 @@ def random_circuit(lq, n: int, family: str, depth: int, seed: int):
 
 
 FAMILIES = ("u3cx", "u3cxcp", "pauli", "local", "qft")
-CANDIDATES = tuple((plan, low) for plan in ("greedy", "gain") for low in (2, 3, 4))
+CANDIDATES = tuple((plan, low) for plan in ("greedy", "gain") for low in (2, 3, 4)) + \
+    tuple(("dense", k) for k in (4, 5, 6))  # (plan, low qubits), or ("dense", block width)
 # Specialized passes: (plan, tile width, low qubits), as the engine plans them
 # for registers of 17 qubits and more.
 SPEC_CANDIDATES = tuple((plan, L, low) for L in (12, 13) for plan in ("greedy", "gain") for low in (3, 4, 5))
@@ -193,6 +194,8 @@ def generate(lq, specs, reps: int = 1, log=print, kernel: str = "register", only
                     if L == 12:
                         raise
                     continue  # 13-qubit tiles refuse Pauli rotations wider than 4 qubits; so does the engine's pick
+            elif cand[0] == "dense":
+                f, ms, tot = time_plan(lq, comp, sv, "dense", 0, reps, L=cand[1])
             else:
                 plan, low = cand
                 f, ms, tot = time_plan(lq, comp, sv, plan, low, reps)

@@ -287,11 +287,11 @@ Plan choose_plan(const lqcs_exec_options* opts, const std::vector<DecodedOp>& op
             }
         }
     }
-    // Specialized passes' model is also trained on dense 4..6-qubit blocks
-    // (FP64 MFMA; 12 qubits and up, so that the blocks take the MFMA kernel),
-    // which win on deep circuits over few qubits: candidates too, unless the
-    // call fixes the tile shape.
-    if (g_rtile_spec && !L_req && !low_req && nq >= 12)
+    // Both models are also trained on dense 4..6-qubit blocks (FP64 MFMA; 12
+    // qubits and up, so that the blocks take the MFMA kernel), which win on
+    // deep circuits over few qubits: candidates too, unless the call fixes the
+    // tile shape.
+    if (!L_req && !low_req && nq >= 12)
         for (uint32_t k : {4u, 5u, 6u}) {
             cands.push_back(plan_dense(ops, nq, k));
             meta.push_back(Choice{LQCS_PLAN_DENSE, 0, 0.0, k});

@@ -154,7 +154,8 @@ def test_learned_plan_gives_the_greedy_state(lq):
             lq.costmodel_install(list(CM.init_params(lq, F, 3)), [0.0] * F, [1.0] * F, 0.0, 1.0)
         comp.apply(b, plan="learned")
         choice = lq.last_plan_choice()
-        assert choice["plan"] in ("greedy", "gain") and choice["low_qubits"] in (2, 3, 4)
+        assert (choice["plan"] in ("greedy", "gain") and choice["low_qubits"] in (2, 3, 4)) or \
+            (choice["plan"] == "dense" and choice["tile_qubits"] in (4, 5, 6))
         assert choice["predicted_ms"] > 0
     finally:
         if not installed:
@@ -225,16 +226,19 @@ def test_trained_model_gains_over_greedy(lq):
         sv, ref = lq.statevector(n), lq.statevector(n)
         comp.apply(ref, plan="greedy", tile_qubits=12, low_qubits=4)
         meas, pred = {}, {}
-        for plan, low in CM.CANDIDATES:
+        for cand in CM.CANDIDATES:  # (plan, low qubits), or ("dense", block width)
+            dense = cand[0] == "dense"
+            kw = dict(plan="dense", tile_qubits=cand[1], low_qubits=0) if dense else \
+                dict(plan=cand[0], tile_qubits=12, low_qubits=cand[1])
             times = []
             for _ in range(3):
-                f, ms, t = CM.time_plan(lq, comp, sv, plan, low)
+                f, ms, t = CM.time_plan(lq, comp, sv, kw["plan"], kw["low_qubits"], L=kw["tile_qubits"])
                 times.append(t)
-            meas[(plan, low)] = min(times)
-            pred[(plan, low)] = float(model.predict_ms(lq, f).sum())
+            meas[cand] = min(times)
+            pred[cand] = float(model.predict_ms(lq, f).sum())
             sv.set_basis(0)
-            comp.apply(sv, plan=plan, tile_qubits=12, low_qubits=low)
-            assert sv.max_abs_diff(ref) <= 1e-12
+            comp.apply(sv, **kw)
+            assert sv.max_abs_diff(ref) <= 1e-12, cand
         pick = CM.pick(pred)
         sv.set_basis(0)
         comp.apply(sv, plan="learned")  # the engine makes the same pick (barring a near tie)
@@ -242,7 +246,8 @@ def test_trained_model_gains_over_greedy(lq):
         best = min(pred, key=pred.get)
         margin_tie = abs(pred[best] - (1 - CM.MARGIN) * pred[CM.GREEDY_DEFAULT]) <= 1e-9 * pred[best]
         if ranked[1] - ranked[0] > 1e-9 * ranked[0] and not margin_tie:
-            assert (lq.last_plan_choice()["plan"], lq.last_plan_choice()["low_qubits"]) == pick
+            c = lq.last_plan_choice()
+            assert ((c["plan"], c["tile_qubits"]) if c["plan"] == "dense" else (c["plan"], c["low_qubits"])) == pick
         tot["learned"] += meas[pick]
         tot["greedy"] += meas[("greedy", 4)]
         tot["best"] += min(meas.values())
@@ -297,7 +302,7 @@ def test_learned_plan_matches_oracle_on_planner_workload(lq):
     sv = lq.statevector(n)
     comp.apply(sv, plan="learned")
     choice = lq.last_plan_choice()
-    assert choice["plan"] in ("greedy", "gain")
+    assert choice["plan"] in ("greedy", "gain", "dense")
     assert np.max(np.abs(np.asarray(sv.to_numpy()) - want)) <= 1e-12
 
 
