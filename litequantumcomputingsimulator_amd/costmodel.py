@@ -287,7 +287,7 @@ def default_specs(seed: int = 0, widths=(24, 26, 28, 30)):
 
 def deep_specs(seed: int = 0, widths=(24, 26, 28, 30)):
     """Deep circuits on a window of 5 qubits (one per width), where dense
-    blocks win: part of the specialized passes' training set."""
+    blocks win: part of both models' training sets."""
     rng = np.random.default_rng(seed + 1000)
     return [(n, "deep_local", int(rng.integers(3, 7)), int(rng.integers(1 << 30))) for n in widths]
 
@@ -312,17 +312,13 @@ def main(argv=None) -> int:
     out = args.out or default_model_path(args.kernel)
     lq.costmodel_uninstall(tile_kernel=args.kernel)
     t0 = time.perf_counter()
-    specs = default_specs(args.seed, widths)
-    if args.kernel == "specialized":
-        specs += deep_specs(args.seed, widths)
+    specs = default_specs(args.seed, widths) + deep_specs(args.seed, widths)
     X, T, _ = generate(lq, specs, kernel=args.kernel)
     print(f"training rows: {len(X)} ({time.perf_counter() - t0:.1f} s of timing)", flush=True)
     model, loss = train(lq, X, T, epochs=args.epochs, seed=args.seed)
     print(f"trained: final loss {loss:.4f} (normalised log-ms)", flush=True)
     model.save(out)
-    held_specs = default_specs(args.seed + 1, widths)
-    if args.kernel == "specialized":
-        held_specs += deep_specs(args.seed + 1, widths)
+    held_specs = default_specs(args.seed + 1, widths) + deep_specs(args.seed + 1, widths)
     _, _, held = generate(lq, held_specs, kernel=args.kernel)
     ev = evaluate(lq, model, held)
     tot = {k: sum(e[k] for e in ev) for k in ("learned_ms", "greedy_ms", "best_ms")}
