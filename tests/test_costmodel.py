@@ -108,6 +108,31 @@ def test_plan_features_describe_the_plan(lq):
     assert np.array_equal(g, ln)
 
 
+def test_dense_plan_features_and_pick(lq):
+    """Dense-block candidates as the cost model sees them: every block is a
+    stand-alone 5-target controlled unitary without controls (kind, k,
+    controls), and the pick rule takes a dense candidate only when it is
+    predicted at least MARGIN faster than the greedy default."""
+    from litequantumcomputingsimulator_amd import costmodel as CM
+    from litequantumcomputingsimulator_amd import workloads as W
+
+    n = 16
+    qc = lq.QuantumCircuit(n)
+    W.random_u3_cx_cp(qc, n, 4, seed=6)
+    comp = lq.CompiledCircuit(qc, True)
+    f = comp.features(plan="dense", tile_qubits=5)
+    d = comp.describe(plan="dense", tile_qubits=5)
+    assert f.shape == (len(d), lq.COST_FEATURES)
+    assert np.all(f[:, 0] == 0) and np.all(f[:, 1] == n)
+    assert np.all(f[:, 18] == 5) and np.all(f[:, 19] == 5) and np.all(f[:, 20] == 0)
+    assert [int(x).bit_count() for x in [s["tile_mask"] for s in d]] == [5] * len(d)
+    pred = {c: 100.0 for c in CM.CANDIDATES}
+    pred[("dense", 5)] = 100.0 * (1 - CM.MARGIN) + 1e-6
+    assert CM.pick(pred) == CM.GREEDY_DEFAULT
+    pred[("dense", 5)] = 100.0 * (1 - CM.MARGIN) - 1.0
+    assert CM.pick(pred) == ("dense", 5)
+
+
 @pytest.mark.gpu
 def test_device_network_matches_oracle(lq):
     X, t, p = _data(F=lq.COST_FEATURES, rows=300, seed=5)
