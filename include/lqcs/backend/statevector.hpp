@@ -61,7 +61,9 @@ private:
     mutable aligned_vector<complex_t> host_;
     mutable Fresh fresh_ = Fresh::Device;
     // Set once writable host access was handed out: the mirror may change at
-    // any time after that, so every device use uploads it first.
+    // any time after that, so every device use uploads it first, until device
+    // code changes the register (mark_device_newer), which makes those views
+    // stale.
     bool host_exported_ = false;
 };
 

@@ -145,9 +145,9 @@ lqcs_status lqcs_state_apply(lqcs_dev_state* s, const lqcs_program* prog,
  * address the register s held at capture: lqcs_graph_launch takes the
  * register to replay on and returns LOGIC unless it holds that same device
  * buffer at the same width (a register replaced since, e.g. by assignment,
- * is refused rather than written through a stale address). Stand-alone
- * controlled-unitary and controlled-permutation steps cannot be captured
- * (their tables are staged per call): INVALID_ARGUMENT. */
+ * is refused rather than written through a stale address). The tables of
+ * stand-alone controlled-unitary and controlled-permutation steps (dense
+ * blocks included) live in device memory the graph owns too. */
 typedef struct lqcs_graph lqcs_graph;
 lqcs_status lqcs_graph_create(lqcs_dev_state* s, const lqcs_program* prog, const lqcs_exec_options* opts,
                               lqcs_graph** out);

@@ -6,8 +6,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace lqcs_dev {
 
@@ -98,9 +100,27 @@ void launch_fill_basis(amp_t* s, uint32_t nq, uint64_t index, uint32_t batch = 1
 // kernels that read it).
 void* arena(size_t bytes);
 void arena_reset();
-// Copies a host table into arena memory on the engine stream.
+// Tables of a captured launch sequence (lqcs_graph_create): while set, stage()
+// lays its tables out in memory the graph owns instead of the per-call arena,
+// and keeps their bytes on the host for one copy once the capture ends. With
+// dev null (the sizing capture, whose graph is never launched) it only records
+// sizes and hands out placeholder addresses.
+struct GraphTables {
+    char* dev = nullptr;
+    std::vector<char> host;
+};
+extern thread_local GraphTables* g_graph_tables;
+
+// Copies a host table into arena memory on the engine stream (into the
+// graph's memory during a capture).
 template <class T>
 T* stage(const T* host, size_t count) {
+    if (GraphTables* g = g_graph_tables) {
+        const size_t off = (g->host.size() + 255) & ~size_t{255};
+        g->host.resize(off + count * sizeof(T) + 1);
+        if (count) std::memcpy(g->host.data() + off, host, count * sizeof(T));
+        return reinterpret_cast<T*>(g->dev ? g->dev + off : reinterpret_cast<char*>(uintptr_t{1} << 40) + off);
+    }
     T* dev = static_cast<T*>(arena(count * sizeof(T) + 1));
     if (count) LQCS_HIP(hipMemcpyAsync(dev, host, count * sizeof(T), hipMemcpyHostToDevice, stream()));
     return dev;

@@ -140,6 +140,8 @@ void ensure_ready() {
     LQCS_HIP(hipSetDevice(ctx().device));
 }
 
+thread_local GraphTables* g_graph_tables = nullptr;
+
 void* arena(size_t bytes) {
     Context& c = ctx();
     bytes = (bytes + 255) & ~size_t{255};
@@ -900,7 +902,8 @@ lqcs_status lqcs_state_apply(lqcs_dev_state* s, const lqcs_program* prog, const 
 struct lqcs_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    void* buf = nullptr;  // the tile programs the captured kernels read
+    void* buf = nullptr;     // the tile programs the captured kernels read
+    void* tables = nullptr;  // the stand-alone steps' tables (matrices, offsets, permutations)
     const lqcs_dev::amp_t* d = nullptr;  // the register buffer the kernels address
     uint32_t nq = 0;
 };
@@ -919,11 +922,10 @@ lqcs_status lqcs_graph_create(lqcs_dev_state* s, const lqcs_program* prog, const
         const KernelScope ks(opts);
         uint32_t L = 0;
         const Plan plan = choose_plan(opts, ops, s->nq, 1, L);
+        bool has_tables = false;  // stand-alone steps that stage tables per call
         for (const PlanStep& st : plan.steps)
-            if (!st.tiled && (step_op(plan, ops, st).kind == LQCS_OP_CONTROLLED_UNITARY ||
-                              step_op(plan, ops, st).kind == LQCS_OP_CONTROLLED_PERMUTATION))
-                throw Error(LQCS_ERR_INVALID_ARGUMENT,
-                            "graph capture: a stand-alone controlled-unitary or permutation step stages tables per call");
+            has_tables |= !st.tiled && (step_op(plan, ops, st).kind == LQCS_OP_CONTROLLED_UNITARY ||
+                                        step_op(plan, ops, st).kind == LQCS_OP_CONTROLLED_PERMUTATION);
         const StagedPlan sp = stage_plan(plan, s->nq, L);
         auto pad = [](size_t b) { return (b + 255) & ~size_t{255}; };
         const size_t b_ops = pad(sp.flat.size() * sizeof(TileOp)), b_ph = pad(sp.rph.size() * sizeof(RPhase));
@@ -941,22 +943,50 @@ lqcs_status lqcs_graph_create(lqcs_dev_state* s, const lqcs_program* prog, const
             if (!sp.flat.empty()) LQCS_HIP(hipMemcpy(d_ops, sp.flat.data(), sp.flat.size() * sizeof(TileOp), hipMemcpyHostToDevice));
             if (!sp.rph.empty()) LQCS_HIP(hipMemcpy(d_ph, sp.rph.data(), sp.rph.size() * sizeof(RPhase), hipMemcpyHostToDevice));
             if (!sp.rwords.empty()) LQCS_HIP(hipMemcpy(d_w, sp.rwords.data(), sp.rwords.size() * sizeof(amp_t), hipMemcpyHostToDevice));
-            LQCS_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
-            try {
-                launch_staged(s->d, s->nq, L, plan, ops, 1, sp, sp.flat.empty() ? nullptr : d_ops,
-                              sp.rph.empty() ? nullptr : d_ph, sp.rwords.empty() ? nullptr : d_w, false);
-            } catch (...) {
-                hipGraph_t dead = nullptr;
-                (void)hipStreamEndCapture(stream(), &dead);
-                if (dead) (void)hipGraphDestroy(dead);
-                throw;
+            // One capture of the launch sequence; with `tables`, stage() lays the
+            // stand-alone steps' tables out in graph-owned memory.
+            auto capture = [&](GraphTables* tables, hipGraph_t* out_graph) {
+                struct Scope {
+                    explicit Scope(GraphTables* t) { g_graph_tables = t; }
+                    ~Scope() { g_graph_tables = nullptr; }
+                } scope(tables);
+                LQCS_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+                try {
+                    launch_staged(s->d, s->nq, L, plan, ops, 1, sp, sp.flat.empty() ? nullptr : d_ops,
+                                  sp.rph.empty() ? nullptr : d_ph, sp.rwords.empty() ? nullptr : d_w, false);
+                } catch (...) {
+                    hipGraph_t dead = nullptr;
+                    (void)hipStreamEndCapture(stream(), &dead);
+                    if (dead) (void)hipGraphDestroy(dead);
+                    throw;
+                }
+                LQCS_HIP(hipStreamEndCapture(stream(), out_graph));
+            };
+            if (!has_tables) {
+                capture(nullptr, &g->graph);
+            } else {
+                // A sizing capture (never launched) measures the tables; the
+                // second capture addresses them in the graph's memory, filled once
+                // the capture has ended.
+                GraphTables tables;
+                hipGraph_t sizing = nullptr;
+                capture(&tables, &sizing);
+                if (sizing) LQCS_HIP(hipGraphDestroy(sizing));
+                const size_t bytes = tables.host.size();
+                g->tables = device_alloc(std::max<size_t>(bytes, 256));
+                tables.host.clear();
+                tables.dev = static_cast<char*>(g->tables);
+                capture(&tables, &g->graph);
+                if (tables.host.size() != bytes)
+                    throw Error(LQCS_ERR_LOGIC, "graph capture: the tables changed between the two captures");
+                LQCS_HIP(hipMemcpy(g->tables, tables.host.data(), bytes, hipMemcpyHostToDevice));
             }
-            LQCS_HIP(hipStreamEndCapture(stream(), &g->graph));
             LQCS_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
         } catch (...) {
             if (g->exec) (void)hipGraphExecDestroy(g->exec);
             if (g->graph) (void)hipGraphDestroy(g->graph);
             if (g->buf) device_free(g->buf);
+            if (g->tables) device_free(g->tables);
             delete g;
             throw;
         }
@@ -982,6 +1012,7 @@ lqcs_status lqcs_graph_destroy(lqcs_graph* g) {
         if (g->exec) LQCS_HIP(hipGraphExecDestroy(g->exec));
         if (g->graph) LQCS_HIP(hipGraphDestroy(g->graph));
         if (g->buf) device_free(g->buf);
+        if (g->tables) device_free(g->tables);
         delete g;
     });
 }

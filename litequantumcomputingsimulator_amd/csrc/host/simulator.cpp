@@ -102,7 +102,12 @@ lqcs_dev_state* Statevector::device() const {
     return dev_;
 }
 
-void Statevector::mark_device_newer() { fresh_ = Fresh::Device; }
+void Statevector::mark_device_newer() {
+    // Host views handed out before are stale now: the next host access
+    // downloads into the same buffer (and hands it out again).
+    fresh_ = Fresh::Device;
+    host_exported_ = false;
+}
 
 void Statevector::set_basis(std::size_t index) {
     if (index >= size()) throw std::out_of_range("set_basis: index out of range");

@@ -93,6 +93,62 @@ def test_captured_graph_replays_match_eager_applies(lq, kernel):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("eng", [dict(plan="greedy", tile_qubits=12, low_qubits=4), dict(plan="dense", tile_qubits=5)],
+                         ids=["greedy", "dense"])
+def test_captured_graph_with_table_steps_matches_eager_applies(lq, eng):
+    """Stand-alone steps that stage tables per call (a 3-qubit unitary, a
+    4-controlled X, a 3-qubit permutation, a Toffoli; on the dense plan every
+    block): captured, their tables live in memory the graph owns. Three
+    replays agree with eager applies, and the first with the oracle."""
+    from oracle import oracle as O
+
+    n = 16
+    rng = np.random.default_rng(12)
+    c = O.Circuit(n)
+    W.random_u3_cx_cp(c, n, 3, 12)
+    m = np.linalg.qr(rng.standard_normal((8, 8)) + 1j * rng.standard_normal((8, 8)))[0]
+    c.unitary(m, [0, 9, 15]).mcx([1, 2, 3, 4], 14).permutation([int(v) for v in rng.permutation(8)], [5, 11, 7])
+    c.ccx(6, 13, 8)
+    W.random_u3_cx_cp(c, n, 2, 13)
+    want = O.run_statevector(c, fuse=True)
+    comp = lq.CompiledCircuit(c.to_product(lq), True)
+    a, b = lq.statevector(n), lq.statevector(n)
+    g = lq.CircuitGraph(comp, a, **eng)
+    for k in range(3):
+        g.launch()
+        comp.apply(b, **eng)
+        lq.synchronize()
+        if k == 0:
+            assert np.max(np.abs(np.asarray(a.to_numpy()) - want)) <= 1e-12
+        assert a.max_abs_diff(b) <= 1e-12, k
+    del g
+
+
+@pytest.mark.gpu
+def test_host_view_then_two_applies_compose(lq):
+    """A host view taken before device work is stale afterwards: a register
+    exported with to_numpy() and then applied to twice holds U^2|0>, not U
+    applied twice to the view's old contents; a view taken again shows the
+    device result, and writes through a fresh view reach the device."""
+    n = 14
+    comp = lq.CompiledCircuit(_circuit(lq, n, 3, 21), True)
+    a, b = lq.statevector(n), lq.statevector(n)
+    view = a.to_numpy()
+    comp.apply(a)
+    comp.apply(a)
+    comp.apply(b)
+    comp.apply(b)
+    assert a.max_abs_diff(b) <= 1e-12
+    assert np.max(np.abs(np.asarray(a.to_numpy()) - np.asarray(b.to_numpy()))) <= 1e-12
+    fresh = a.to_numpy()
+    fresh[:] = 0
+    fresh[3] = 1.0
+    b.set_basis(3)
+    assert a.max_abs_diff(b) == 0.0
+    del view
+
+
+@pytest.mark.gpu
 def test_per_call_tile_kernels_agree(lq):
     n = 22
     comp = lq.CompiledCircuit(_circuit(lq, n, 10, 9), True)
