@@ -149,6 +149,28 @@ def test_host_view_then_two_applies_compose(lq):
 
 
 @pytest.mark.gpu
+def test_host_writes_reach_device_reductions(lq):
+    """Writes through a host view reach the device before its next use: a
+    state written on the host gives its exact norm and Pauli expectations;
+    after an apply the view is refreshed, not replayed."""
+    n = 10
+    a = lq.statevector(n)
+    v = a.to_numpy()
+    v[:] = 0
+    v[0] = v[3] = 1 / np.sqrt(2)  # (|0> + |3>)/sqrt2 on qubits 0 and 1: <XX> = 1, <ZZ> = 1, <Z0> = 0
+    assert abs(a.norm() - 1.0) <= 1e-14
+    assert abs(a.expectation_value("I" * (n - 2) + "XX") - 1.0) <= 1e-14
+    assert abs(a.expectation_value("I" * (n - 2) + "ZZ") - 1.0) <= 1e-14
+    assert abs(a.expectation_value("I" * (n - 1) + "Z")) <= 1e-14
+    qc = lq.QuantumCircuit(n)
+    qc.x(0)
+    lq.CompiledCircuit(qc, False).apply(a)  # X on qubit 0: (|1> + |2>)/sqrt2
+    w = np.asarray(a.to_numpy())
+    assert abs(w[1] - 1 / np.sqrt(2)) <= 1e-14 and abs(w[2] - 1 / np.sqrt(2)) <= 1e-14
+    assert abs(w[0]) == 0.0 and abs(w[3]) == 0.0
+
+
+@pytest.mark.gpu
 def test_per_call_tile_kernels_agree(lq):
     n = 22
     comp = lq.CompiledCircuit(_circuit(lq, n, 10, 9), True)
