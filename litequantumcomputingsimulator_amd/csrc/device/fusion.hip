@@ -25,17 +25,19 @@
 namespace lqcs_dev {
 namespace {
 
-// x <- d x on a k-qubit vector; loc[q] is the local bit of global qubit q.
-void apply_small(const DecodedOp& d, const std::vector<int>& loc, uint32_t k, std::vector<amp_t>& x) {
+// M <- d M for the 2^k x 2^k block matrix M (row-major; row = local index);
+// loc[q] is the local bit of global qubit q. Every column at once, so that
+// the op's index tables are built once.
+void apply_small(const DecodedOp& d, const std::vector<int>& loc, uint32_t k, std::vector<amp_t>& M) {
     const uint64_t n = uint64_t{1} << k;
     auto lbit = [&](uint32_t q) { return uint64_t{1} << loc[q]; };
+    auto at = [&](uint64_t row, uint64_t col) -> amp_t& { return M[row * n + col]; };
     uint64_t cm = 0;
     for (uint32_t c : d.controls) cm |= lbit(c);
-    std::vector<uint64_t> off;  // local offset of each target-subspace index (target j <-> bit j)
     const uint32_t m = static_cast<uint32_t>(d.targets.size());
     uint64_t tm = 0;
     for (uint32_t t : d.targets) tm |= lbit(t);
-    off.assign(uint64_t{1} << m, 0);
+    std::vector<uint64_t> off(uint64_t{1} << m, 0);  // local offset of target-subspace index a (target j <-> bit j)
     for (uint64_t a = 0; a < off.size(); ++a)
         for (uint32_t j = 0; j < m; ++j)
             if (a & (uint64_t{1} << j)) off[a] |= lbit(d.targets[j]);
@@ -45,37 +47,42 @@ void apply_small(const DecodedOp& d, const std::vector<int>& loc, uint32_t k, st
         case LQCS_OP_TWO_QUBIT:
         case LQCS_OP_CONTROLLED_UNITARY: {
             const uint64_t dim = off.size();
-            std::vector<amp_t> v(dim), o(dim);
+            std::vector<amp_t> v(dim);
             for (uint64_t i = 0; i < n; ++i) {
                 if ((i & tm) || (i & cm) != cm) continue;
-                for (uint64_t a = 0; a < dim; ++a) v[a] = x[i | off[a]];
-                for (uint64_t r = 0; r < dim; ++r) {
-                    amp_t acc = make_double2(0.0, 0.0);
-                    for (uint64_t a = 0; a < dim; ++a) acc = cadd(acc, cmul(d.m[r * dim + a], v[a]));
-                    o[r] = acc;
+                for (uint64_t col = 0; col < n; ++col) {
+                    for (uint64_t a = 0; a < dim; ++a) v[a] = at(i | off[a], col);
+                    for (uint64_t r = 0; r < dim; ++r) {
+                        amp_t acc = make_double2(0.0, 0.0);
+                        for (uint64_t a = 0; a < dim; ++a) acc = cadd(acc, cmul(d.m[r * dim + a], v[a]));
+                        at(i | off[r], col) = acc;
+                    }
                 }
-                for (uint64_t a = 0; a < dim; ++a) x[i | off[a]] = o[a];
             }
             return;
         }
         case LQCS_OP_DIAGONAL_SINGLE: {
             const uint64_t b = lbit(d.targets[0]);
             for (uint64_t i = 0; i < n; ++i)
-                if ((i & cm) == cm) x[i] = cmul(x[i], (i & b) ? d.m[1] : d.m[0]);
+                if ((i & cm) == cm)
+                    for (uint64_t col = 0; col < n; ++col) at(i, col) = cmul(at(i, col), (i & b) ? d.m[1] : d.m[0]);
             return;
         }
         case LQCS_OP_SWAP: {
             const uint64_t a = lbit(d.targets[0]), b = lbit(d.targets[1]);
             for (uint64_t i = 0; i < n; ++i)
-                if ((i & a) && !(i & b) && (i & cm) == cm) std::swap(x[i], x[(i & ~a) | b]);
+                if ((i & a) && !(i & b) && (i & cm) == cm)
+                    for (uint64_t col = 0; col < n; ++col) std::swap(at(i, col), at((i & ~a) | b, col));
             return;
         }
         case LQCS_OP_CONTROLLED_PERMUTATION: {
             std::vector<amp_t> v(off.size());
             for (uint64_t i = 0; i < n; ++i) {
                 if ((i & tm) || (i & cm) != cm) continue;
-                for (uint64_t a = 0; a < off.size(); ++a) v[a] = x[i | off[a]];
-                for (uint64_t a = 0; a < off.size(); ++a) x[i | off[d.perm[a]]] = v[a];
+                for (uint64_t col = 0; col < n; ++col) {
+                    for (uint64_t a = 0; a < off.size(); ++a) v[a] = at(i | off[a], col);
+                    for (uint64_t a = 0; a < off.size(); ++a) at(i | off[d.perm[a]], col) = v[a];
+                }
             }
             return;
         }
@@ -94,16 +101,22 @@ void apply_small(const DecodedOp& d, const std::vector<int>& loc, uint32_t k, st
             const amp_t y = cmul(make_double2(0.0, -sn), ip);
             auto sgn = [&](uint64_t i) { return (__builtin_popcountll(i & zl) & 1) ? -1.0 : 1.0; };
             if (xl == 0) {
-                for (uint64_t i = 0; i < n; ++i) x[i] = cmul(x[i], cadd(make_double2(c, 0.0), cscale(y, sgn(i))));
+                for (uint64_t i = 0; i < n; ++i) {
+                    const amp_t f = cadd(make_double2(c, 0.0), cscale(y, sgn(i)));
+                    for (uint64_t col = 0; col < n; ++col) at(i, col) = cmul(at(i, col), f);
+                }
                 return;
             }
             const uint64_t lowx = xl & (~xl + 1);
             for (uint64_t i = 0; i < n; ++i) {
                 if (i & lowx) continue;
                 const uint64_t j = i ^ xl;
-                const amp_t ai = x[i], aj = x[j];
-                x[i] = cadd(cscale(ai, c), cmul(cscale(y, sgn(j)), aj));
-                x[j] = cadd(cmul(cscale(y, sgn(i)), ai), cscale(aj, c));
+                const amp_t yi = cscale(y, sgn(i)), yj = cscale(y, sgn(j));
+                for (uint64_t col = 0; col < n; ++col) {
+                    const amp_t ai = at(i, col), aj = at(j, col);
+                    at(i, col) = cadd(cscale(ai, c), cmul(yj, aj));
+                    at(j, col) = cadd(cmul(yi, ai), cscale(aj, c));
+                }
             }
             return;
         }
@@ -119,13 +132,9 @@ std::vector<amp_t> block_matrix(const std::vector<DecodedOp>& ops, const std::ve
     const uint64_t n = uint64_t{1} << k;
     std::vector<int> loc(64, -1);
     for (uint32_t j = 0; j < k; ++j) loc[S[j]] = static_cast<int>(j);
-    std::vector<amp_t> M(n * n);
-    std::vector<amp_t> col(n);
-    for (uint64_t c = 0; c < n; ++c) {
-        for (uint64_t r = 0; r < n; ++r) col[r] = make_double2(r == c ? 1.0 : 0.0, 0.0);
-        for (uint32_t idx : members) apply_small(ops[idx], loc, k, col);
-        for (uint64_t r = 0; r < n; ++r) M[r * n + c] = col[r];
-    }
+    std::vector<amp_t> M(n * n, make_double2(0.0, 0.0));
+    for (uint64_t r = 0; r < n; ++r) M[r * n + r] = make_double2(1.0, 0.0);
+    for (uint32_t idx : members) apply_small(ops[idx], loc, k, M);
     return M;
 }
 
